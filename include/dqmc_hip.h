@@ -1,0 +1,175 @@
+/*
+ * dqmc_hip.h -- C ABI of the MI355X DQMC engine (libdqmc_hip.so).
+ *
+ * The engine replaces the hot path of katherineding/dqmc-dev behind its own
+ * seams:
+ *   - the sweep engine `static int dqmc(struct sim_data *)`  (src/dqmc.c:84-511)
+ *   - the kernels it calls: mul_seq / calc_eq_g / calc_ue_g  (src/greens.c:8-494),
+ *     update_delayed (src/updates.c:7-84), measure_eqlt / measure_uneqlt
+ *     (src/meas.c:66-1383), the RNG (src/rand.h:6-32)
+ *   - the in-memory data it works on: struct params / state / meas_eqlt /
+ *     meas_uneqlt (src/data.h:7-105), flattened here into plain pointers.
+ * The reference has no plugin API; its FFI-equivalent seam is the C call
+ * `dqmc(sim)` made by dqmc_wrapper (src/dqmc.c:621).  dqmc_engine_run() is the
+ * drop-in for that call, dqmc_engine_{set,get}_* carry struct sim_data across
+ * the boundary, and the host-side CLI (dqmc_1 / dqmc_stack) keeps the file
+ * seam (the .h5 layout of src/data.c:345-789).
+ *
+ * Conventions
+ *   - every matrix is column-major, element (i,j) at i + N*j, exactly the raw
+ *     buffer the reference reads from the .h5 file (a numpy C-order (N,N) array
+ *     A therefore appears as A^T, as in the reference);
+ *   - `num` arrays are double (real build) or interleaved (re,im) double pairs
+ *     (complex build, the reference's USE_CPLX); the choice is per engine at
+ *     run time via dqmc_params.is_complex;
+ *   - callers own every host buffer; the engine owns device memory;
+ *   - return codes: 0 ok, DQMC_INTERRUPTED when the stop flag was seen,
+ *     negative on error (dqmc_last_error() gives the message).
+ *   - one HIP stream per engine; an engine handle is not thread-safe.
+ */
+#ifndef DQMC_HIP_H
+#define DQMC_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DQMC_OK 0
+#define DQMC_INTERRUPTED 5 /* same value as src/dqmc.c:73 */
+#define DQMC_ERR_ARG (-10)
+#define DQMC_ERR_HIP (-11)
+#define DQMC_ERR_NOMEM (-12)
+#define DQMC_ERR_UNSUPPORTED (-13)
+
+/* struct params (src/data.h:7-39), flattened.  Pointers refer to host memory
+ * laid out exactly as the .h5 datasets of the same name. */
+typedef struct dqmc_params {
+    int N, L, Nx, Ny, F;
+    int n_matmul, n_delay;
+    int n_sweep_warm, n_sweep_meas, n_sweep;
+    int period_eqlt, period_uneqlt;
+    int checkpoint_every;
+    int meas_bond_corr, meas_thermal, meas_2bond_corr, meas_energy_corr;
+    int meas_nematic_corr, meas_pair_bb_only, meas_chiral, meas_local_JQ;
+    int meas_gen_suscept;
+    int num_i, num_ij, num_plaq_accum, num_b_accum, num_b2_accum;
+    int num_plaq, num_b, num_b2, num_bs, num_bb, num_bb2, num_b2b, num_b2b2;
+    int degen_i, degen_ij, degen_plaq, degen_b, degen_b2, degen_bs, degen_bb;
+    int degen_bb2, degen_b2b, degen_b2b2;
+    int is_complex;
+    const int32_t *map_i, *map_ij, *bonds, *bond2s, *map_bs, *map_bb;
+    const int32_t *map_b2b, *map_bb2, *map_b2b2, *plaqs, *map_plaq, *map_b, *map_b2;
+    const double *peierlsu, *peierlsd, *pp_u, *pp_d, *ppr_u, *ppr_d; /* num */
+    const double *exp_Ku, *exp_Kd, *inv_exp_Ku, *inv_exp_Kd;         /* num N*N */
+    const double *exp_halfKu, *exp_halfKd, *inv_exp_halfKu, *inv_exp_halfKd;
+    const double *exp_lambda, *del; /* double 2*N */
+} dqmc_params;
+
+/* Equal-time accumulators (struct meas_eqlt, src/data.h:46-64). */
+enum dqmc_eq_acc {
+    DQ_EQ_density, DQ_EQ_density_u, DQ_EQ_density_d, DQ_EQ_double_occ,
+    DQ_EQ_g00, DQ_EQ_g00_u, DQ_EQ_g00_d, DQ_EQ_nn, DQ_EQ_xx, DQ_EQ_zz, DQ_EQ_pair_sw,
+    DQ_EQ_uuuu, DQ_EQ_dddd, DQ_EQ_uudd, DQ_EQ_dduu, DQ_EQ_chi,
+    DQ_EQ_kk, DQ_EQ_kv, DQ_EQ_kn, DQ_EQ_vv, DQ_EQ_vn,
+    DQ_EQ_j, DQ_EQ_j2, DQ_EQ_jn,
+    DQ_EQ_COUNT
+};
+
+/* Unequal-time accumulators (struct meas_uneqlt, src/data.h:66-91). */
+enum dqmc_ue_acc {
+    DQ_UE_gt0, DQ_UE_gt0_u, DQ_UE_gt0_d, DQ_UE_nn, DQ_UE_xx, DQ_UE_zz, DQ_UE_pair_sw,
+    DQ_UE_uuuu, DQ_UE_dddd, DQ_UE_uudd, DQ_UE_dduu,
+    DQ_UE_pair_bb, DQ_UE_jj, DQ_UE_jsjs, DQ_UE_kk, DQ_UE_ksks,
+    DQ_UE_pair_b2b2, DQ_UE_j2jn, DQ_UE_jnj2, DQ_UE_jnjn, DQ_UE_j2j, DQ_UE_jj2,
+    DQ_UE_j2j2, DQ_UE_js2js2, DQ_UE_k2k2, DQ_UE_ks2ks2, DQ_UE_jnj, DQ_UE_jjn,
+    DQ_UE_kv, DQ_UE_kn, DQ_UE_vv, DQ_UE_vn, DQ_UE_nem_nnnn, DQ_UE_nem_ssss,
+    DQ_UE_COUNT
+};
+
+#define DQMC_MAX_ACC 40
+
+/* One accumulator group.  acc[k] points to a `num` array of the size the
+ * reference allocates for it (src/data.c:441-517), or NULL when the group is
+ * switched off by the meas_* toggles. */
+typedef struct dqmc_meas {
+    int n_sample;
+    double sign[2]; /* num: (re, im); im is 0 in real builds */
+    double *acc[DQMC_MAX_ACC];
+} dqmc_meas;
+
+/* Length (in num elements) of accumulator `k` of the equal-time (ue=0) or
+ * unequal-time (ue=1) group, or 0 when the parameters switch it off. */
+long dqmc_acc_len(const dqmc_params *p, int ue, int k);
+
+/* ---------------------------------------------------------------------- */
+/* Engine: a batch of independent Markov chains of one shape (N, L, F,
+ * n_matmul, dtype and measurement set equal; all physical parameters may
+ * differ per chain).  Chains with other shapes go into other engines.        */
+typedef struct dqmc_engine dqmc_engine;
+
+/* Tuning knobs picked by the autotuner; zero fields mean "engine default". */
+typedef struct dqmc_tuning {
+    int delay_rank; /* rank of the delayed Metropolis update (register slots) */
+    int flags;      /* reserved */
+} dqmc_tuning;
+
+/* Create an engine for n_chains chains on HIP device `device`.  params[c]
+ * describes chain c (its arrays are copied to the device). */
+int dqmc_engine_create(dqmc_engine **out, const dqmc_params *const *params, int n_chains,
+                       int device, const dqmc_tuning *tune);
+void dqmc_engine_destroy(dqmc_engine *e);
+
+/* struct state (src/data.h:41-45): rng[17], sweep, hs[L*N] (0/1).  Setting the
+ * state marks G stale: the next run recomputes it from scratch as dqmc()
+ * does on entry (src/dqmc.c:188-220). */
+int dqmc_engine_set_state(dqmc_engine *e, int chain, const uint64_t *rng, int sweep,
+                          const int32_t *hs);
+int dqmc_engine_get_state(dqmc_engine *e, int chain, uint64_t *rng, int *sweep, int32_t *hs);
+
+/* Accumulators: copy host -> device (set) or device -> host (get). ue may be
+ * NULL when period_uneqlt == 0. */
+int dqmc_engine_set_meas(dqmc_engine *e, int chain, const dqmc_meas *eq, const dqmc_meas *ue);
+int dqmc_engine_get_meas(dqmc_engine *e, int chain, dqmc_meas *eq, dqmc_meas *ue);
+
+/* Advance every chain by up to n_sweeps sweeps (a chain stops at its
+ * n_sweep).  The loop body is that of src/dqmc.c:222-459 without the
+ * checkpoint, which the host performs between calls.  *stop_flag (may be NULL)
+ * is polled before each sweep as sig_check_state is (src/dqmc.c:224-230);
+ * when it is non-zero the call returns DQMC_INTERRUPTED. */
+int dqmc_engine_run(dqmc_engine *e, int n_sweeps, const volatile int *stop_flag);
+
+/* Oracle hooks (read-only):
+ *  - the engine's current equal-time Green's function of one spin and the
+ *    current phase of the weight;
+ *  - G computed from scratch for the current field at slice boundary l by the
+ *    stabilized product (calc_eq_g((l/n_matmul)%F ...), src/greens.c:69-208)
+ *    and the phase of det(I + B..B) it returns;
+ *  - the unequal-time blocks G(0,t), G(t,t), G(t,0) for all t
+ *    (calc_ue_g, src/greens.c:471-494), before the half-wrap. */
+int dqmc_engine_green(dqmc_engine *e, int chain, int spin, double *G, double *phase);
+int dqmc_engine_calc_eq_g(dqmc_engine *e, int chain, int spin, int f, double *G,
+                          double *phase);
+int dqmc_engine_calc_ue_g(dqmc_engine *e, int chain, int spin, double *G0t, double *Gtt,
+                          double *Gt0);
+
+/* Accepted flips so far, summed over chains (for the flop model). */
+int64_t dqmc_engine_accepts(dqmc_engine *e);
+
+/* Per-kernel-class timing of the last run (ms); names are the reference's
+ * profile sections (src/prof.h:8-22).  Returns the number of entries. */
+int dqmc_engine_profile(dqmc_engine *e, const char **names, double *ms, int max);
+
+/* Average duration (ms) of the dominant GEMM kernel in the last run, measured
+ * with HIP events on the engine stream, and its launch count. */
+int dqmc_engine_gemm_stats(dqmc_engine *e, double *avg_ms, int64_t *launches,
+                           double *flops_per_launch);
+
+const char *dqmc_last_error(void);
+const char *dqmc_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
