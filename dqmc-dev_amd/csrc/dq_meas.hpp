@@ -1,0 +1,197 @@
+// dq_meas.hpp -- equal- and unequal-time measurement contractions
+// (src/meas.c:66-120 measure_eqlt, src/meas.c:728-801 measure_uneqlt).
+//
+// Deterministic scatter-free reductions: the host builds, per chain, the
+// inverse of map_ij / map_i (bin -> list of linear indices i + N*j in
+// increasing order).  One thread owns one accumulator bin and adds the terms
+// of its list in that order, which is the order in which the reference's
+// (j outer, i inner) loops add them, so no atomics are needed and results do
+// not depend on scheduling.  Accumulators stay resident in HBM for the whole
+// run and are copied to the host only when the caller asks for them.
+#pragma once
+#include "dq_common.hpp"
+
+template <typename T>
+struct MeasArgs {
+    int N, L, num_i, num_ij, num_bb, num_b;
+    int energy, bond;
+    const T* G; long sG;              // half-wrapped equal-time G[b] (eq) / G(t,t) etc (ue)
+    const T* G0t; const T* Gtt; const T* Gt0; long sU; // ue arrays [b][L*N*N]
+    const T* phase;                   // [c]
+    const int* flag;                  // [c]: measure this chain now
+    const int* rp_i; const int* ix_i; long srp_i, six_i;     // CSR of map_i  [c]
+    const int* rp_ij; const int* ix_ij; long srp_ij, six_ij; // CSR of map_ij [c]
+    const int* degen;                 // [c][2]: degen_i, degen_ij ; [c][3] degen_bb
+    const T* peierlsu; const T* peierlsd; long sP; // [c][N*N] (complex only)
+    T* acc; long sacc;                // accumulator block of chain c
+    const long* off;                  // offsets of each accumulator in the block (-1: off)
+    int* n_sample; T* sign;           // [c]
+    const int* bonds; long sbonds;    // [c][2*num_b]
+    const int* rp_bb; const int* ix_bb; long srp_bb, six_bb;  // CSR of map_bb
+};
+
+// accumulator indices (match enum dqmc_eq_acc / dqmc_ue_acc)
+enum { EQ_density = 0, EQ_density_u, EQ_density_d, EQ_double_occ, EQ_g00, EQ_g00_u, EQ_g00_d,
+       EQ_nn, EQ_xx, EQ_zz, EQ_pair_sw, EQ_vv = 19, EQ_vn = 20 };
+enum { UE_gt0 = 0, UE_gt0_u, UE_gt0_d, UE_nn, UE_xx, UE_zz, UE_pair_sw, UE_pair_bb = 11, UE_jj,
+       UE_jsjs, UE_kk, UE_ksks, UE_vv = 30, UE_vn = 31 };
+
+template <typename T>
+__device__ inline T peierls_mul(const T* P, long idx, T g);
+template <>
+__device__ inline double peierls_mul<double>(const double*, long, double g) { return g; }
+template <>
+__device__ inline zd peierls_mul<zd>(const zd* P, long idx, zd g) { return g * P[idx]; }
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_meas_eq(MeasArgs<T> a)
+{
+    const int c = blockIdx.y;
+    if (!a.flag[c]) return;
+    const int N = a.N;
+    const T* gu = a.G + (2 * c) * a.sG;
+    const T* gd = a.G + (2 * c + 1) * a.sG;
+    const T phase = a.phase[c];
+    T* acc = a.acc + c * a.sacc;
+    const long* off = a.off;
+    const int degen_i = a.degen[3 * c], degen_ij = a.degen[3 * c + 1];
+    const T* pu = a.peierlsu + c * a.sP;
+    const T* pd = a.peierlsd + c * a.sP;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t == 0) {
+        a.n_sample[c] += 1;
+        a.sign[c] += phase;
+    }
+    // one-site bins
+    if (t < a.num_i) {
+        const int* rp = a.rp_i + c * a.srp_i;
+        const int* ix = a.ix_i + c * a.six_i;
+        const T pre = phase * (1.0 / degen_i);
+        T s0 = acc[off[EQ_density] + t], s1 = acc[off[EQ_density_u] + t];
+        T s2 = acc[off[EQ_density_d] + t], s3 = acc[off[EQ_double_occ] + t];
+        for (int q = rp[t]; q < rp[t + 1]; q++) {
+            const int i = ix[q];
+            const T guii = gu[i + (long)N * i], gdii = gd[i + (long)N * i];
+            const T one = from_re<T>(1.0);
+            s0 += pre * (from_re<T>(2.0) - guii - gdii);
+            s1 += pre * (one - guii);
+            s2 += pre * (one - gdii);
+            s3 += pre * (one - guii) * (one - gdii);
+        }
+        acc[off[EQ_density] + t] = s0;
+        acc[off[EQ_density_u] + t] = s1;
+        acc[off[EQ_density_d] + t] = s2;
+        acc[off[EQ_double_occ] + t] = s3;
+    }
+    if (t < a.num_ij) {
+        const int* rp = a.rp_ij + c * a.srp_ij;
+        const int* ix = a.ix_ij + c * a.six_ij;
+        const T pre = phase * (1.0 / degen_ij);
+        T g00 = acc[off[EQ_g00] + t], g00u = acc[off[EQ_g00_u] + t], g00d = acc[off[EQ_g00_d] + t];
+        T nn = acc[off[EQ_nn] + t], xx = acc[off[EQ_xx] + t], zz = acc[off[EQ_zz] + t];
+        T psw = acc[off[EQ_pair_sw] + t];
+        T vv = from_re<T>(0.0), vn = from_re<T>(0.0);
+        if (a.energy) { vv = acc[off[EQ_vv] + t]; vn = acc[off[EQ_vn] + t]; }
+        const T one = from_re<T>(1.0), two = from_re<T>(2.0);
+        for (int q = rp[t]; q < rp[t + 1]; q++) {
+            const int lin = ix[q];
+            const int i = lin % N, j = lin / N;
+            const double delta = (i == j) ? 1.0 : 0.0;
+            const T guii = gu[i + (long)N * i], gdii = gd[i + (long)N * i];
+            const T guij = gu[i + (long)N * j], gdij = gd[i + (long)N * j];
+            const T guji = gu[j + (long)N * i], gdji = gd[j + (long)N * i];
+            const T gujj = gu[j + (long)N * j], gdjj = gd[j + (long)N * j];
+            const T pgu = peierls_mul(pu, j + (long)N * i, guij);
+            const T pgd = peierls_mul(pd, j + (long)N * i, gdij);
+            g00 += 0.5 * (pre * (pgu + pgd));
+            g00u += pre * pgu;
+            g00d += pre * pgd;
+            const T x = delta * (guii + gdii) - (guji * guij + gdji * gdij);
+            nn += pre * ((two - guii - gdii) * (two - gujj - gdjj) + x);
+            xx += 0.25 * (pre * (delta * (guii + gdii) - (guji * gdij + gdji * guij)));
+            zz += 0.25 * (pre * ((gdii - guii) * (gdjj - gujj) + x));
+            psw += pre * guij * gdij;
+            if (a.energy) {
+                const T nuinuj = (one - guii) * (one - gujj) + (delta - guji) * guij;
+                const T ndindj = (one - gdii) * (one - gdjj) + (delta - gdji) * gdij;
+                vv += pre * nuinuj * ndindj;
+                vn += pre * (nuinuj * (one - gdii) + (one - guii) * ndindj);
+            }
+        }
+        acc[off[EQ_g00] + t] = g00; acc[off[EQ_g00_u] + t] = g00u; acc[off[EQ_g00_d] + t] = g00d;
+        acc[off[EQ_nn] + t] = nn; acc[off[EQ_xx] + t] = xx; acc[off[EQ_zz] + t] = zz;
+        acc[off[EQ_pair_sw] + t] = psw;
+        if (a.energy) { acc[off[EQ_vv] + t] = vv; acc[off[EQ_vn] + t] = vn; }
+    }
+}
+
+// unequal time, default set + vv/vn (src/meas.c:749-801): one thread per (bin, t)
+template <typename T>
+__global__ __launch_bounds__(256) void k_meas_ue(MeasArgs<T> a)
+{
+    const int c = blockIdx.z;
+    if (!a.flag[c]) return;
+    const int N = a.N;
+    const long NN = (long)N * N;
+    const int tt = blockIdx.y;
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    const T phase = a.phase[c];
+    T* acc = a.acc + c * a.sacc;
+    const long* off = a.off;
+    if (r == 0 && tt == 0) {
+        a.n_sample[c] += 1;
+        a.sign[c] += phase;
+    }
+    if (r >= a.num_ij) return;
+    const int degen_ij = a.degen[3 * c + 1];
+    const T* pu = a.peierlsu + c * a.sP;
+    const T* pd = a.peierlsd + c * a.sP;
+    const T* Gu0t = a.G0t + (2 * c) * a.sU + NN * tt;
+    const T* Gutt = a.Gtt + (2 * c) * a.sU + NN * tt;
+    const T* Gut0 = a.Gt0 + (2 * c) * a.sU + NN * tt;
+    const T* Gd0t = a.G0t + (2 * c + 1) * a.sU + NN * tt;
+    const T* Gdtt = a.Gtt + (2 * c + 1) * a.sU + NN * tt;
+    const T* Gdt0 = a.Gt0 + (2 * c + 1) * a.sU + NN * tt;
+    const T* Gu00 = a.Gtt + (2 * c) * a.sU;
+    const T* Gd00 = a.Gtt + (2 * c + 1) * a.sU;
+    const int* rp = a.rp_ij + c * a.srp_ij;
+    const int* ix = a.ix_ij + c * a.six_ij;
+    const T pre = phase * (1.0 / degen_ij);
+    const long o = r + (long)a.num_ij * tt;
+    T gt0 = acc[off[UE_gt0] + o], gt0u = acc[off[UE_gt0_u] + o], gt0d = acc[off[UE_gt0_d] + o];
+    T nn = acc[off[UE_nn] + o], xx = acc[off[UE_xx] + o], zz = acc[off[UE_zz] + o];
+    T psw = acc[off[UE_pair_sw] + o];
+    T vv = from_re<T>(0.0), vn = from_re<T>(0.0);
+    if (a.energy) { vv = acc[off[UE_vv] + o]; vn = acc[off[UE_vn] + o]; }
+    const T one = from_re<T>(1.0), two = from_re<T>(2.0);
+    const double delta_t = (tt == 0) ? 1.0 : 0.0;
+    for (int q = rp[r]; q < rp[r + 1]; q++) {
+        const int lin = ix[q];
+        const int i = lin % N, j = lin / N;
+        const double dtij = delta_t * ((i == j) ? 1.0 : 0.0);
+        const T guii = Gutt[i + (long)N * i], guij = Gut0[i + (long)N * j];
+        const T guji = Gu0t[j + (long)N * i], gujj = Gu00[j + (long)N * j];
+        const T gdii = Gdtt[i + (long)N * i], gdij = Gdt0[i + (long)N * j];
+        const T gdji = Gd0t[j + (long)N * i], gdjj = Gd00[j + (long)N * j];
+        const T pgu = peierls_mul(pu, j + (long)N * i, guij);
+        const T pgd = peierls_mul(pd, j + (long)N * i, gdij);
+        gt0 += 0.5 * (pre * (pgu + pgd));
+        gt0u += pre * pgu;
+        gt0d += pre * pgd;
+        const T x = dtij * (guii + gdii) - (guji * guij + gdji * gdij);
+        nn += pre * ((two - guii - gdii) * (two - gujj - gdjj) + x);
+        xx += 0.25 * (pre * (dtij * (guii + gdii) - (guji * gdij + gdji * guij)));
+        zz += 0.25 * (pre * ((gdii - guii) * (gdjj - gujj) + x));
+        psw += pre * guij * gdij;
+        if (a.energy) {
+            const T nuinuj = (one - guii) * (one - gujj) + (dtij - guji) * guij;
+            const T ndindj = (one - gdii) * (one - gdjj) + (dtij - gdji) * gdij;
+            vv += pre * nuinuj * ndindj;
+            vn += pre * (nuinuj * (one - gdii) + (one - guii) * ndindj);
+        }
+    }
+    acc[off[UE_gt0] + o] = gt0; acc[off[UE_gt0_u] + o] = gt0u; acc[off[UE_gt0_d] + o] = gt0d;
+    acc[off[UE_nn] + o] = nn; acc[off[UE_xx] + o] = xx; acc[off[UE_zz] + o] = zz;
+    acc[off[UE_pair_sw] + o] = psw;
+    if (a.energy) { acc[off[UE_vv] + o] = vv; acc[off[UE_vn] + o] = vn; }
+}

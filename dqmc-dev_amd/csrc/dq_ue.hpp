@@ -1,0 +1,155 @@
+// dq_ue.hpp -- helper kernels for the unequal-time Green's functions
+// (calc_o / bsofi / expand_g, src/greens.c:247-469): block copies with
+// leading dimensions, triangle extraction, identity set-up, the triangular
+// solve that inverts R (xtrtri), and row scalings that materialise B_l^-1.
+#pragma once
+#include "dq_common.hpp"
+
+// dst[b] (m x n, ldd) = alpha * src[b] (m x n, lds)
+template <typename T>
+__global__ void k_copy_mn(int m, int n, const T* src, long ssrc, int lds, T alpha, T* dst,
+                          long sdst, int ldd, int batch)
+{
+    const long mn = (long)m * n;
+    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < mn * batch;
+         e += (long)gridDim.x * blockDim.x) {
+        const int b = e / mn;
+        const long r = e % mn;
+        const int i = r % m, j = r / m;
+        dst[b * sdst + i + (long)ldd * j] = alpha * src[b * ssrc + i + (long)lds * j];
+    }
+}
+
+// dst[b] (n x n, ldd) = upper triangle of src[b] (lds), zeros below
+template <typename T>
+__global__ void k_copy_upper(int n, const T* src, long ssrc, int lds, T* dst, long sdst, int ldd,
+                             int batch)
+{
+    const long nn = (long)n * n;
+    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < nn * batch;
+         e += (long)gridDim.x * blockDim.x) {
+        const int b = e / nn;
+        const long r = e % nn;
+        const int i = r % n, j = r / n;
+        dst[b * sdst + i + (long)ldd * j] =
+            (i <= j) ? src[b * ssrc + i + (long)lds * j] : from_re<T>(0.0);
+    }
+}
+
+// upper triangle (incl. diagonal) of src written into dst; dst's strictly
+// lower part is left untouched (xtrtri semantics)
+template <typename T>
+__global__ void k_store_upper(int n, const T* src, long ssrc, int lds, T* dst, long sdst, int ldd,
+                              int batch)
+{
+    const long nn = (long)n * n;
+    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < nn * batch;
+         e += (long)gridDim.x * blockDim.x) {
+        const int b = e / nn;
+        const long r = e % nn;
+        const int i = r % n, j = r / n;
+        if (i <= j) dst[b * sdst + i + (long)ldd * j] = src[b * ssrc + i + (long)lds * j];
+    }
+}
+
+// zero the strictly lower part of an m x n panel (ld)
+template <typename T>
+__global__ void k_zero_lower(int m, int n, T* A, long sA, int lda, int batch)
+{
+    const long mn = (long)m * n;
+    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < mn * batch;
+         e += (long)gridDim.x * blockDim.x) {
+        const int b = e / mn;
+        const long r = e % mn;
+        const int i = r % m, j = r / m;
+        if (i > j) A[b * sA + i + (long)lda * j] = from_re<T>(0.0);
+    }
+}
+
+// A[b] (n x n, lda) = alpha * I + beta_keep * A  (beta_keep 0 or 1)
+template <typename T>
+__global__ void k_set_identity(int n, T* A, long sA, int lda, int keep, int batch)
+{
+    const long nn = (long)n * n;
+    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < nn * batch;
+         e += (long)gridDim.x * blockDim.x) {
+        const int b = e / nn;
+        const long r = e % nn;
+        const int i = r % n, j = r / n;
+        T* p = A + b * sA + i + (long)lda * j;
+        const T d = from_re<T>(i == j ? 1.0 : 0.0);
+        *p = keep ? (*p + d) : d;
+    }
+}
+
+// X[b] = diag(s[b]) * A[b / adiv]  (B_l^-1 = diag(e'_l) inv_expK, src/dqmc.c:45-59)
+template <typename T>
+__global__ void k_rowscale(int N, const T* A, long sA, int adiv, const double* s, long ss, T* X,
+                           long sX, int batch)
+{
+    const long NN = (long)N * N;
+    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < NN * batch;
+         e += (long)gridDim.x * blockDim.x) {
+        const int b = e / NN;
+        const long r = e % NN;
+        const int i = r % N;
+        X[b * sX + r] = s[b * ss + i] * A[(b / adiv) * sA + r];
+    }
+}
+
+// Solve U X = X in place for the n x n upper-triangular U[b] (ldu) and
+// ncols right-hand sides X[b] (ldx): back substitution, TS_NC columns per
+// wave held in registers (RPL rows per lane).  With X = I this is xtrtri.
+#define TS_NC 4
+template <typename T, int RPL>
+__global__ __launch_bounds__(256) void k_trsm_upper(int n, int ncols, const T* U_, long sU, int ldu,
+                                                    T* X_, long sX, int ldx)
+{
+    const int b = blockIdx.y;
+    const T* U = U_ + b * sU;
+    T* X = X_ + b * sX;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int c0 = (blockIdx.x * (blockDim.x >> 6) + wave) * TS_NC;
+    if (c0 >= ncols) return;
+    T x[TS_NC][RPL];
+#pragma unroll
+    for (int q = 0; q < TS_NC; q++)
+#pragma unroll
+        for (int r = 0; r < RPL; r++) {
+            const int i = lane + 64 * r;
+            x[q][r] = (i < n && c0 + q < ncols) ? X[i + (long)ldx * (c0 + q)] : from_re<T>(0.0);
+        }
+    for (int kk = n - 1; kk >= 0; kk--) {
+        const int owner = kk & 63, rk = kk >> 6;
+        const T ukk = U[kk + (long)ldu * kk];
+        T xk[TS_NC];
+#pragma unroll
+        for (int q = 0; q < TS_NC; q++) {
+            T val = from_re<T>(0.0);
+#pragma unroll
+            for (int r = 0; r < RPL; r++)
+                if (r == rk) val = x[q][r];
+            val = rdiv(val, ukk);
+            xk[q] = shfl(val, owner);
+#pragma unroll
+            for (int r = 0; r < RPL; r++)
+                if (r == rk && lane == owner) x[q][r] = xk[q];
+        }
+#pragma unroll
+        for (int r = 0; r < RPL; r++) {
+            const int i = lane + 64 * r;
+            if (i < kk) {
+                const T u = U[i + (long)ldu * kk];
+#pragma unroll
+                for (int q = 0; q < TS_NC; q++) x[q][r] -= u * xk[q];
+            }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < TS_NC; q++)
+#pragma unroll
+        for (int r = 0; r < RPL; r++) {
+            const int i = lane + 64 * r;
+            if (i < n && c0 + q < ncols) X[i + (long)ldx * (c0 + q)] = x[q][r];
+        }
+}

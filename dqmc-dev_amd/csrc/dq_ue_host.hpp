@@ -1,0 +1,346 @@
+// dq_ue_host.hpp -- unequal-time Green's functions on the GPU for every
+// (chain, spin): calc_ue_g = calc_o -> bsofi -> expand_g (src/greens.c:247-494),
+// the half-wrap of all 3L matrices (src/dqmc.c:415-451) and the contraction
+// (src/meas.c:728-801).  Included by engine.hip.
+//
+// Every step is one batched launch over all (chain, spin) matrices.  The
+// block-structured orthogonal factorization (bsofi) keeps the reference's
+// block order: per block column a Householder QR of the 2N x N panel (with the
+// explicit Q kept for the final right-multiplication by Q^H), the 2N x 2N QR
+// of the last panel, the structured inversion of R (triangular solves for the
+// diagonal blocks, GEMMs for the rest) and the right-multiplications by the
+// Q_l^H -- each Q application is a GEMM on FP64 MFMA.
+#pragma once
+#include "dq_ue.hpp"
+
+static long ue_stride(const dqmc_engine* e)
+{
+    const long N = e->N, E = 1 + (e->F - 1) / 2, NE = N * E;
+    long s = NE * NE;
+    if (2 * N * NE > s) s = 2 * N * NE;
+    if (9 * N * N > s) s = 9 * N * N;
+    return s;
+}
+
+static int ue_alloc(dqmc_engine* e)
+{
+    const long N = e->N, E = 1 + (e->F - 1) / 2, NE = N * E, B = 2L * e->nc;
+    const long sU2 = ue_stride(e);
+    const size_t es = e->esz;
+    DALLOC(e->Gred, B * sU2 * es);
+    DALLOC(e->Wa, B * sU2 * es);
+    DALLOC(e->Wb, B * sU2 * es);
+    DALLOC(e->Qb, B * (E > 1 ? E - 1 : 1) * 4 * N * N * es);
+    DALLOC(e->taub, B * NE * es);
+    DALLOC(e->T2, B * (long)e->L * N * N * es);
+    return 0;
+}
+
+template <typename T>
+static int launch_trsm_upper(dqmc_engine* e, int n, int ncols, const T* U, long sU, int ldu, T* X,
+                             long sX, int ldx, int batch)
+{
+    dim3 grid((ncols + 4 * TS_NC - 1) / (4 * TS_NC), batch);
+#define TRS(R) k_trsm_upper<T, R><<<grid, 256, 0, e->st>>>(n, ncols, U, sU, ldu, X, sX, ldx)
+    if (n <= 64) TRS(1);
+    else if (n <= 128) TRS(2);
+    else if (n <= 256) TRS(4);
+    else if (n <= 512) TRS(8);
+    else if (n <= 1024) TRS(16);
+    else return set_err(DQMC_ERR_UNSUPPORTED, "triangular block of order %d > 1024", n);
+#undef TRS
+    LCHK();
+    return 0;
+}
+
+// X (n x n, ld) <- X^-1 in the upper triangle (xtrtri), lower part untouched
+template <typename T>
+static int trtri_upper(dqmc_engine* e, int n, T* X, long sX, int ldx, T* W, long sW, int B)
+{
+    k_set_identity<T><<<nblk((long)n * n * B), 256, 0, e->st>>>(n, W, sW, n, 0, B);
+    LCHK();
+    RC(launch_trsm_upper<T>(e, n, n, X, sX, ldx, W, sW, n, B));
+    k_store_upper<T><<<nblk((long)n * n * B), 256, 0, e->st>>>(n, W, sW, n, X, sX, ldx, B);
+    LCHK();
+    return 0;
+}
+
+// G0t, Gtt, Gt0 <- calc_ue_g for all b (before the half-wrap)
+template <typename T>
+static int ue_calc(dqmc_engine* e)
+{
+    const int N = e->N, L = e->L, F = e->F, B = 2 * e->nc;
+    const int E = 1 + (F - 1) / 2, NE = N * E, NL = NE, N2 = 2 * N;
+    const long NN = (long)N * N, LNN = (long)L * NN, LN = (long)L * N;
+    const long sU2 = ue_stride(e);
+    T* Gred = dp<T>(e->Gred);
+    T* Wa = dp<T>(e->Wa);
+    T* Wb = dp<T>(e->Wb);
+    T* tauB = dp<T>(e->taub);
+    T* Qb = dp<T>(e->Qb);
+    const long sQb = (long)(E > 1 ? E - 1 : 1) * 4 * NN;
+    const T one = from_re<T>(1.0), mone = from_re<T>(-1.0), zero = from_re<T>(0.0);
+    auto BLK = [&](int i, int j) { return Gred + (long)N * i + (long)NL * N * j; };
+
+    // calc_o (src/greens.c:247-264)
+    HIPCHK(hipMemsetAsync(Gred, 0, (size_t)B * sU2 * sizeof(T), e->st));
+    for (int q = 0; q < E - 1; q++)
+        RC(mul_seq_C<T>(e, q * 2, (q + 1) * 2, mone, BLK(q + 1, q), NE, sU2));
+    RC(mul_seq_C<T>(e, (E - 1) * 2, 0, one, BLK(0, E - 1), NE, sU2));
+    k_set_identity<T><<<nblk((long)NE * NE * B), 256, 0, e->st>>>(NE, Gred, sU2, NE, 1, B);
+    LCHK();
+
+    // bsofi (src/greens.c:266-337)
+    if (E == 1) {
+        k_set_identity<T><<<nblk(NN * B), 256, 0, e->st>>>(N, Wa, sU2, N, 0, B);
+        LCHK();
+        k_getrf<T><<<B, 1024, 0, e->st>>>(N, Gred, Wa, sU2, e->ipiv, N);
+        LCHK();
+        const int wpb = 4;
+        dim3 grid((N + wpb * LU_NC - 1) / (wpb * LU_NC), B);
+#define GRS(R)                                                                                 \
+    k_getrs_phase<T, R><<<grid, 256, 0, e->st>>>(N, Gred, Wa, sU2, e->ipiv, N, nullptr, 0,       \
+                                                 nullptr, 0, nullptr, 0, nullptr)
+        if (N <= 64) GRS(1);
+        else if (N <= 128) GRS(2);
+        else if (N <= 256) GRS(4);
+        else if (N <= 512) GRS(8);
+        else GRS(16);
+#undef GRS
+        LCHK();
+        k_copy_mn<T><<<nblk(NN * B), 256, 0, e->st>>>(N, N, Wa, sU2, N, one, Gred, sU2, NE, B);
+        LCHK();
+    } else {
+        for (int l = 0; l < E - 2; l++) {
+            T* Ql = Qb + (long)l * 4 * NN;
+            k_qr<T, false><<<B, QR_THREADS, 0, e->st>>>(N2, N, BLK(l, l), NL, sU2, tauB + (long)N * l,
+                                                        NE, nullptr, 0, Ql, N2, sQb);
+            LCHK();
+            const int cols[2] = {l + 1, E - 1};
+            for (int t = 0; t < 2; t++) {
+                T* P = BLK(l, cols[t]);
+                RC(gemm<T>(e, OP_C, OP_N, N2, N, N2, one, Ql, N2, sQb, P, NL, sU2, zero, Wa, N2,
+                           sU2, B));
+                k_copy_mn<T><<<nblk((long)N2 * N * B), 256, 0, e->st>>>(N2, N, Wa, sU2, N2, one, P,
+                                                                       sU2, NL, B);
+                LCHK();
+            }
+        }
+        T* Qlast = Qb + (long)(E - 2) * 4 * NN;
+        k_qr<T, false><<<B, QR_THREADS, 0, e->st>>>(N2, N2, BLK(E - 2, E - 2), NL, sU2,
+                                                    tauB + (long)N * (E - 2), NE, nullptr, 0, Qlast,
+                                                    N2, sQb);
+        LCHK();
+        // invert R
+        if (E <= 2) {
+            RC(trtri_upper<T>(e, NL, Gred, sU2, NL, Wa, sU2, B));
+        } else {
+            RC(trtri_upper<T>(e, 3 * N, BLK(E - 3, E - 3), sU2, NL, Wa, sU2, B));
+            if (E > 3) {
+                const int m3 = N * (E - 3);
+                k_copy_upper<T><<<nblk(NN * B), 256, 0, e->st>>>(N, BLK(E - 1, E - 1), sU2, NL, Wb,
+                                                                sU2, N, B);
+                LCHK();
+                RC(gemm<T>(e, OP_N, OP_N, m3, N, N, one, BLK(0, E - 1), NL, sU2, Wb, N, sU2, zero,
+                           Wa, m3, sU2, B));
+                k_copy_mn<T><<<nblk((long)m3 * N * B), 256, 0, e->st>>>(m3, N, Wa, sU2, m3, one,
+                                                                       BLK(0, E - 1), sU2, NL, B);
+                LCHK();
+                for (int l = E - 4; l >= 0; l--) {
+                    RC(trtri_upper<T>(e, N, BLK(l, l), sU2, NL, Wa, sU2, B));
+                    k_copy_upper<T><<<nblk(NN * B), 256, 0, e->st>>>(N, BLK(l, l), sU2, NL, Wb,
+                                                                    sU2, N, B);
+                    LCHK();
+                    const int cols[2] = {E - 1, l + 1};
+                    for (int t = 0; t < 2; t++) {
+                        RC(gemm<T>(e, OP_N, OP_N, N, N, N, mone, Wb, N, sU2, BLK(l, cols[t]), NL,
+                                   sU2, zero, Wa, N, sU2, B));
+                        k_copy_mn<T><<<nblk(NN * B), 256, 0, e->st>>>(N, N, Wa, sU2, N, one,
+                                                                     BLK(l, cols[t]), sU2, NL, B);
+                        LCHK();
+                    }
+                    RC(gemm<T>(e, OP_N, OP_N, N, N * (E - l - 2), N, one, BLK(l, l + 1), NL, sU2,
+                               BLK(l + 1, l + 2), NL, sU2, one, BLK(l, l + 2), NL, sU2, B));
+                    k_copy_upper<T><<<nblk(NN * B), 256, 0, e->st>>>(N, BLK(l + 1, l + 1), sU2, NL,
+                                                                    Wb, sU2, N, B);
+                    LCHK();
+                    RC(gemm<T>(e, OP_N, OP_N, N, N, N, one, BLK(l, l + 1), NL, sU2, Wb, N, sU2,
+                               zero, Wa, N, sU2, B));
+                    k_copy_mn<T><<<nblk(NN * B), 256, 0, e->st>>>(N, N, Wa, sU2, N, one,
+                                                                 BLK(l, l + 1), sU2, NL, B);
+                    LCHK();
+                }
+            }
+        }
+        // multiply by Q^H from the right
+        k_zero_lower<T><<<nblk(4 * NN * B), 256, 0, e->st>>>(N2, N2, BLK(E - 2, E - 2), sU2, NL, B);
+        LCHK();
+        RC(gemm<T>(e, OP_N, OP_C, NL, N2, N2, one, BLK(0, E - 2), NL, sU2, Qlast, N2, sQb, zero,
+                   Wa, NL, sU2, B));
+        k_copy_mn<T><<<nblk((long)NL * N2 * B), 256, 0, e->st>>>(NL, N2, Wa, sU2, NL, one,
+                                                                BLK(0, E - 2), sU2, NL, B);
+        LCHK();
+        for (int l = E - 3; l >= 0; l--) {
+            T* Ql = Qb + (long)l * 4 * NN;
+            k_zero_lower<T><<<nblk(2 * NN * B), 256, 0, e->st>>>(N2, N, BLK(l, l), sU2, NL, B);
+            LCHK();
+            RC(gemm<T>(e, OP_N, OP_C, NL, N2, N2, one, BLK(0, l), NL, sU2, Ql, N2, sQb, zero, Wa,
+                       NL, sU2, B));
+            k_copy_mn<T><<<nblk((long)NL * N2 * B), 256, 0, e->st>>>(NL, N2, Wa, sU2, NL, one,
+                                                                    BLK(0, l), sU2, NL, B);
+            LCHK();
+        }
+    }
+
+    // expand_g (src/greens.c:339-469) with nm = (L/F)*2 slices per block
+    T* G0t = dp<T>(e->G0t);
+    T* Gtt = dp<T>(e->Gtt);
+    T* Gt0 = dp<T>(e->Gt0);
+    T* tmp = dp<T>(e->T1);
+    const T* expK = dp<T>(e->expK);
+    const T* iexpK = dp<T>(e->iexpK);
+    const double* es = e->escale;
+    const double* is = e->iscale;
+    const int nm = (L / F) * 2;
+    const int n_left = (nm - 1) / 2, n_right = nm / 2, n_up = n_left, n_down = n_right;
+    const int rstop_last = ((E - 1) * nm + L) / 2;
+    const int lstop_first = (rstop_last + 1) % L;
+    const int dstop_last = rstop_last, ustop_first = lstop_first;
+    auto cpy = [&](const T* src, T* dst) -> int {
+        k_copy_mn<T><<<nblk(NN * B), 256, 0, e->st>>>(N, N, src, sU2, NE, one, dst, LNN, N, B);
+        LCHK();
+        return 0;
+    };
+    for (int f = 0; f < E; f++) RC(cpy(BLK(0, f), G0t + (long)f * nm * NN));
+    for (int f = 0; f < E; f++) {
+        const int l = f * nm;
+        const int lstop = (f == 0) ? lstop_first : l - n_left;
+        const int rstop = (f == E - 1) ? rstop_last : l + n_right;
+        for (int m = l; m != lstop;) {
+            const int next = (m - 1 + L) % L;
+            const T alpha = (m == 0) ? mone : one;
+            RC(gemm<T>(e, OP_N, OP_N, N, N, N, alpha, G0t + m * NN, N, LNN, expK, N, NN, zero,
+                       G0t + next * NN, N, LNN, B, nullptr, 0, nullptr, 0, es + (long)N * next, LN));
+            m = next;
+        }
+        for (int m = l; m != rstop;) {
+            const int next = (m + 1) % L;
+            const T alpha = (next == 0) ? mone : one;
+            const T beta = (m == 0) ? from_re<T>(-re(alpha)) : zero;
+            if (m == 0) {
+                k_rowscale<T><<<nblk(NN * B), 256, 0, e->st>>>(N, iexpK, NN, 1, is + (long)N * m, LN,
+                                                              G0t + next * NN, LNN, B);
+                LCHK();
+            }
+            RC(gemm<T>(e, OP_N, OP_N, N, N, N, alpha, G0t + m * NN, N, LNN, iexpK, N, NN, beta,
+                       G0t + next * NN, N, LNN, B, nullptr, 0, is + (long)N * m, LN));
+            m = next;
+        }
+    }
+    for (int q = 0; q < E; q++) RC(cpy(BLK(q, q), Gtt + (long)q * nm * NN));
+    for (int q = 0; q < E; q++) {
+        const int k = q * nm;
+        const int ustop = (q == 0) ? ustop_first : k - n_up;
+        const int dstop = (q == E - 1) ? dstop_last : k + n_down;
+        for (int m = k; m != ustop;) {
+            const int next = (m - 1 + L) % L;
+            RC(gemm<T>(e, OP_N, OP_N, N, N, N, one, Gtt + m * NN, N, LNN, expK, N, NN, zero, tmp, N,
+                       NN, B, nullptr, 0, nullptr, 0, es + (long)N * next, LN));
+            RC(gemm<T>(e, OP_N, OP_N, N, N, N, one, iexpK, N, NN, tmp, N, NN, zero, Gtt + next * NN,
+                       N, LNN, B, is + (long)N * next, LN));
+            m = next;
+        }
+        for (int m = k; m != dstop;) {
+            const int next = (m + 1) % L;
+            RC(gemm<T>(e, OP_N, OP_N, N, N, N, one, Gtt + m * NN, N, LNN, iexpK, N, NN, zero, tmp,
+                       N, NN, B, nullptr, 0, is + (long)N * m, LN));
+            RC(gemm<T>(e, OP_N, OP_N, N, N, N, one, expK, N, NN, tmp, N, NN, zero, Gtt + next * NN,
+                       N, LNN, B, nullptr, 0, es + (long)N * m, LN));
+            m = next;
+        }
+    }
+    for (int q = 0; q < E; q++) RC(cpy(BLK(q, 0), Gt0 + (long)q * nm * NN));
+    for (int q = 0; q < E; q++) {
+        const int k = q * nm;
+        const int ustop = (q == 0) ? ustop_first : k - n_up;
+        const int dstop = (q == E - 1) ? dstop_last : k + n_down;
+        for (int m = k; m != ustop;) {
+            const int next = (m - 1 + L) % L;
+            const T alpha = (m == 0) ? mone : one;
+            const T beta = (m == 0) ? from_re<T>(-re(alpha)) : zero;
+            if (m == 0) {
+                k_rowscale<T><<<nblk(NN * B), 256, 0, e->st>>>(N, iexpK, NN, 1, is + (long)N * next,
+                                                              LN, Gt0 + next * NN, LNN, B);
+                LCHK();
+            }
+            RC(gemm<T>(e, OP_N, OP_N, N, N, N, alpha, iexpK, N, NN, Gt0 + m * NN, N, LNN, beta,
+                       Gt0 + next * NN, N, LNN, B, is + (long)N * next, LN));
+            m = next;
+        }
+        for (int m = k; m != dstop;) {
+            const int next = (m + 1) % L;
+            const T alpha = (next == 0) ? mone : one;
+            RC(gemm<T>(e, OP_N, OP_N, N, N, N, alpha, expK, N, NN, Gt0 + m * NN, N, LNN, zero,
+                       Gt0 + next * NN, N, LNN, B, nullptr, 0, es + (long)N * m, LN));
+            if (next == 0) {
+                k_set_identity<T><<<nblk(NN * B), 256, 0, e->st>>>(N, Gt0 + next * NN, LNN, N, 1, B);
+                LCHK();
+            }
+            m = next;
+        }
+    }
+    return 0;
+}
+
+// X[b][l] <- e^{dtK/2} X[b][l] e^{-dtK/2} for all b, l (src/dqmc.c:420-431)
+template <typename T>
+static int ue_half_wrap(dqmc_engine* e, T* X)
+{
+    const int N = e->N, L = e->L, B = 2 * e->nc;
+    const long NN = (long)N * N;
+    const T one = from_re<T>(1.0), zero = from_re<T>(0.0);
+    T* T2 = dp<T>(e->T2);
+    e->gemm_bdiv = L;
+    RC(gemm<T>(e, OP_N, OP_N, N, N, N, one, X, N, NN, dp<T>(e->ehalf), N, NN, zero, T2, N, NN,
+               B * L));
+    e->gemm_adiv = L;
+    RC(gemm<T>(e, OP_N, OP_N, N, N, N, one, dp<T>(e->iehalf), N, NN, T2, N, NN, zero, X, N, NN,
+               B * L));
+    return 0;
+}
+
+template <typename T>
+static int ue_measure(dqmc_engine* e)
+{
+    RC(ue_calc<T>(e));
+    RC(ue_half_wrap<T>(e, dp<T>(e->G0t)));
+    RC(ue_half_wrap<T>(e, dp<T>(e->Gtt)));
+    RC(ue_half_wrap<T>(e, dp<T>(e->Gt0)));
+    MeasArgs<T> ma = meas_args<T>(e, 1);
+    ma.G0t = dp<T>(e->G0t);
+    ma.Gtt = dp<T>(e->Gtt);
+    ma.Gt0 = dp<T>(e->Gt0);
+    ma.sU = (long)e->L * e->N * e->N;
+    k_meas_ue<T><<<dim3((e->num_ij + 255) / 256, e->L, e->nc), 256, 0, e->st>>>(ma);
+    LCHK();
+    return 0;
+}
+
+template <typename T>
+static int ue_hook(dqmc_engine* e, int c, int spin, double* G0t, double* Gtt, double* Gt0)
+{
+    const int N = e->N;
+    k_scales<<<nblk((long)e->nc * e->L * N), 256, 0, e->st>>>(N, e->L, e->nc, e->hs, e->exp_lambda,
+                                                               e->escale, e->iscale);
+    LCHK();
+    RC(build_C<T>(e, 0, e->F));
+    RC(ue_calc<T>(e));
+    HIPCHK(hipStreamSynchronize(e->st));
+    const size_t bytes = (size_t)e->L * N * N * sizeof(T);
+    const int b = 2 * c + spin;
+    HIPCHK(hipMemcpy(G0t, (char*)e->G0t + b * bytes, bytes, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(Gtt, (char*)e->Gtt + b * bytes, bytes, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(Gt0, (char*)e->Gt0 + b * bytes, bytes, hipMemcpyDeviceToHost));
+    e->stale = true;
+    return 0;
+}

@@ -1,0 +1,953 @@
+// engine.hip -- the batched DQMC engine behind the C ABI of include/dqmc_hip.h.
+//
+// Replaces `static int dqmc(struct sim_data *)` (src/dqmc.c:84-511) for a
+// batch of independent chains of one shape.  All chains advance in lock step:
+// every launch covers all chains and both spins (batch index b = 2*chain +
+// spin), so small-N work still fills the GPU when many chains are present.
+// Per sweep and slice l the launch sequence mirrors src/dqmc.c:247-386:
+//   k_update (Metropolis + delayed updates, src/updates.c:7-84)
+//   recalc slice:  C_f from the HS scalings (GEMM chain) + stabilized G
+//                  (QRP -> [GEMM, pre-pivot, QR, GEMM]* -> LU solve + phase)
+//   other slices:  wrap G <- B_l G B_l^-1 (2 GEMMs with fused HS scalings)
+//   eq measurement: half-wrap (2 GEMMs) + contraction kernel
+// and once per measured sweep the unequal-time block (src/dqmc.c:389-458).
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "dq_common.hpp"
+#include "dq_factor.hpp"
+#include "dq_gemm.hpp"
+#include "dq_meas.hpp"
+#include "dq_update.hpp"
+#include "dqmc_hip.h"
+
+static thread_local std::string g_err;
+
+static int set_err(int code, const char* fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIPCHK(x)                                                                        \
+    do {                                                                                 \
+        hipError_t e_ = (x);                                                             \
+        if (e_ != hipSuccess)                                                            \
+            return set_err(DQMC_ERR_HIP, "%s:%d %s: %s", __FILE__, __LINE__, #x,         \
+                           hipGetErrorString(e_));                                       \
+    } while (0)
+
+#define LCHK()                                                                          \
+    do {                                                                                \
+        hipError_t e_ = hipGetLastError();                                              \
+        if (e_ != hipSuccess)                                                           \
+            return set_err(DQMC_ERR_HIP, "%s:%d launch: %s", __FILE__, __LINE__,         \
+                           hipGetErrorString(e_));                                      \
+    } while (0)
+
+#define RC(x)                 \
+    do {                      \
+        int rc_ = (x);        \
+        if (rc_) return rc_;  \
+    } while (0)
+
+extern "C" const char* dqmc_last_error(void) { return g_err.c_str(); }
+extern "C" const char* dqmc_version(void) { return "dqmc-mi355x 0.1 (gfx950, FP64 MFMA)"; }
+
+// --------------------------------------------------------------------------
+// accumulator sizes (src/data.c:441-517)
+// --------------------------------------------------------------------------
+extern "C" long dqmc_acc_len(const dqmc_params* p, int ue, int k)
+{
+    const long L = p->L, nij = p->num_ij;
+    if (!ue) {
+        switch (k) {
+        case DQ_EQ_density: case DQ_EQ_density_u: case DQ_EQ_density_d: case DQ_EQ_double_occ:
+            return p->num_i;
+        case DQ_EQ_g00: case DQ_EQ_g00_u: case DQ_EQ_g00_d: case DQ_EQ_nn: case DQ_EQ_xx:
+        case DQ_EQ_zz: case DQ_EQ_pair_sw:
+            return nij;
+        case DQ_EQ_uuuu: case DQ_EQ_dddd: case DQ_EQ_uudd: case DQ_EQ_dduu:
+            return p->meas_gen_suscept ? nij * nij : 0;
+        case DQ_EQ_chi: return p->meas_chiral ? p->num_plaq_accum : 0;
+        case DQ_EQ_kk: return p->meas_energy_corr ? p->num_bb : 0;
+        case DQ_EQ_kv: case DQ_EQ_kn: return p->meas_energy_corr ? p->num_bs : 0;
+        case DQ_EQ_vv: case DQ_EQ_vn: return p->meas_energy_corr ? nij : 0;
+        case DQ_EQ_j: case DQ_EQ_jn: return p->meas_local_JQ ? p->num_b_accum : 0;
+        case DQ_EQ_j2: return p->meas_local_JQ ? p->num_b2_accum : 0;
+        }
+        return 0;
+    }
+    if (p->period_uneqlt <= 0) return 0;
+    switch (k) {
+    case DQ_UE_gt0: case DQ_UE_gt0_u: case DQ_UE_gt0_d: case DQ_UE_nn: case DQ_UE_xx:
+    case DQ_UE_zz: case DQ_UE_pair_sw:
+        return nij * L;
+    case DQ_UE_uuuu: case DQ_UE_dddd: case DQ_UE_uudd: case DQ_UE_dduu:
+        return p->meas_gen_suscept ? nij * nij * L : 0;
+    case DQ_UE_pair_bb:
+        return (p->meas_bond_corr || p->meas_pair_bb_only) ? (long)p->num_bb * L : 0;
+    case DQ_UE_jj: case DQ_UE_jsjs: case DQ_UE_kk: case DQ_UE_ksks:
+        return p->meas_bond_corr ? (long)p->num_bb * L : 0;
+    case DQ_UE_jnjn: case DQ_UE_jnj: case DQ_UE_jjn:
+        return p->meas_thermal ? (long)p->num_bb * L : 0;
+    case DQ_UE_j2jn: return p->meas_thermal ? (long)p->num_b2b * L : 0;
+    case DQ_UE_jnj2: return p->meas_thermal ? (long)p->num_bb2 * L : 0;
+    case DQ_UE_pair_b2b2: case DQ_UE_j2j2: case DQ_UE_js2js2: case DQ_UE_k2k2: case DQ_UE_ks2ks2:
+        return p->meas_2bond_corr ? (long)p->num_b2b2 * L : 0;
+    case DQ_UE_j2j: return p->meas_2bond_corr ? (long)p->num_b2b * L : 0;
+    case DQ_UE_jj2: return p->meas_2bond_corr ? (long)p->num_bb2 * L : 0;
+    case DQ_UE_kv: case DQ_UE_kn: return p->meas_energy_corr ? (long)p->num_bs * L : 0;
+    case DQ_UE_vv: case DQ_UE_vn: return p->meas_energy_corr ? nij * L : 0;
+    case DQ_UE_nem_nnnn: case DQ_UE_nem_ssss:
+        return p->meas_nematic_corr ? (long)p->num_bb * L : 0;
+    }
+    return 0;
+}
+
+// accumulators computed on the GPU by this version
+static bool eq_supported(int k) { return k <= DQ_EQ_pair_sw || k == DQ_EQ_vv || k == DQ_EQ_vn; }
+static bool ue_supported(int k) { return k <= DQ_UE_pair_sw || k == DQ_UE_vv || k == DQ_UE_vn; }
+
+// --------------------------------------------------------------------------
+// engine state
+// --------------------------------------------------------------------------
+struct dqmc_engine {
+    int nc = 0, N = 0, L = 0, F = 0, n_matmul = 0, cplx = 0, kd = 16, device = 0;
+    int period_eqlt = 0, period_uneqlt = 0, energy = 0;
+    int num_i = 0, num_ij = 0;
+    size_t esz = 8;
+    hipStream_t st = nullptr;
+    std::vector<void*> allocs;
+    std::vector<int> sweep, n_sweep, n_sweep_warm;  // host mirrors
+    bool stale = true;
+    void *expK = 0, *iexpK = 0, *ehalf = 0, *iehalf = 0;  // [B][NN] T
+    double *exp_lambda = 0, *del = 0;                       // [nc][2N]
+    int* hs = 0;                                            // [nc][L N]
+    uint64_t* rng = 0;                                      // [nc][17]
+    void *G = 0, *C = 0, *Ab = 0, *Bb = 0, *W = 0;          // G [B][NN]; C, W [B][F][NN]
+    double *escale = 0, *iscale = 0;                        // [B][L N]
+    void *X = 0, *Y = 0, *Y2 = 0, *Z = 0, *Q = 0, *Tm = 0, *Rs = 0, *Tp = 0, *Mm = 0, *T1 = 0;
+    void* tau = 0;                                          // [B][N] T
+    int *jpvt = 0, *perm = 0, *ipiv = 0;                    // [B][N]
+    double *dvec = 0, *vvec = 0;                            // [B][N]
+    void *phb = 0, *phb2 = 0, *phase = 0;                   // [B], [B], [nc] T
+    unsigned long long* nacc = 0;                           // [nc]
+    int *active = 0, *flag_eq = 0, *flag_ue = 0;            // [nc]
+    int *d_sweep = 0, *d_nsweep = 0, *d_warm = 0;           // [nc]
+    long eq_off[DQMC_MAX_ACC], ue_off[DQMC_MAX_ACC];
+    long eq_len[DQMC_MAX_ACC], ue_len[DQMC_MAX_ACC];
+    long eq_total = 0, ue_total = 0;
+    long *d_eq_off = 0, *d_ue_off = 0;
+    void *acc_eq = 0, *acc_ue = 0, *sign_eq = 0, *sign_ue = 0;
+    int *ns_eq = 0, *ns_ue = 0;
+    int *rp_i = 0, *ix_i = 0, *rp_ij = 0, *ix_ij = 0, *degen = 0;
+    void *peierlsu = 0, *peierlsd = 0;
+    // unequal time
+    void *G0t = 0, *Gtt = 0, *Gt0 = 0;                      // [B][L][NN]
+    void *Gred = 0, *Wa = 0, *Wb = 0, *Qb = 0, *taub = 0, *T2 = 0;
+    int gemm_adiv = 1, gemm_bdiv = 1;  // one-shot batch divisors for the next gemm()
+    // GEMM timing (bench)
+    bool time_gemm = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> gemm_events;
+    double gemm_ms = 0.0, gemm_flops = 0.0;
+    long long gemm_launches = 0;
+};
+
+template <typename T>
+static T* dp(void* p) { return (T*)p; }
+
+static int dalloc(dqmc_engine* e, void** p, size_t bytes)
+{
+    *p = nullptr;
+    if (bytes == 0) bytes = 256;
+    hipError_t err = hipMalloc(p, bytes);
+    if (err != hipSuccess)
+        return set_err(DQMC_ERR_NOMEM, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(err));
+    e->allocs.push_back(*p);
+    // zero on the engine stream and wait: later set-up copies use the null
+    // stream, which does not order against a non-blocking stream
+    err = hipMemsetAsync(*p, 0, bytes, e->st);
+    if (err == hipSuccess) err = hipStreamSynchronize(e->st);
+    if (err != hipSuccess) return set_err(DQMC_ERR_HIP, "hipMemset: %s", hipGetErrorString(err));
+    return 0;
+}
+
+#define DALLOC(ptr, bytes) RC(dalloc(e, (void**)&(ptr), (size_t)(bytes)))
+
+static inline unsigned nblk(long total, int tpb = 256)
+{
+    long b = (total + tpb - 1) / tpb;
+    if (b > 65535L * 16) b = 65535L * 16;
+    return (unsigned)(b < 1 ? 1 : b);
+}
+
+// --------------------------------------------------------------------------
+// GEMM helper
+// --------------------------------------------------------------------------
+template <typename T>
+static int gemm(dqmc_engine* e, int opa, int opb, int M, int N, int K, T alpha, const T* A, int lda,
+                long sA, const T* B, int ldb, long sB, T beta, T* C, int ldc, long sC, int batch,
+                const double* r = nullptr, long sr = 0, const double* u = nullptr, long su = 0,
+                const double* w = nullptr, long sw = 0)
+{
+    GemmArgs<T> g;
+    g.M = M; g.N = N; g.K = K;
+    g.A = A; g.lda = lda; g.sA = sA;
+    g.B = B; g.ldb = ldb; g.sB = sB;
+    g.C = C; g.ldc = ldc; g.sC = sC;
+    g.r = r; g.sr = sr; g.u = u; g.su = su; g.w = w; g.sw = sw;
+    g.alpha = alpha; g.beta = beta;
+    g.beta_zero = (re(beta) == 0.0 && im(beta) == 0.0);
+    g.adiv = e->gemm_adiv > 0 ? e->gemm_adiv : 1;
+    g.bdiv = e->gemm_bdiv > 0 ? e->gemm_bdiv : 1;
+    e->gemm_adiv = e->gemm_bdiv = 1;
+    hipEvent_t a0 = 0, a1 = 0;
+    const bool timed = e->time_gemm && M == e->N && N == e->N && K == e->N && batch == 2 * e->nc;
+    if (timed) {
+        hipEventCreate(&a0);
+        hipEventCreate(&a1);
+        hipEventRecord(a0, e->st);
+    }
+    hipError_t err = GemmLauncher<T>::launch(opa, opb, g, batch, e->st);
+    if (timed) {
+        hipEventRecord(a1, e->st);
+        e->gemm_events.push_back({a0, a1});
+        e->gemm_flops = 2.0 * M * N * (double)K * batch * (is_cplx<T>::value ? 4 : 1);
+    }
+    if (err != hipSuccess) return set_err(DQMC_ERR_HIP, "gemm launch: %s", hipGetErrorString(err));
+    return 0;
+}
+
+template <typename T>
+__global__ void k_copy_block(int N, const T* src, long ssrc, int lds, T alpha, T* dst, long sdst,
+                             int ldd, int batch)
+{
+    const long NN = (long)N * N;
+    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < NN * batch;
+         e += (long)gridDim.x * blockDim.x) {
+        const int b = e / NN;
+        const long r = e % NN;
+        const int i = r % N, j = r / N;
+        dst[b * sdst + i + (long)ldd * j] = alpha * src[b * ssrc + i + (long)lds * j];
+    }
+}
+
+// --------------------------------------------------------------------------
+// B-product chains
+// --------------------------------------------------------------------------
+// C_f = B_{(f+1)n-1} ... B_{fn} for groups f0..f0+nf-1 of every (chain, spin)
+// (mul_seq over B, src/greens.c:8-41, called from src/dqmc.c:199-201, 273-274).
+template <typename T>
+static int build_C(dqmc_engine* e, int f0, int nf)
+{
+    const int N = e->N, n = e->n_matmul, B = 2 * e->nc, F = e->F;
+    const long NN = (long)N * N, LN = (long)e->L * N;
+    T* C = dp<T>(e->C);
+    T* W = dp<T>(e->W);
+    const T one = from_re<T>(1.0), zero = from_re<T>(0.0);
+    for (int f = f0; f < f0 + nf; f++) {
+        const int l0 = f * n;
+        T* bufs[2] = {C + f * NN, W + f * NN};
+        int cur = ((n - 1) % 2 == 0) ? 0 : 1;  // the last product lands in C_f
+        k_colscale<T><<<nblk(NN * B), 256, 0, e->st>>>(N, dp<T>(e->expK), NN, 1,
+                                                      e->escale + (long)N * l0, LN, bufs[cur],
+                                                      (long)F * NN, B);
+        LCHK();
+        for (int s = 1; s < n; s++) {
+            RC(gemm<T>(e, OP_N, OP_N, N, N, N, one, dp<T>(e->expK), N, NN, bufs[cur], N,
+                       (long)F * NN, zero, bufs[1 - cur], N, (long)F * NN, B, nullptr, 0,
+                       e->escale + (long)N * (l0 + s), LN));
+            cur = 1 - cur;
+        }
+    }
+    return 0;
+}
+
+// out[b] = alpha * C[maxp1-1] ... C[min] (indices mod F), as mul_seq over the
+// C_f array (src/greens.c:8-41).  Products of more than two factors use Rs/Tp.
+template <typename T>
+static int mul_seq_C(dqmc_engine* e, int min, int maxp1, T alpha, T* out, int ldo, long so)
+{
+    const int N = e->N, F = e->F, B = 2 * e->nc;
+    const long NN = (long)N * N, sC = (long)F * NN;
+    const T* C = dp<T>(e->C);
+    const int n_mul = (min == maxp1) ? F : (F + maxp1 - min) % F;
+    if (n_mul == 1) {
+        k_copy_block<T><<<nblk(NN * B), 256, 0, e->st>>>(N, C + min * NN, sC, N, alpha, out, so,
+                                                        ldo, B);
+        LCHK();
+        return 0;
+    }
+    const T one = from_re<T>(1.0), zero = from_re<T>(0.0);
+    T* tmp[2] = {dp<T>(e->Rs), dp<T>(e->Tp)};
+    const T* prev = C + min * NN;
+    long sprev = sC;
+    int l = (min + 1) % F;
+    for (int step = 1; step < n_mul; step++, l = (l + 1) % F) {
+        const bool last = (step == n_mul - 1);
+        T* dst = last ? out : tmp[step & 1];
+        RC(gemm<T>(e, OP_N, OP_N, N, N, N, last ? alpha : one, C + l * NN, N, sC, prev, N, sprev,
+                   zero, dst, last ? ldo : N, last ? so : NN, B));
+        prev = dst;
+        sprev = last ? so : NN;
+    }
+    return 0;
+}
+
+// --------------------------------------------------------------------------
+// stabilized equal-time G (calc_eq_g, src/greens.c:69-208) for all b:
+// Gout[b] <- (I + C_{l-1} ... C_l)^-1, phase_b[b] <- phase of det(I + ...)
+// --------------------------------------------------------------------------
+template <typename T>
+static int launch_getrs(dqmc_engine* e, T* Gout, T* Vlast, T* phase_b)
+{
+    const int N = e->N, B = 2 * e->nc;
+    const long NN = (long)N * N;
+    const int waves_per_blk = 4;
+    dim3 grid((N + waves_per_blk * LU_NC - 1) / (waves_per_blk * LU_NC), B);
+#define GRS(R)                                                                                  \
+    k_getrs_phase<T, R><<<grid, 256, 0, e->st>>>(N, dp<T>(e->Mm), Gout, NN, e->ipiv, N, e->vvec, \
+                                                 N, Vlast, NN, dp<T>(e->tau), N, phase_b)
+    if (N <= 64) GRS(1);
+    else if (N <= 128) GRS(2);
+    else if (N <= 256) GRS(4);
+    else if (N <= 512) GRS(8);
+    else GRS(16);
+#undef GRS
+    LCHK();
+    return 0;
+}
+
+template <typename T>
+static int calc_eq_g(dqmc_engine* e, int l, T* Gout, T* phase_b)
+{
+    const int N = e->N, F = e->F, B = 2 * e->nc;
+    const long NN = (long)N * N;
+    const int n_mul = 2;  // N_MUL, src/greens.h:5
+    T *X = dp<T>(e->X), *Y = dp<T>(e->Y), *Y2 = dp<T>(e->Y2), *Z = dp<T>(e->Z);
+    T *Q = dp<T>(e->Q), *Tm = dp<T>(e->Tm), *Rs = dp<T>(e->Rs), *Tp = dp<T>(e->Tp);
+    T* tau = dp<T>(e->tau);
+    const T one = from_re<T>(1.0), zero = from_re<T>(0.0);
+    int m = (l + 1 + (F - 1) % n_mul) % F;
+    RC(mul_seq_C<T>(e, l, m, one, X, N, NN));
+    k_qr<T, true><<<B, QR_THREADS, 0, e->st>>>(N, N, X, N, NN, tau, N, e->jpvt, N, Q, N, NN);
+    LCHK();
+    k_eqg_init_T<T><<<dim3(N, B), 256, 0, e->st>>>(N, X, NN, e->jpvt, N, e->dvec, N, Tm, NN);
+    LCHK();
+    T* Vlast = X;
+    while (m != l) {
+        const int next = (m + n_mul) % F;
+        RC(mul_seq_C<T>(e, m, next, one, Y, N, NN));
+        m = next;
+        // Y2 = Y * Q * diag(d)   (xunmqr R,N + column scaling)
+        RC(gemm<T>(e, OP_N, OP_N, N, N, N, one, Y, N, NN, Q, N, NN, zero, Y2, N, NN, B, nullptr,
+                   0, nullptr, 0, e->dvec, N));
+        k_prepivot<T><<<B, 1024, 0, e->st>>>(N, Y2, NN, Z, NN, e->perm, N);
+        LCHK();
+        k_qr<T, false><<<B, QR_THREADS, 0, e->st>>>(N, N, Z, N, NN, tau, N, nullptr, 0, Q, N, NN);
+        LCHK();
+        k_eqg_post_qr<T><<<dim3(N, B), 256, 0, e->st>>>(N, Z, NN, e->perm, N, Tm, NN, e->dvec, N,
+                                                       Rs, Tp, NN);
+        LCHK();
+        // T = diag(1/d) R * P^T T   (xtrmm L,U)
+        RC(gemm<T>(e, OP_N, OP_N, N, N, N, one, Rs, N, NN, Tp, N, NN, zero, Tm, N, NN, B));
+        Vlast = Z;
+    }
+    k_eqg_final_prep<T><<<dim3(N, B), 256, 0, e->st>>>(N, Q, NN, e->dvec, N, Tm, NN, dp<T>(e->Mm),
+                                                      Gout, NN, e->vvec, N);
+    LCHK();
+    k_getrf<T><<<B, 1024, 0, e->st>>>(N, dp<T>(e->Mm), Gout, NN, e->ipiv, N);
+    LCHK();
+    return launch_getrs<T>(e, Gout, Vlast, phase_b);
+}
+
+// --------------------------------------------------------------------------
+// sweep bookkeeping on the device (graph-friendly: no host round trips)
+// --------------------------------------------------------------------------
+__global__ void k_sweep_flags(int nc, int period_uneqlt, const int* sweep, const int* nsweep,
+                              const int* warm, int* active, int* flag_eq, int* flag_ue)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nc) return;
+    const int s = sweep[c];
+    const int act = s < nsweep[c];
+    active[c] = act;
+    flag_eq[c] = act && s >= warm[c];
+    flag_ue[c] = act && s >= warm[c] && period_uneqlt > 0 && (s % period_uneqlt) == 0;
+}
+
+__global__ void k_sweep_advance(int nc, int* sweep, const int* active)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < nc) sweep[c] += active[c];
+}
+
+// --------------------------------------------------------------------------
+// typed engine operations
+// --------------------------------------------------------------------------
+template <typename T>
+static int init_green(dqmc_engine* e)
+{
+    const int nc = e->nc;
+    k_scales<<<nblk((long)nc * e->L * e->N), 256, 0, e->st>>>(e->N, e->L, nc, e->hs, e->exp_lambda,
+                                                               e->escale, e->iscale);
+    LCHK();
+    RC(build_C<T>(e, 0, e->F));
+    RC(calc_eq_g<T>(e, 0, dp<T>(e->G), dp<T>(e->phb)));
+    // all chains: phase = phase_up * phase_dn (src/dqmc.c:219)
+    std::vector<int> ones(nc, 1);
+    HIPCHK(hipMemcpyAsync(e->active, ones.data(), nc * sizeof(int), hipMemcpyHostToDevice, e->st));
+    k_phase_combine<T><<<(nc + 255) / 256, 256, 0, e->st>>>(nc, dp<T>(e->phb), dp<T>(e->phase),
+                                                             e->active);
+    LCHK();
+    HIPCHK(hipStreamSynchronize(e->st));
+    e->stale = false;
+    return 0;
+}
+
+template <typename T>
+static MeasArgs<T> meas_args(dqmc_engine* e, int ue)
+{
+    MeasArgs<T> a;
+    memset(&a, 0, sizeof a);
+    a.N = e->N; a.L = e->L; a.num_i = e->num_i; a.num_ij = e->num_ij;
+    a.energy = e->energy;
+    a.phase = dp<T>(e->phase);
+    a.rp_i = e->rp_i; a.ix_i = e->ix_i; a.srp_i = e->num_i + 1; a.six_i = e->N;
+    a.rp_ij = e->rp_ij; a.ix_ij = e->ix_ij; a.srp_ij = e->num_ij + 1; a.six_ij = (long)e->N * e->N;
+    a.degen = e->degen;
+    a.peierlsu = dp<T>(e->peierlsu); a.peierlsd = dp<T>(e->peierlsd);
+    a.sP = is_cplx<T>::value ? (long)e->N * e->N : 0;
+    if (ue) {
+        a.flag = e->flag_ue; a.acc = dp<T>(e->acc_ue); a.sacc = e->ue_total; a.off = e->d_ue_off;
+        a.n_sample = e->ns_ue; a.sign = dp<T>(e->sign_ue);
+    } else {
+        a.flag = e->flag_eq; a.acc = dp<T>(e->acc_eq); a.sacc = e->eq_total; a.off = e->d_eq_off;
+        a.n_sample = e->ns_eq; a.sign = dp<T>(e->sign_eq);
+    }
+    return a;
+}
+
+// unequal-time Green's functions and measurements
+#include "dq_ue_host.hpp"
+
+template <typename T>
+static int run_t(dqmc_engine* e, int n_sweeps, const volatile int* stop)
+{
+    const int N = e->N, L = e->L, nc = e->nc, B = 2 * nc, n = e->n_matmul, F = e->F;
+    const long NN = (long)N * N, LN = (long)L * N;
+    const T one = from_re<T>(1.0), zero = from_re<T>(0.0);
+    if (e->stale) RC(init_green<T>(e));
+    T* G = dp<T>(e->G);
+    T* T1 = dp<T>(e->T1);
+    for (int s = 0; s < n_sweeps; s++) {
+        if (stop && *stop) {
+            HIPCHK(hipStreamSynchronize(e->st));
+            return DQMC_INTERRUPTED;
+        }
+        bool any_active = false, any_eq = false, any_ue = false;
+        for (int c = 0; c < nc; c++) {
+            const bool act = e->sweep[c] < e->n_sweep[c];
+            any_active |= act;
+            any_eq |= act && e->sweep[c] >= e->n_sweep_warm[c] && e->period_eqlt > 0;
+            any_ue |= act && e->sweep[c] >= e->n_sweep_warm[c] && e->period_uneqlt > 0 &&
+                      e->sweep[c] % e->period_uneqlt == 0;
+        }
+        if (!any_active) break;
+        k_sweep_flags<<<(nc + 255) / 256, 256, 0, e->st>>>(nc, e->period_uneqlt, e->d_sweep,
+                                                            e->d_nsweep, e->d_warm, e->active,
+                                                            e->flag_eq, e->flag_ue);
+        LCHK();
+        for (int l = 0; l < L; l++) {
+            UpdArgs<T> ua;
+            ua.N = N; ua.L = L; ua.l = l; ua.kd = e->kd;
+            ua.G = G; ua.sG = NN;
+            ua.Ab = dp<T>(e->Ab); ua.Bb = dp<T>(e->Bb); ua.sAB = (long)N * e->kd;
+            ua.del = e->del; ua.exp_lambda = e->exp_lambda; ua.hs = e->hs; ua.rng = e->rng;
+            ua.phase = dp<T>(e->phase); ua.escale = e->escale; ua.iscale = e->iscale;
+            ua.n_acc = e->nacc; ua.active = e->active;
+            k_update<T><<<nc, UPD_THREADS, 0, e->st>>>(ua);
+            LCHK();
+            const int f = l / n;
+            const bool recalc = ((l + 1) % n == 0);
+            if (recalc) {
+                RC(build_C<T>(e, f, 1));
+                RC(calc_eq_g<T>(e, (f + 1) % F, G, dp<T>(e->phb)));
+                k_phase_combine<T><<<(nc + 255) / 256, 256, 0, e->st>>>(nc, dp<T>(e->phb),
+                                                                         dp<T>(e->phase), e->active);
+                LCHK();
+            } else {
+                // G <- B_l G B_l^-1 = expK diag(e_l) [G diag(e'_l) invexpK]  (src/dqmc.c:294-295)
+                RC(gemm<T>(e, OP_N, OP_N, N, N, N, one, G, N, NN, dp<T>(e->iexpK), N, NN, zero, T1,
+                           N, NN, B, nullptr, 0, e->iscale + (long)N * l, LN));
+                RC(gemm<T>(e, OP_N, OP_N, N, N, N, one, dp<T>(e->expK), N, NN, T1, N, NN, zero, G,
+                           N, NN, B, nullptr, 0, e->escale + (long)N * l, LN));
+            }
+            if (any_eq && (l + 1) % e->period_eqlt == 0) {
+                // half-wrap e^{dtK/2} G e^{-dtK/2} (src/dqmc.c:364-380), into Y2
+                RC(gemm<T>(e, OP_N, OP_N, N, N, N, one, G, N, NN, dp<T>(e->ehalf), N, NN, zero, T1,
+                           N, NN, B));
+                RC(gemm<T>(e, OP_N, OP_N, N, N, N, one, dp<T>(e->iehalf), N, NN, T1, N, NN, zero,
+                           dp<T>(e->Y2), N, NN, B));
+                MeasArgs<T> ma = meas_args<T>(e, 0);
+                ma.G = dp<T>(e->Y2);
+                ma.sG = NN;
+                const int nb = (std::max(e->num_i, e->num_ij) + 255) / 256;
+                k_meas_eq<T><<<dim3(nb, nc), 256, 0, e->st>>>(ma);
+                LCHK();
+            }
+        }
+        if (any_ue) RC(ue_measure<T>(e));
+        k_sweep_advance<<<(nc + 255) / 256, 256, 0, e->st>>>(nc, e->d_sweep, e->active);
+        LCHK();
+        for (int c = 0; c < nc; c++)
+            if (e->sweep[c] < e->n_sweep[c]) e->sweep[c]++;
+    }
+    HIPCHK(hipStreamSynchronize(e->st));
+    return 0;
+}
+
+// --------------------------------------------------------------------------
+// C ABI
+// --------------------------------------------------------------------------
+static int same_shape(const dqmc_params* a, const dqmc_params* b)
+{
+    return a->N == b->N && a->L == b->L && a->F == b->F && a->n_matmul == b->n_matmul &&
+           a->is_complex == b->is_complex && a->period_eqlt == b->period_eqlt &&
+           a->period_uneqlt == b->period_uneqlt && a->num_i == b->num_i &&
+           a->num_ij == b->num_ij && a->meas_energy_corr == b->meas_energy_corr &&
+           a->meas_bond_corr == b->meas_bond_corr && a->meas_thermal == b->meas_thermal &&
+           a->meas_2bond_corr == b->meas_2bond_corr && a->meas_chiral == b->meas_chiral &&
+           a->meas_gen_suscept == b->meas_gen_suscept && a->meas_local_JQ == b->meas_local_JQ &&
+           a->meas_nematic_corr == b->meas_nematic_corr &&
+           a->meas_pair_bb_only == b->meas_pair_bb_only && a->num_b == b->num_b &&
+           a->num_bb == b->num_bb && a->num_bs == b->num_bs;
+}
+
+static void csr_of_map(const int32_t* map, int n_in, int nbins, std::vector<int>& rp,
+                       std::vector<int>& ix)
+{
+    rp.assign(nbins + 1, 0);
+    for (int q = 0; q < n_in; q++) rp[map[q] + 1]++;
+    for (int r = 0; r < nbins; r++) rp[r + 1] += rp[r];
+    std::vector<int> fill(rp.begin(), rp.end() - 1);
+    ix.assign(n_in, 0);
+    for (int q = 0; q < n_in; q++) ix[fill[map[q]]++] = q;  // increasing q within a bin
+}
+
+static int engine_setup(dqmc_engine* e, const dqmc_params* const* P)
+{
+    const dqmc_params* p0 = P[0];
+    const int nc = e->nc, N = e->N, L = e->L, F = e->F, B = 2 * nc;
+    const long NN = (long)N * N;
+    const size_t es = e->esz;
+    // check the measurement set against what the GPU computes
+    for (int k = 0; k < DQ_EQ_COUNT; k++)
+        if (dqmc_acc_len(p0, 0, k) > 0 && !eq_supported(k))
+            return set_err(DQMC_ERR_UNSUPPORTED,
+                           "equal-time accumulator %d is enabled but not implemented on the GPU", k);
+    for (int k = 0; k < DQ_UE_COUNT; k++)
+        if (dqmc_acc_len(p0, 1, k) > 0 && !ue_supported(k))
+            return set_err(DQMC_ERR_UNSUPPORTED,
+                           "unequal-time accumulator %d is enabled but not implemented on the GPU",
+                           k);
+    DALLOC(e->expK, B * NN * es);
+    DALLOC(e->iexpK, B * NN * es);
+    DALLOC(e->ehalf, B * NN * es);
+    DALLOC(e->iehalf, B * NN * es);
+    DALLOC(e->exp_lambda, nc * 2L * N * 8);
+    DALLOC(e->del, nc * 2L * N * 8);
+    DALLOC(e->hs, (long)nc * L * N * 4);
+    DALLOC(e->rng, nc * 17 * 8);
+    DALLOC(e->G, B * NN * es);
+    DALLOC(e->C, B * F * NN * es);
+    DALLOC(e->W, B * F * NN * es);
+    DALLOC(e->Ab, (long)B * N * e->kd * es);
+    DALLOC(e->Bb, (long)B * N * e->kd * es);
+    DALLOC(e->escale, (long)B * L * N * 8);
+    DALLOC(e->iscale, (long)B * L * N * 8);
+    void** work[] = {&e->X, &e->Y, &e->Y2, &e->Z, &e->Q, &e->Tm, &e->Rs, &e->Tp, &e->Mm, &e->T1};
+    for (void** w : work) DALLOC(*w, B * NN * es);
+    DALLOC(e->tau, (long)B * N * es);
+    DALLOC(e->jpvt, (long)B * N * 4);
+    DALLOC(e->perm, (long)B * N * 4);
+    DALLOC(e->ipiv, (long)B * N * 4);
+    DALLOC(e->dvec, (long)B * N * 8);
+    DALLOC(e->vvec, (long)B * N * 8);
+    DALLOC(e->phb, B * es);
+    DALLOC(e->phb2, B * es);
+    DALLOC(e->phase, nc * es);
+    DALLOC(e->nacc, nc * 8);
+    DALLOC(e->active, nc * 4);
+    DALLOC(e->flag_eq, nc * 4);
+    DALLOC(e->flag_ue, nc * 4);
+    DALLOC(e->d_sweep, nc * 4);
+    DALLOC(e->d_nsweep, nc * 4);
+    DALLOC(e->d_warm, nc * 4);
+    // accumulators
+    e->eq_total = 0;
+    for (int k = 0; k < DQMC_MAX_ACC; k++) {
+        e->eq_len[k] = k < DQ_EQ_COUNT ? dqmc_acc_len(p0, 0, k) : 0;
+        e->eq_off[k] = e->eq_len[k] > 0 ? e->eq_total : 0;
+        e->eq_total += e->eq_len[k];
+        e->ue_len[k] = k < DQ_UE_COUNT ? dqmc_acc_len(p0, 1, k) : 0;
+        e->ue_off[k] = e->ue_len[k] > 0 ? e->ue_total : 0;
+        e->ue_total += e->ue_len[k];
+    }
+    DALLOC(e->d_eq_off, DQMC_MAX_ACC * 8);
+    DALLOC(e->d_ue_off, DQMC_MAX_ACC * 8);
+    HIPCHK(hipMemcpyAsync(e->d_eq_off, e->eq_off, sizeof e->eq_off, hipMemcpyHostToDevice, e->st));
+    HIPCHK(hipMemcpyAsync(e->d_ue_off, e->ue_off, sizeof e->ue_off, hipMemcpyHostToDevice, e->st));
+    DALLOC(e->acc_eq, (long)nc * e->eq_total * es);
+    DALLOC(e->acc_ue, (long)nc * (e->ue_total ? e->ue_total : 1) * es);
+    DALLOC(e->sign_eq, nc * es);
+    DALLOC(e->sign_ue, nc * es);
+    DALLOC(e->ns_eq, nc * 4);
+    DALLOC(e->ns_ue, nc * 4);
+    DALLOC(e->rp_i, (long)nc * (e->num_i + 1) * 4);
+    DALLOC(e->ix_i, (long)nc * N * 4);
+    DALLOC(e->rp_ij, (long)nc * (e->num_ij + 1) * 4);
+    DALLOC(e->ix_ij, (long)nc * NN * 4);
+    DALLOC(e->degen, nc * 3 * 4);
+    DALLOC(e->peierlsu, (e->cplx ? nc * NN : 1) * es);
+    DALLOC(e->peierlsd, (e->cplx ? nc * NN : 1) * es);
+    if (e->period_uneqlt > 0) {
+        const long LNN = (long)L * NN;
+        DALLOC(e->G0t, B * LNN * es);
+        DALLOC(e->Gtt, B * LNN * es);
+        DALLOC(e->Gt0, B * LNN * es);
+        RC(ue_alloc(e));
+    }
+    // per-chain parameters
+    std::vector<int> rp, ix, deg(3 * nc);
+    for (int c = 0; c < nc; c++) {
+        const dqmc_params* p = P[c];
+        const size_t mb = NN * es;
+        HIPCHK(hipMemcpyAsync((char*)e->expK + (2 * c) * mb, p->exp_Ku, mb, hipMemcpyHostToDevice, e->st));
+        HIPCHK(hipMemcpyAsync((char*)e->expK + (2 * c + 1) * mb, p->exp_Kd, mb, hipMemcpyHostToDevice, e->st));
+        HIPCHK(hipMemcpyAsync((char*)e->iexpK + (2 * c) * mb, p->inv_exp_Ku, mb, hipMemcpyHostToDevice, e->st));
+        HIPCHK(hipMemcpyAsync((char*)e->iexpK + (2 * c + 1) * mb, p->inv_exp_Kd, mb, hipMemcpyHostToDevice, e->st));
+        HIPCHK(hipMemcpyAsync((char*)e->ehalf + (2 * c) * mb, p->exp_halfKu, mb, hipMemcpyHostToDevice, e->st));
+        HIPCHK(hipMemcpyAsync((char*)e->ehalf + (2 * c + 1) * mb, p->exp_halfKd, mb, hipMemcpyHostToDevice, e->st));
+        HIPCHK(hipMemcpyAsync((char*)e->iehalf + (2 * c) * mb, p->inv_exp_halfKu, mb, hipMemcpyHostToDevice, e->st));
+        HIPCHK(hipMemcpyAsync((char*)e->iehalf + (2 * c + 1) * mb, p->inv_exp_halfKd, mb, hipMemcpyHostToDevice, e->st));
+        HIPCHK(hipMemcpyAsync(e->exp_lambda + c * 2L * N, p->exp_lambda, 2L * N * 8, hipMemcpyHostToDevice, e->st));
+        HIPCHK(hipMemcpyAsync(e->del + c * 2L * N, p->del, 2L * N * 8, hipMemcpyHostToDevice, e->st));
+        if (e->cplx) {
+            HIPCHK(hipMemcpyAsync((char*)e->peierlsu + c * mb, p->peierlsu, mb, hipMemcpyHostToDevice, e->st));
+            HIPCHK(hipMemcpyAsync((char*)e->peierlsd + c * mb, p->peierlsd, mb, hipMemcpyHostToDevice, e->st));
+        }
+        csr_of_map(p->map_i, N, e->num_i, rp, ix);
+        HIPCHK(hipMemcpy(e->rp_i + c * (e->num_i + 1L), rp.data(), rp.size() * 4, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(e->ix_i + c * (long)N, ix.data(), ix.size() * 4, hipMemcpyHostToDevice));
+        csr_of_map(p->map_ij, (int)NN, e->num_ij, rp, ix);
+        HIPCHK(hipMemcpy(e->rp_ij + c * (e->num_ij + 1L), rp.data(), rp.size() * 4, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(e->ix_ij + c * NN, ix.data(), ix.size() * 4, hipMemcpyHostToDevice));
+        deg[3 * c] = p->degen_i;
+        deg[3 * c + 1] = p->degen_ij;
+        deg[3 * c + 2] = p->degen_bb;
+        e->n_sweep[c] = p->n_sweep;
+        e->n_sweep_warm[c] = p->n_sweep_warm;
+    }
+    HIPCHK(hipMemcpy(e->degen, deg.data(), deg.size() * 4, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(e->d_nsweep, e->n_sweep.data(), nc * 4, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(e->d_warm, e->n_sweep_warm.data(), nc * 4, hipMemcpyHostToDevice));
+    HIPCHK(hipStreamSynchronize(e->st));
+    return 0;
+}
+
+extern "C" int dqmc_engine_create(dqmc_engine** out, const dqmc_params* const* P, int n_chains,
+                                  int device, const dqmc_tuning* tune)
+{
+    if (!out || !P || n_chains <= 0) return set_err(DQMC_ERR_ARG, "bad arguments");
+    *out = nullptr;
+    const dqmc_params* p0 = P[0];
+    if (p0->N <= 0 || p0->N > QR_MAXM || p0->L <= 0 || p0->n_matmul <= 0 ||
+        p0->L % p0->n_matmul != 0 || p0->F != p0->L / p0->n_matmul)
+        return set_err(DQMC_ERR_ARG, "bad shape N=%d L=%d n_matmul=%d F=%d", p0->N, p0->L,
+                       p0->n_matmul, p0->F);
+    if (p0->N > UPD_MAXN) return set_err(DQMC_ERR_ARG, "N=%d exceeds %d", p0->N, UPD_MAXN);
+    for (int c = 1; c < n_chains; c++)
+        if (!same_shape(p0, P[c]))
+            return set_err(DQMC_ERR_ARG, "chain %d has a different shape than chain 0", c);
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+        return set_err(DQMC_ERR_HIP, "no HIP device available");
+    if (device < 0 || device >= ndev) return set_err(DQMC_ERR_ARG, "bad device %d", device);
+    HIPCHK(hipSetDevice(device));
+    dqmc_engine* e = new dqmc_engine();
+    e->nc = n_chains; e->N = p0->N; e->L = p0->L; e->F = p0->F; e->n_matmul = p0->n_matmul;
+    e->cplx = p0->is_complex; e->device = device;
+    e->period_eqlt = p0->period_eqlt; e->period_uneqlt = p0->period_uneqlt;
+    e->energy = p0->meas_energy_corr;
+    e->num_i = p0->num_i; e->num_ij = p0->num_ij;
+    e->esz = e->cplx ? 16 : 8;
+    e->kd = (tune && tune->delay_rank > 0) ? tune->delay_rank : (p0->n_delay > 0 ? p0->n_delay : 16);
+    if (e->kd > e->N) e->kd = e->N;
+    e->time_gemm = tune && (tune->flags & 1);
+    e->sweep.assign(n_chains, 0);
+    e->n_sweep.assign(n_chains, 0);
+    e->n_sweep_warm.assign(n_chains, 0);
+    if (hipStreamCreateWithFlags(&e->st, hipStreamNonBlocking) != hipSuccess) {
+        delete e;
+        return set_err(DQMC_ERR_HIP, "hipStreamCreate failed");
+    }
+    int rc = engine_setup(e, P);
+    if (rc) {
+        dqmc_engine_destroy(e);
+        return rc;
+    }
+    *out = e;
+    return 0;
+}
+
+extern "C" void dqmc_engine_destroy(dqmc_engine* e)
+{
+    if (!e) return;
+    hipSetDevice(e->device);
+    if (e->st) hipStreamSynchronize(e->st);
+    for (auto& ev : e->gemm_events) {
+        hipEventDestroy(ev.first);
+        hipEventDestroy(ev.second);
+    }
+    for (void* p : e->allocs) hipFree(p);
+    if (e->st) hipStreamDestroy(e->st);
+    delete e;
+}
+
+extern "C" int dqmc_engine_set_state(dqmc_engine* e, int c, const uint64_t* rng, int sweep,
+                                     const int32_t* hs)
+{
+    if (!e || c < 0 || c >= e->nc) return set_err(DQMC_ERR_ARG, "bad chain");
+    HIPCHK(hipSetDevice(e->device));
+    const long LN = (long)e->L * e->N;
+    HIPCHK(hipMemcpy(e->rng + c * 17, rng, 17 * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(e->hs + c * LN, hs, LN * 4, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(e->d_sweep + c, &sweep, 4, hipMemcpyHostToDevice));
+    e->sweep[c] = sweep;
+    e->stale = true;
+    return 0;
+}
+
+extern "C" int dqmc_engine_get_state(dqmc_engine* e, int c, uint64_t* rng, int* sweep, int32_t* hs)
+{
+    if (!e || c < 0 || c >= e->nc) return set_err(DQMC_ERR_ARG, "bad chain");
+    HIPCHK(hipSetDevice(e->device));
+    HIPCHK(hipStreamSynchronize(e->st));
+    const long LN = (long)e->L * e->N;
+    if (rng) HIPCHK(hipMemcpy(rng, e->rng + c * 17, 17 * 8, hipMemcpyDeviceToHost));
+    if (hs) HIPCHK(hipMemcpy(hs, e->hs + c * LN, LN * 4, hipMemcpyDeviceToHost));
+    if (sweep) *sweep = e->sweep[c];
+    return 0;
+}
+
+static int meas_copy(dqmc_engine* e, int c, dqmc_meas* m, int ue, int to_dev)
+{
+    const long* off = ue ? e->ue_off : e->eq_off;
+    const long* len = ue ? e->ue_len : e->eq_len;
+    const long total = ue ? e->ue_total : e->eq_total;
+    char* acc = (char*)(ue ? e->acc_ue : e->acc_eq) + (size_t)c * total * e->esz;
+    const int nk = ue ? DQ_UE_COUNT : DQ_EQ_COUNT;
+    for (int k = 0; k < nk; k++) {
+        if (len[k] <= 0 || !m->acc[k]) continue;
+        if (to_dev)
+            HIPCHK(hipMemcpy(acc + off[k] * e->esz, m->acc[k], len[k] * e->esz, hipMemcpyHostToDevice));
+        else
+            HIPCHK(hipMemcpy(m->acc[k], acc + off[k] * e->esz, len[k] * e->esz, hipMemcpyDeviceToHost));
+    }
+    int* ns = (ue ? e->ns_ue : e->ns_eq) + c;
+    char* sg = (char*)(ue ? e->sign_ue : e->sign_eq) + c * e->esz;
+    if (to_dev) {
+        HIPCHK(hipMemcpy(ns, &m->n_sample, 4, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(sg, m->sign, e->esz, hipMemcpyHostToDevice));
+    } else {
+        HIPCHK(hipMemcpy(&m->n_sample, ns, 4, hipMemcpyDeviceToHost));
+        double sgv[2] = {0.0, 0.0};
+        HIPCHK(hipMemcpy(sgv, sg, e->esz, hipMemcpyDeviceToHost));
+        m->sign[0] = sgv[0];
+        m->sign[1] = sgv[1];
+    }
+    return 0;
+}
+
+extern "C" int dqmc_engine_set_meas(dqmc_engine* e, int c, const dqmc_meas* eq, const dqmc_meas* ue)
+{
+    if (!e || c < 0 || c >= e->nc) return set_err(DQMC_ERR_ARG, "bad chain");
+    HIPCHK(hipSetDevice(e->device));
+    HIPCHK(hipStreamSynchronize(e->st));
+    if (eq) RC(meas_copy(e, c, (dqmc_meas*)eq, 0, 1));
+    if (ue && e->period_uneqlt > 0) RC(meas_copy(e, c, (dqmc_meas*)ue, 1, 1));
+    return 0;
+}
+
+extern "C" int dqmc_engine_get_meas(dqmc_engine* e, int c, dqmc_meas* eq, dqmc_meas* ue)
+{
+    if (!e || c < 0 || c >= e->nc) return set_err(DQMC_ERR_ARG, "bad chain");
+    HIPCHK(hipSetDevice(e->device));
+    HIPCHK(hipStreamSynchronize(e->st));
+    if (eq) RC(meas_copy(e, c, eq, 0, 0));
+    if (ue && e->period_uneqlt > 0) RC(meas_copy(e, c, ue, 1, 0));
+    return 0;
+}
+
+extern "C" int dqmc_engine_run(dqmc_engine* e, int n_sweeps, const volatile int* stop_flag)
+{
+    if (!e || n_sweeps < 0) return set_err(DQMC_ERR_ARG, "bad arguments");
+    HIPCHK(hipSetDevice(e->device));
+    return e->cplx ? run_t<zd>(e, n_sweeps, stop_flag) : run_t<double>(e, n_sweeps, stop_flag);
+}
+
+extern "C" int dqmc_engine_green(dqmc_engine* e, int c, int spin, double* G, double* phase)
+{
+    if (!e || c < 0 || c >= e->nc || spin < 0 || spin > 1) return set_err(DQMC_ERR_ARG, "bad args");
+    HIPCHK(hipSetDevice(e->device));
+    if (e->stale) RC(e->cplx ? init_green<zd>(e) : init_green<double>(e));
+    HIPCHK(hipStreamSynchronize(e->st));
+    const size_t mb = (size_t)e->N * e->N * e->esz;
+    if (G) HIPCHK(hipMemcpy(G, (char*)e->G + (2 * c + spin) * mb, mb, hipMemcpyDeviceToHost));
+    if (phase) {
+        double v[2] = {0.0, 0.0};
+        HIPCHK(hipMemcpy(v, (char*)e->phase + c * e->esz, e->esz, hipMemcpyDeviceToHost));
+        phase[0] = v[0];
+        phase[1] = v[1];
+    }
+    return 0;
+}
+
+template <typename T>
+static int calc_eq_g_hook(dqmc_engine* e, int c, int spin, int f, double* G, double* phase)
+{
+    const int N = e->N;
+    const long NN = (long)N * N;
+    k_scales<<<nblk((long)e->nc * e->L * N), 256, 0, e->st>>>(N, e->L, e->nc, e->hs, e->exp_lambda,
+                                                               e->escale, e->iscale);
+    LCHK();
+    RC(build_C<T>(e, 0, e->F));
+    T* out = dp<T>(e->W);  // free after build_C
+    RC(calc_eq_g<T>(e, f, out, dp<T>(e->phb2)));
+    HIPCHK(hipStreamSynchronize(e->st));
+    const int b = 2 * c + spin;
+    HIPCHK(hipMemcpy(G, out + b * NN, NN * sizeof(T), hipMemcpyDeviceToHost));
+    T ph;
+    HIPCHK(hipMemcpy(&ph, dp<T>(e->phb2) + b, sizeof(T), hipMemcpyDeviceToHost));
+    phase[0] = re(ph);
+    phase[1] = im(ph);
+    e->stale = true;  // C_f was rebuilt for the current field; G must follow
+    return 0;
+}
+
+extern "C" int dqmc_engine_calc_eq_g(dqmc_engine* e, int c, int spin, int f, double* G,
+                                     double* phase)
+{
+    if (!e || c < 0 || c >= e->nc || spin < 0 || spin > 1 || f < 0 || f >= e->F)
+        return set_err(DQMC_ERR_ARG, "bad args");
+    HIPCHK(hipSetDevice(e->device));
+    return e->cplx ? calc_eq_g_hook<zd>(e, c, spin, f, G, phase)
+                   : calc_eq_g_hook<double>(e, c, spin, f, G, phase);
+}
+
+extern "C" int dqmc_engine_calc_ue_g(dqmc_engine* e, int c, int spin, double* G0t, double* Gtt,
+                                     double* Gt0)
+{
+    if (!e || c < 0 || c >= e->nc || spin < 0 || spin > 1) return set_err(DQMC_ERR_ARG, "bad args");
+    if (e->period_uneqlt <= 0)
+        return set_err(DQMC_ERR_ARG, "engine created with period_uneqlt == 0");
+    HIPCHK(hipSetDevice(e->device));
+    return e->cplx ? ue_hook<zd>(e, c, spin, G0t, Gtt, Gt0)
+                   : ue_hook<double>(e, c, spin, G0t, Gtt, Gt0);
+}
+
+extern "C" int64_t dqmc_engine_accepts(dqmc_engine* e)
+{
+    if (!e) return -1;
+    hipSetDevice(e->device);
+    hipStreamSynchronize(e->st);
+    std::vector<unsigned long long> v(e->nc);
+    if (hipMemcpy(v.data(), e->nacc, e->nc * 8, hipMemcpyDeviceToHost) != hipSuccess) return -1;
+    int64_t s = 0;
+    for (auto x : v) s += (int64_t)x;
+    return s;
+}
+
+extern "C" int dqmc_engine_profile(dqmc_engine* e, const char** names, double* ms, int max)
+{
+    (void)e; (void)names; (void)ms; (void)max;
+    return 0;
+}
+
+extern "C" int dqmc_engine_gemm_stats(dqmc_engine* e, double* avg_ms, int64_t* launches,
+                                      double* flops_per_launch)
+{
+    if (!e) return set_err(DQMC_ERR_ARG, "null engine");
+    HIPCHK(hipStreamSynchronize(e->st));
+    double tot = 0.0;
+    for (auto& ev : e->gemm_events) {
+        float t = 0.f;
+        hipEventElapsedTime(&t, ev.first, ev.second);
+        tot += t;
+        hipEventDestroy(ev.first);
+        hipEventDestroy(ev.second);
+    }
+    const long long n = (long long)e->gemm_events.size();
+    e->gemm_events.clear();
+    if (avg_ms) *avg_ms = n ? tot / n : 0.0;
+    if (launches) *launches = n;
+    if (flops_per_launch) *flops_per_launch = e->gemm_flops;
+    return 0;
+}
+
+// Kernel-level test hook: one GEMM through the engine's MFMA kernel on host
+// buffers, C = alpha diag(r) op(A) diag(u) op(B) diag(w) + beta C.
+template <typename T>
+static int test_gemm_t(int opa, int opb, int M, int N, int K, const double* alpha,
+                       const double* A, int lda, const double* B, int ldb, const double* beta,
+                       double* Cm, int ldc, const double* r, const double* u, const double* w)
+{
+    dqmc_engine tmp;
+    dqmc_engine* e = &tmp;
+    HIPCHK(hipStreamCreate(&e->st));
+    const int ka = (opa == OP_N) ? K : M, kb = (opb == OP_N) ? N : K;
+    void *dA, *dB, *dC, *dr = 0, *du = 0, *dw = 0;
+    const size_t es = sizeof(T);
+    DALLOC(dA, (size_t)lda * ka * es);
+    DALLOC(dB, (size_t)ldb * kb * es);
+    DALLOC(dC, (size_t)ldc * N * es);
+    HIPCHK(hipMemcpy(dA, A, (size_t)lda * ka * es, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(dB, B, (size_t)ldb * kb * es, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(dC, Cm, (size_t)ldc * N * es, hipMemcpyHostToDevice));
+    if (r) { DALLOC(dr, M * 8); HIPCHK(hipMemcpy(dr, r, M * 8, hipMemcpyHostToDevice)); }
+    if (u) { DALLOC(du, K * 8); HIPCHK(hipMemcpy(du, u, K * 8, hipMemcpyHostToDevice)); }
+    if (w) { DALLOC(dw, N * 8); HIPCHK(hipMemcpy(dw, w, N * 8, hipMemcpyHostToDevice)); }
+    T al, be;
+    memcpy(&al, alpha, es);
+    memcpy(&be, beta, es);
+    RC(gemm<T>(e, opa, opb, M, N, K, al, (T*)dA, lda, 0, (T*)dB, ldb, 0, be, (T*)dC, ldc, 0, 1,
+               (double*)dr, 0, (double*)du, 0, (double*)dw, 0));
+    HIPCHK(hipStreamSynchronize(e->st));
+    HIPCHK(hipMemcpy(Cm, dC, (size_t)ldc * N * es, hipMemcpyDeviceToHost));
+    for (void* p : e->allocs) (void)hipFree(p);
+    e->allocs.clear();
+    (void)hipStreamDestroy(e->st);
+    return 0;
+}
+
+extern "C" int dqmc_test_gemm(int is_complex, int opa, int opb, int M, int N, int K,
+                              const double* alpha, const double* A, int lda, const double* B,
+                              int ldb, const double* beta, double* Cm, int ldc, const double* r,
+                              const double* u, const double* w)
+{
+    if (M <= 0 || N <= 0 || K <= 0) return set_err(DQMC_ERR_ARG, "bad GEMM shape");
+    return is_complex ? test_gemm_t<zd>(opa, opb, M, N, K, alpha, A, lda, B, ldb, beta, Cm, ldc, r, u, w)
+                      : test_gemm_t<double>(opa, opb, M, N, K, alpha, A, lda, B, ldb, beta, Cm, ldc, r, u, w);
+}

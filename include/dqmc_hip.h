@@ -166,6 +166,15 @@ int dqmc_engine_profile(dqmc_engine *e, const char **names, double *ms, int max)
 int dqmc_engine_gemm_stats(dqmc_engine *e, double *avg_ms, int64_t *launches,
                            double *flops_per_launch);
 
+/* Kernel-level test hook (no counterpart in the reference): one GEMM through
+ * the engine's FP64 MFMA kernel on host buffers,
+ *   C = alpha diag(r) op(A) diag(u) op(B) diag(w) + beta C
+ * op: 0 = N, 1 = T, 2 = conjugate transpose; alpha/beta are num; r/u/w may
+ * be NULL.  Used by the parity tests of the GEMM kernel. */
+int dqmc_test_gemm(int is_complex, int opa, int opb, int M, int N, int K, const double *alpha,
+                   const double *A, int lda, const double *B, int ldb, const double *beta,
+                   double *C, int ldc, const double *r, const double *u, const double *w);
+
 const char *dqmc_last_error(void);
 const char *dqmc_version(void);
 
