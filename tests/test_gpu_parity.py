@@ -1,0 +1,185 @@
+"""GPU parity: the HIP engine (through the C ABI) against the oracle and the
+reference's golden data.
+
+Tolerances: integer HS fields and RNG states bit-exact; equal-time G 1e-10
+relative max-norm; unequal-time blocks 1e-8 relative
+max-norm (products over up to beta/2 in FP64, both sides carry ~1e-10 of
+rounding); accumulators 1e-9 relative to the largest entry.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from dqmc_amd import engine, gen
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return np.abs(np.asarray(a) - np.asarray(b)).max() / max(np.abs(np.asarray(b)).max(), 1e-300)
+
+
+def _gemm(cplx, opa, opb, M, N, K, rs):
+    L = engine.lib()
+    P = C.POINTER(C.c_double)
+    L.dqmc_test_gemm.argtypes = [C.c_int] * 6 + [P, P, C.c_int, P, C.c_int, P, P, C.c_int, P, P, P]
+    dt = np.complex128 if cplx else np.float64
+
+    def rnd(*s):
+        x = rs.randn(*s)
+        return (x + 1j * rs.randn(*s)) if cplx else x
+
+    op = lambda X, o: X if o == 0 else (X.T if o == 1 else X.conj().T)
+    A = rnd(*((M, K) if opa == 0 else (K, M)))
+    B = rnd(*((K, N) if opb == 0 else (N, K)))
+    Cm = rnd(M, N)
+    r, u, w = rs.rand(M) + .5, rs.rand(K) + .5, rs.rand(N) + .5
+    al = np.array([1.3, -0.2] if cplx else [1.3])
+    be = np.array([0.7, 0.1] if cplx else [0.7])
+    alc = complex(*al) if cplx else al[0]
+    bec = complex(*be) if cplx else be[0]
+    ref = alc * np.diag(r) @ op(A, opa) @ np.diag(u) @ op(B, opb) @ np.diag(w) + bec * Cm
+    Af, Bf, Cf = (np.asfortranarray(X.astype(dt)) for X in (A, B, Cm))
+    dp = lambda a: a.ctypes.data_as(P)
+    rc = L.dqmc_test_gemm(int(cplx), opa, opb, M, N, K, dp(al), dp(Af), Af.shape[0], dp(Bf),
+                          Bf.shape[0], dp(be), dp(Cf), M, dp(r), dp(u), dp(w))
+    assert rc == 0
+    return rel(Cf, ref)
+
+
+@pytest.mark.parametrize("cplx", [0, 1])
+def test_gemm_kernel(cplx):
+    rs = np.random.RandomState(0)
+    ops = (0, 1, 2) if cplx else (0, 1)
+    for (M, N, K) in [(16, 16, 16), (70, 33, 45), (256, 256, 256), (1, 130, 7)]:
+        for a in ops:
+            for b in ops:
+                assert _gemm(cplx, a, b, M, N, K, rs) < 1e-14
+
+
+def test_golden_trajectory_on_gpu(golden):
+    sim = gen.create_batch(seed=1234, n_sweep_warm=20, n_sweep_meas=20)[0]
+    eng = engine.Engine([sim])
+    try:
+        eng.run(40)
+        st = eng.get_state(0)
+        assert np.array_equal(st["hs"], np.array(golden["hs_end"]["square"]))
+        out = oracle.run(sim)
+        assert [int(v) for v in st["rng"]] == [int(v) for v in out["state"]["rng"]]
+        assert int(st["sweep"]) == 40
+        assert eng.accepts() == out["n_acc"]
+        eq, _ = eng.get_meas(0)
+        for k in ("density", "density_u", "density_d", "double_occ", "g00", "g00_u", "g00_d",
+                  "nn", "xx", "zz", "pair_sw"):
+            assert rel(eq[k], out["meas_eqlt"][k]) < 1e-9, k
+        assert eq["n_sample"] == out["meas_eqlt"]["n_sample"]
+    finally:
+        eng.close()
+
+
+@pytest.mark.parametrize("nflux", [0, 1])
+@pytest.mark.parametrize("shape", [(4, 4, 20, 5), (6, 6, 40, 8), (8, 8, 32, 8)])
+def test_calc_eq_g(nflux, shape):
+    Nx, Ny, L, nm = shape
+    sim = gen.create_batch(seed=21, Nx=Nx, Ny=Ny, nflux=nflux, U=5.0, mu=-0.4, tp=-0.2, L=L,
+                           n_matmul=nm, period_eqlt=nm)[0]
+    eng = engine.Engine([sim])
+    try:
+        F = L // nm
+        for spin in (0, 1):
+            for f in sorted({0, F - 1}):
+                G, ph = eng.calc_eq_g(0, spin, f)
+                Go, pho = oracle.calc_eq_g(sim, spin, f)
+                assert rel(G, Go) < 1e-10
+                assert abs(ph - pho) < 1e-10
+    finally:
+        eng.close()
+
+
+@pytest.mark.parametrize("nflux", [0, 1])
+def test_trajectory_and_measurements(nflux):
+    sim = gen.create_batch(seed=8, Nx=4, Ny=6, nflux=nflux, U=4.0, mu=-0.6, tp=-0.15, L=30,
+                           n_matmul=6, n_delay=5, period_eqlt=6, n_sweep_warm=3,
+                           n_sweep_meas=9, meas_energy_corr=1)[0]
+    eng = engine.Engine([sim])
+    try:
+        eng.run(12)
+        st = eng.get_state(0)
+        out = oracle.run(sim)
+        assert np.array_equal(st["hs"], out["state"]["hs"])
+        eq, _ = eng.get_meas(0)
+        for k in ("density", "double_occ", "g00", "nn", "xx", "zz", "pair_sw", "vv", "vn"):
+            assert rel(eq[k], out["meas_eqlt"][k]) < 1e-9, k
+        assert abs(complex(eq["sign"]) - complex(out["meas_eqlt"]["sign"])) < 1e-9
+        G, ph = eng.green(0, 0)
+        assert abs(abs(ph) - 1) < 1e-12
+    finally:
+        eng.close()
+
+
+@pytest.mark.parametrize("nflux", [0, 1])
+@pytest.mark.parametrize("Lnm", [(20, 10), (40, 10), (40, 8), (80, 10)])
+def test_unequal_time(nflux, Lnm):
+    L, nm = Lnm  # E = 1, 2, 3, 4 blocks
+    sim = gen.create_batch(seed=11, Nx=4, Ny=4, nflux=nflux, U=4.0, mu=-0.3, L=L, n_matmul=nm,
+                           period_eqlt=nm, period_uneqlt=1, n_sweep_warm=2, n_sweep_meas=4)[0]
+    eng = engine.Engine([sim])
+    try:
+        for spin in (0, 1):
+            for a, b in zip(eng.calc_ue_g(0, spin), oracle.calc_ue_g(sim, spin)):
+                assert rel(a, b) < 1e-8
+        eng.run(6)
+        out = oracle.run(sim)
+        st = eng.get_state(0)
+        assert np.array_equal(st["hs"], out["state"]["hs"])
+        _, ue = eng.get_meas(0)
+        for k in ("gt0", "gt0_u", "gt0_d", "nn", "xx", "zz", "pair_sw"):
+            assert rel(ue[k], out["meas_uneqlt"][k]) < 1e-8, k
+        assert ue["n_sample"] == out["meas_uneqlt"]["n_sample"] == 4
+    finally:
+        eng.close()
+
+
+def test_batched_chains_match_individual_oracle_runs():
+    sims = gen.create_batch(Nfiles=5, seed=99, Nx=4, Ny=4, U=6.0, mu=-0.5, L=24, n_matmul=6,
+                            period_eqlt=6, period_uneqlt=2, n_sweep_warm=1, n_sweep_meas=4)
+    # uneven physical parameters inside one shape bucket
+    sims[2] = gen.create_batch(Nfiles=1, seed=7, Nx=4, Ny=4, U=2.0, mu=-1.2, L=24, n_matmul=6,
+                               period_eqlt=6, period_uneqlt=2, n_sweep_warm=1, n_sweep_meas=4)[0]
+    eng = engine.Engine(sims)
+    try:
+        eng.run(5)
+        for c, s in enumerate(sims):
+            out = oracle.run(s)
+            st = eng.get_state(c)
+            assert np.array_equal(st["hs"], out["state"]["hs"]), c
+            eq, ue = eng.get_meas(c)
+            assert rel(eq["nn"], out["meas_eqlt"]["nn"]) < 1e-9
+            assert rel(ue["zz"], out["meas_uneqlt"]["zz"]) < 1e-8
+    finally:
+        eng.close()
+
+
+def test_full_size_green_consistency():
+    """N=256, L=80 (the benchmark shape): the engine's G after a sweep of
+    wraps/recalcs equals G recomputed from scratch by the stabilized product
+    (CHECK_G_WRP of src/dqmc.c:277-287), and the GPU's from-scratch G equals
+    the oracle's."""
+    sim = gen.create_1(init_rng=gen.splitmix64_seed(7), Nx=16, Ny=16, tp=-0.25, U=6.0, mu=-1.0,
+                       dt=0.1, L=80, n_matmul=10, n_delay=32, period_eqlt=8, n_sweep_warm=0,
+                       n_sweep_meas=10)
+    eng = engine.Engine([sim])
+    try:
+        eng.run(1)
+        G_run, ph_run = eng.green(0, 0)
+        st = eng.get_state(0)
+        s2 = dict(sim, state=dict(sim["state"], hs=st["hs"]))
+        G0, ph0 = eng.calc_eq_g(0, 0, 0)
+        assert rel(G_run, G0) < 1e-9
+        Go, pho = oracle.calc_eq_g(s2, 0, 0)
+        assert rel(G0, Go) < 1e-10
+        assert abs(ph0 - pho) < 1e-10
+    finally:
+        eng.close()
