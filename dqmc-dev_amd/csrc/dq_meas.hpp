@@ -21,18 +21,20 @@ struct MeasArgs {
     const int* flag;                  // [c]: measure this chain now
     const int* rp_i; const int* ix_i; long srp_i, six_i;     // CSR of map_i  [c]
     const int* rp_ij; const int* ix_ij; long srp_ij, six_ij; // CSR of map_ij [c]
-    const int* degen;                 // [c][2]: degen_i, degen_ij ; [c][3] degen_bb
+    const int* degen;                 // [c][4]: degen_i, degen_ij, degen_bb, degen_bs
     const T* peierlsu; const T* peierlsd; long sP; // [c][N*N] (complex only)
     T* acc; long sacc;                // accumulator block of chain c
     const long* off;                  // offsets of each accumulator in the block (-1: off)
     int* n_sample; T* sign;           // [c]
     const int* bonds; long sbonds;    // [c][2*num_b]
     const int* rp_bb; const int* ix_bb; long srp_bb, six_bb;  // CSR of map_bb
+    const int* rp_bs; const int* ix_bs; long srp_bs, six_bs;  // CSR of map_bs
+    int num_bs;
 };
 
 // accumulator indices (match enum dqmc_eq_acc / dqmc_ue_acc)
 enum { EQ_density = 0, EQ_density_u, EQ_density_d, EQ_double_occ, EQ_g00, EQ_g00_u, EQ_g00_d,
-       EQ_nn, EQ_xx, EQ_zz, EQ_pair_sw, EQ_vv = 19, EQ_vn = 20 };
+       EQ_nn, EQ_xx, EQ_zz, EQ_pair_sw, EQ_kk = 16, EQ_kv = 17, EQ_kn = 18, EQ_vv = 19, EQ_vn = 20 };
 enum { UE_gt0 = 0, UE_gt0_u, UE_gt0_d, UE_nn, UE_xx, UE_zz, UE_pair_sw, UE_pair_bb = 11, UE_jj,
        UE_jsjs, UE_kk, UE_ksks, UE_vv = 30, UE_vn = 31 };
 
@@ -54,7 +56,7 @@ __global__ __launch_bounds__(256) void k_meas_eq(MeasArgs<T> a)
     const T phase = a.phase[c];
     T* acc = a.acc + c * a.sacc;
     const long* off = a.off;
-    const int degen_i = a.degen[3 * c], degen_ij = a.degen[3 * c + 1];
+    const int degen_i = a.degen[4 * c], degen_ij = a.degen[4 * c + 1];
     const T* pu = a.peierlsu + c * a.sP;
     const T* pd = a.peierlsd + c * a.sP;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -143,7 +145,7 @@ __global__ __launch_bounds__(256) void k_meas_ue(MeasArgs<T> a)
         a.sign[c] += phase;
     }
     if (r >= a.num_ij) return;
-    const int degen_ij = a.degen[3 * c + 1];
+    const int degen_ij = a.degen[4 * c + 1];
     const T* pu = a.peierlsu + c * a.sP;
     const T* pd = a.peierlsd + c * a.sP;
     const T* Gu0t = a.G0t + (2 * c) * a.sU + NN * tt;
@@ -194,4 +196,174 @@ __global__ __launch_bounds__(256) void k_meas_ue(MeasArgs<T> a)
     acc[off[UE_nn] + o] = nn; acc[off[UE_xx] + o] = xx; acc[off[UE_zz] + o] = zz;
     acc[off[UE_pair_sw] + o] = psw;
     if (a.energy) { acc[off[UE_vv] + o] = vv; acc[off[UE_vn] + o] = vn; }
+}
+
+// Peierls factor p[i + N j] (complex builds; 1 in real builds)
+template <typename T>
+__device__ inline T pf(const T* P, long idx);
+template <>
+__device__ inline double pf<double>(const double*, long) { return 1.0; }
+template <>
+__device__ inline zd pf<zd>(const zd* P, long idx) { return P[idx]; }
+
+// equal time, meas_energy_corr bond terms: kv, kn (1 bond - 1 site,
+// src/meas.c:290-330) and kk (1 bond - 1 bond, src/meas.c:333-406)
+template <typename T>
+__global__ __launch_bounds__(256) void k_meas_eq_energy(MeasArgs<T> a)
+{
+    const int c = blockIdx.y;
+    if (!a.flag[c]) return;
+    const int N = a.N, nb = a.num_b;
+    const T* gu = a.G + (2 * c) * a.sG;
+    const T* gd = a.G + (2 * c + 1) * a.sG;
+    const T phase = a.phase[c];
+    T* acc = a.acc + c * a.sacc;
+    const long* off = a.off;
+    const T* pu = a.peierlsu + c * a.sP;
+    const T* pd = a.peierlsd + c * a.sP;
+    const int* bonds = a.bonds + c * a.sbonds;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const T one = from_re<T>(1.0), two = from_re<T>(2.0);
+    if (t < a.num_bs) {
+        const int* rp = a.rp_bs + c * a.srp_bs;
+        const int* ix = a.ix_bs + c * a.six_bs;
+        const T pre = phase * (1.0 / a.degen[4 * c + 3]);
+        T kv = acc[off[EQ_kv] + t], kn = acc[off[EQ_kn] + t];
+        for (int q = rp[t]; q < rp[t + 1]; q++) {
+            const int lin = ix[q];
+            const int b = lin % nb, j = lin / nb;
+            const int i0 = bonds[b], i1 = bonds[b + nb];
+            const T pui0i1 = pf(pu, i0 + (long)N * i1), pui1i0 = pf(pu, i1 + (long)N * i0);
+            const T pdi0i1 = pf(pd, i0 + (long)N * i1), pdi1i0 = pf(pd, i1 + (long)N * i0);
+            const double d0j = (i0 == j) ? 1.0 : 0.0, d1j = (i1 == j) ? 1.0 : 0.0;
+            const T gui0j = gu[i0 + (long)N * j], guji0 = gu[j + (long)N * i0];
+            const T gdi0j = gd[i0 + (long)N * j], gdji0 = gd[j + (long)N * i0];
+            const T gui1j = gu[i1 + (long)N * j], guji1 = gu[j + (long)N * i1];
+            const T gdi1j = gd[i1 + (long)N * j], gdji1 = gd[j + (long)N * i1];
+            const T gui0i1 = gu[i0 + (long)N * i1], gui1i0 = gu[i1 + (long)N * i0];
+            const T gdi0i1 = gd[i0 + (long)N * i1], gdi1i0 = gd[i1 + (long)N * i0];
+            const T gujj = gu[j + (long)N * j], gdjj = gd[j + (long)N * j];
+            const T ku = pui1i0 * (0.0 - gui0i1) + pui0i1 * (0.0 - gui1i0);
+            const T kd = pdi1i0 * (0.0 - gdi0i1) + pdi0i1 * (0.0 - gdi1i0);
+            const T xu = pui0i1 * (d0j - guji0) * gui1j + pui1i0 * (d1j - guji1) * gui0j;
+            const T xd = pdi0i1 * (d0j - gdji0) * gdi1j + pdi1i0 * (d1j - gdji1) * gdi0j;
+            kv += pre * ((ku * (one - gujj) + xu) * (one - gdjj) + (kd * (one - gdjj) + xd) * (one - gujj));
+            kn += pre * ((ku + kd) * (two - gujj - gdjj) + xu + xd);
+        }
+        acc[off[EQ_kv] + t] = kv;
+        acc[off[EQ_kn] + t] = kn;
+    }
+    if (t < a.num_bb) {
+        const int* rp = a.rp_bb + c * a.srp_bb;
+        const int* ix = a.ix_bb + c * a.six_bb;
+        const T pre = phase * (1.0 / a.degen[4 * c + 2]);
+        T kk = acc[off[EQ_kk] + t];
+        for (int q = rp[t]; q < rp[t + 1]; q++) {
+            const int lin = ix[q];
+            const int b = lin % nb, cc = lin / nb;
+            const int j0 = bonds[cc], j1 = bonds[cc + nb];
+            const int i0 = bonds[b], i1 = bonds[b + nb];
+            const T puj0j1 = pf(pu, j0 + (long)N * j1), puj1j0 = pf(pu, j1 + (long)N * j0);
+            const T pdj0j1 = pf(pd, j0 + (long)N * j1), pdj1j0 = pf(pd, j1 + (long)N * j0);
+            const T pui0i1 = pf(pu, i0 + (long)N * i1), pui1i0 = pf(pu, i1 + (long)N * i0);
+            const T pdi0i1 = pf(pd, i0 + (long)N * i1), pdi1i0 = pf(pd, i1 + (long)N * i0);
+            const double d00 = (i0 == j0), d10 = (i1 == j0), d01 = (i0 == j1), d11 = (i1 == j1);
+#define GU(x, y) gu[(x) + (long)N * (y)]
+#define GD(x, y) gd[(x) + (long)N * (y)]
+            const T x = pui0i1 * puj0j1 * (d01 - GU(j1, i0)) * GU(i1, j0) +
+                        pui1i0 * puj1j0 * (d10 - GU(j0, i1)) * GU(i0, j1) +
+                        pdi0i1 * pdj0j1 * (d01 - GD(j1, i0)) * GD(i1, j0) +
+                        pdi1i0 * pdj1j0 * (d10 - GD(j0, i1)) * GD(i0, j1);
+            const T y = pui0i1 * puj1j0 * (d00 - GU(j0, i0)) * GU(i1, j1) +
+                        pui1i0 * puj0j1 * (d11 - GU(j1, i1)) * GU(i0, j0) +
+                        pdi0i1 * pdj1j0 * (d00 - GD(j0, i0)) * GD(i1, j1) +
+                        pdi1i0 * pdj0j1 * (d11 - GD(j1, i1)) * GD(i0, j0);
+            kk += pre * ((pui1i0 * GU(i0, i1) + pui0i1 * GU(i1, i0) + pdi1i0 * GD(i0, i1) +
+                          pdi0i1 * GD(i1, i0)) *
+                             (puj1j0 * GU(j0, j1) + puj0j1 * GU(j1, j0) + pdj1j0 * GD(j0, j1) +
+                              pdj0j1 * GD(j1, j0)) +
+                         x + y);
+#undef GU
+#undef GD
+        }
+        acc[off[EQ_kk] + t] = kk;
+    }
+}
+
+// unequal time, meas_bond_corr: pair_bb, jj, jsjs, kk, ksks
+// (src/meas.c:860-955), one thread per (bin, t)
+template <typename T>
+__global__ __launch_bounds__(256) void k_meas_ue_bond(MeasArgs<T> a)
+{
+    const int c = blockIdx.z;
+    if (!a.flag[c]) return;
+    const int N = a.N, nb = a.num_b;
+    const long NN = (long)N * N;
+    const int tt = blockIdx.y;
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= a.num_bb) return;
+    const T phase = a.phase[c];
+    T* acc = a.acc + c * a.sacc;
+    const long* off = a.off;
+    const T* pu = a.peierlsu + c * a.sP;
+    const T* pd = a.peierlsd + c * a.sP;
+    const int* bonds = a.bonds + c * a.sbonds;
+    const T* Gu0t = a.G0t + (2 * c) * a.sU + NN * tt;
+    const T* Gutt = a.Gtt + (2 * c) * a.sU + NN * tt;
+    const T* Gut0 = a.Gt0 + (2 * c) * a.sU + NN * tt;
+    const T* Gd0t = a.G0t + (2 * c + 1) * a.sU + NN * tt;
+    const T* Gdtt = a.Gtt + (2 * c + 1) * a.sU + NN * tt;
+    const T* Gdt0 = a.Gt0 + (2 * c + 1) * a.sU + NN * tt;
+    const T* Gu00 = a.Gtt + (2 * c) * a.sU;
+    const T* Gd00 = a.Gtt + (2 * c + 1) * a.sU;
+    const int* rp = a.rp_bb + c * a.srp_bb;
+    const int* ix = a.ix_bb + c * a.six_bb;
+    const T pre = phase * (1.0 / a.degen[4 * c + 2]);
+    const long o = r + (long)a.num_bb * tt;
+    T pbb = acc[off[UE_pair_bb] + o], jj = acc[off[UE_jj] + o], jsjs = acc[off[UE_jsjs] + o];
+    T kk = acc[off[UE_kk] + o], ksks = acc[off[UE_ksks] + o];
+    const double dt = (tt == 0) ? 1.0 : 0.0;
+    for (int q = rp[r]; q < rp[r + 1]; q++) {
+        const int lin = ix[q];
+        const int b = lin % nb, cc = lin / nb;
+        const int j0 = bonds[cc], j1 = bonds[cc + nb];
+        const int i0 = bonds[b], i1 = bonds[b + nb];
+        const T puj0j1 = pf(pu, j0 + (long)N * j1), puj1j0 = pf(pu, j1 + (long)N * j0);
+        const T pdj0j1 = pf(pd, j0 + (long)N * j1), pdj1j0 = pf(pd, j1 + (long)N * j0);
+        const T pui0i1 = pf(pu, i0 + (long)N * i1), pui1i0 = pf(pu, i1 + (long)N * i0);
+        const T pdi0i1 = pf(pd, i0 + (long)N * i1), pdi1i0 = pf(pd, i1 + (long)N * i0);
+        const T gui1i0 = Gutt[i1 + i0 * (long)N], gui0i1 = Gutt[i0 + i1 * (long)N];
+        const T gui0j0 = Gut0[i0 + j0 * (long)N], gui1j0 = Gut0[i1 + j0 * (long)N];
+        const T gui0j1 = Gut0[i0 + j1 * (long)N], gui1j1 = Gut0[i1 + j1 * (long)N];
+        const T guj0i0 = Gu0t[j0 + i0 * (long)N], guj1i0 = Gu0t[j1 + i0 * (long)N];
+        const T guj0i1 = Gu0t[j0 + i1 * (long)N], guj1i1 = Gu0t[j1 + i1 * (long)N];
+        const T guj1j0 = Gu00[j1 + j0 * (long)N], guj0j1 = Gu00[j0 + j1 * (long)N];
+        const T gdi1i0 = Gdtt[i1 + i0 * (long)N], gdi0i1 = Gdtt[i0 + i1 * (long)N];
+        const T gdi0j0 = Gdt0[i0 + j0 * (long)N], gdi1j0 = Gdt0[i1 + j0 * (long)N];
+        const T gdi0j1 = Gdt0[i0 + j1 * (long)N], gdi1j1 = Gdt0[i1 + j1 * (long)N];
+        const T gdj0i0 = Gd0t[j0 + i0 * (long)N], gdj1i0 = Gd0t[j1 + i0 * (long)N];
+        const T gdj0i1 = Gd0t[j0 + i1 * (long)N], gdj1i1 = Gd0t[j1 + i1 * (long)N];
+        const T gdj1j0 = Gd00[j1 + j0 * (long)N], gdj0j1 = Gd00[j0 + j1 * (long)N];
+        const double d00 = dt * (i0 == j0), d10 = dt * (i1 == j0);
+        const double d01 = dt * (i0 == j1), d11 = dt * (i1 == j1);
+        const T arr0 = (d01 - gdj1i0) * gdi1j0, arr1 = (d00 - gdj0i0) * gdi1j1;
+        const T arr2 = (d11 - gdj1i1) * gdi0j0, arr3 = (d10 - gdj0i1) * gdi0j1;
+        const T arr4 = (d01 - guj1i0) * gui1j0, arr5 = (d00 - guj0i0) * gui1j1;
+        const T arr6 = (d11 - guj1i1) * gui0j0, arr7 = (d10 - guj0i1) * gui0j1;
+        const T p0 = pdi0i1 * pdj0j1 * arr0, p1 = pdi0i1 * pdj1j0 * arr1;
+        const T p2 = pdi1i0 * pdj0j1 * arr2, p3 = pdi1i0 * pdj1j0 * arr3;
+        const T p4 = pui0i1 * puj0j1 * arr4, p5 = pui0i1 * puj1j0 * arr5;
+        const T p6 = pui1i0 * puj0j1 * arr6, p7 = pui1i0 * puj1j0 * arr7;
+        pbb += 0.5 * (pre * (gui0j0 * gdi1j1 + gui1j0 * gdi0j1 + gui0j1 * gdi1j0 + gui1j1 * gdi0j0));
+        const T x = p0 + p3 + p4 + p7;
+        const T y = p1 + p2 + p5 + p6;
+        const T t0 = pui1i0 * gui0i1, t1 = pui0i1 * gui1i0, t2 = pdi1i0 * gdi0i1, t3 = pdi0i1 * gdi1i0;
+        const T t4 = puj1j0 * guj0j1, t5 = puj0j1 * guj1j0, t6 = pdj1j0 * gdj0j1, t7 = pdj0j1 * gdj1j0;
+        jj += pre * ((t0 - t1 + t2 - t3) * (t4 - t5 + t6 - t7) + x - y);
+        jsjs += pre * ((t0 - t1 - t2 + t3) * (t4 - t5 - t6 + t7) + x - y);
+        kk += pre * ((t0 + t1 + t2 + t3) * (t4 + t5 + t6 + t7) + x + y);
+        ksks += pre * ((t0 + t1 - t2 - t3) * (t4 + t5 - t6 - t7) + x + y);
+    }
+    acc[off[UE_pair_bb] + o] = pbb; acc[off[UE_jj] + o] = jj; acc[off[UE_jsjs] + o] = jsjs;
+    acc[off[UE_kk] + o] = kk; acc[off[UE_ksks] + o] = ksks;
 }

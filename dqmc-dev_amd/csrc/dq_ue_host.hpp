@@ -323,6 +323,10 @@ static int ue_measure(dqmc_engine* e)
     ma.sU = (long)e->L * e->N * e->N;
     k_meas_ue<T><<<dim3((e->num_ij + 255) / 256, e->L, e->nc), 256, 0, e->st>>>(ma);
     LCHK();
+    if (e->bond) {
+        k_meas_ue_bond<T><<<dim3((e->num_bb + 255) / 256, e->L, e->nc), 256, 0, e->st>>>(ma);
+        LCHK();
+    }
     return 0;
 }
 

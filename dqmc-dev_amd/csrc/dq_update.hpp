@@ -2,9 +2,9 @@
 // rank-k Green's-function updates (src/updates.c:7-84), the RNG
 // (src/rand.h:6-32) and the HS-field scalings of B_l (src/dqmc.c:29-59).
 //
-// One workgroup per chain handles both spins, because the acceptance test
-// couples them (u < |r_up * r_dn|).  Lane 0 of wave 0 owns the xorshift1024*
-// state (in LDS), shuffles the site order and scans sites until the next
+// General-N fallback (N > 512; dq_update2.hpp covers N <= 512).  One workgroup
+// per chain handles both spins, because the acceptance test couples them
+// (u < |r_up * r_dn|).  Lane 0 scans the precomputed site order until the next
 // acceptance without any workgroup barrier; the whole workgroup then builds
 // the new delay columns a_k, b_k for both spins (thread j owns row j), and
 // flushes G += A B^T every `kd` acceptances and at the end of the slice.
@@ -40,7 +40,8 @@ struct UpdArgs {
     const double* del;              // [c][2N]
     const double* exp_lambda;       // [c][2N]
     int* hs;                        // [c][L*N]
-    uint64_t* rng;                  // [c][17]
+    const int* order;               // [c][L][N] site order (dq_rng.hpp)
+    const double* unif;             // [c][L][N] uniform per visited site
     T* phase;                       // [c]
     double* escale; double* iscale; // [b][L*N]
     unsigned long long* n_acc;      // [c]
@@ -66,7 +67,6 @@ __global__ __launch_bounds__(UPD_THREADS) void k_update(UpdArgs<T> a)
 {
     __shared__ T du[UPD_MAXN], dd[UPD_MAXN];
     __shared__ int order[UPD_MAXN];
-    __shared__ uint64_t rng[17];
     __shared__ int s_site;
     __shared__ T s_fu, s_fd;
     const int c = blockIdx.x;
@@ -80,20 +80,14 @@ __global__ __launch_bounds__(UPD_THREADS) void k_update(UpdArgs<T> a)
     T* Bd = a.Bb + (2 * c + 1) * a.sAB;
     const double* del = a.del + (long)c * 2 * N;
     int* hs = a.hs + (long)c * a.L * N + (long)N * a.l;
-    if (tid < 17) rng[tid] = a.rng[c * 17 + tid];
+    const long ol = ((long)c * a.L + a.l) * N;
+    const double* unif = a.unif + ol;
     for (int j = tid; j < N; j += blockDim.x) {
         du[j] = Gu[j + (long)N * j];
         dd[j] = Gd[j + (long)N * j];
+        order[j] = a.order[ol + j];
     }
     __syncthreads();
-    // site order: inside-out Fisher-Yates (src/rand.h:25-32)
-    if (tid == 0) {
-        for (int i = 0; i < N; i++) {
-            const int j = (int)((xs_next(rng) >> 3) % (uint64_t)(i + 1));
-            if (i != j) order[i] = order[j];
-            order[j] = i;
-        }
-    }
     T phase = from_re<T>(0.0);
     unsigned long long nacc = 0;
     if (tid == 0) phase = a.phase[c];
@@ -112,7 +106,7 @@ __global__ __launch_bounds__(UPD_THREADS) void k_update(UpdArgs<T> a)
                 const T rd = from_re<T>(1.0) + (from_re<T>(1.0) - dd[i]) * deld;
                 const T prob = ru * rd;
                 const double absprob = absv(prob);
-                if (xs_doub(rng) < absprob) {
+                if (unif[ii] < absprob) {
                     site = i;
                     s_fu = rdiv(from_re<T>(delu), ru);
                     s_fd = rdiv(from_re<T>(deld), rd);
@@ -174,7 +168,6 @@ __global__ __launch_bounds__(UPD_THREADS) void k_update(UpdArgs<T> a)
         isu[j] = el[j + N * (1 - sv)];
         isd[j] = el[j + N * sv];
     }
-    if (tid < 17) a.rng[c * 17 + tid] = rng[tid];
     if (tid == 0) {
         a.phase[c] = phase;
         a.n_acc[c] += nacc;

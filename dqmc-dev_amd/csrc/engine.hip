@@ -23,7 +23,9 @@
 #include "dq_factor.hpp"
 #include "dq_gemm.hpp"
 #include "dq_meas.hpp"
+#include "dq_rng.hpp"
 #include "dq_update.hpp"
+#include "dq_update2.hpp"
 #include "dqmc_hip.h"
 
 static thread_local std::string g_err;
@@ -116,8 +118,15 @@ extern "C" long dqmc_acc_len(const dqmc_params* p, int ue, int k)
 }
 
 // accumulators computed on the GPU by this version
-static bool eq_supported(int k) { return k <= DQ_EQ_pair_sw || k == DQ_EQ_vv || k == DQ_EQ_vn; }
-static bool ue_supported(int k) { return k <= DQ_UE_pair_sw || k == DQ_UE_vv || k == DQ_UE_vn; }
+static bool eq_supported(int k)
+{
+    return k <= DQ_EQ_pair_sw || (k >= DQ_EQ_kk && k <= DQ_EQ_vn);
+}
+static bool ue_supported(int k)
+{
+    return k <= DQ_UE_pair_sw || k == DQ_UE_vv || k == DQ_UE_vn ||
+           (k >= DQ_UE_pair_bb && k <= DQ_UE_ksks);
+}
 
 // --------------------------------------------------------------------------
 // engine state
@@ -143,6 +152,9 @@ struct dqmc_engine {
     double *dvec = 0, *vvec = 0;                            // [B][N]
     void *phb = 0, *phb2 = 0, *phase = 0;                   // [B], [B], [nc] T
     unsigned long long* nacc = 0;                           // [nc]
+    uint64_t* raw = 0;                                      // [nc][2 L N] draws of a sweep
+    int *order = 0, *draws = 0;                             // [nc][L][N], [nc]
+    double* unif = 0;                                       // [nc][L][N]
     int *active = 0, *flag_eq = 0, *flag_ue = 0;            // [nc]
     int *d_sweep = 0, *d_nsweep = 0, *d_warm = 0;           // [nc]
     long eq_off[DQMC_MAX_ACC], ue_off[DQMC_MAX_ACC];
@@ -152,6 +164,8 @@ struct dqmc_engine {
     void *acc_eq = 0, *acc_ue = 0, *sign_eq = 0, *sign_ue = 0;
     int *ns_eq = 0, *ns_ue = 0;
     int *rp_i = 0, *ix_i = 0, *rp_ij = 0, *ix_ij = 0, *degen = 0;
+    int *rp_bb = 0, *ix_bb = 0, *rp_bs = 0, *ix_bs = 0, *bonds = 0;
+    int num_b = 0, num_bb = 0, num_bs = 0, bond = 0;
     void *peierlsu = 0, *peierlsd = 0;
     // unequal time
     void *G0t = 0, *Gtt = 0, *Gt0 = 0;                      // [B][L][NN]
@@ -427,6 +441,10 @@ static MeasArgs<T> meas_args(dqmc_engine* e, int ue)
     a.rp_i = e->rp_i; a.ix_i = e->ix_i; a.srp_i = e->num_i + 1; a.six_i = e->N;
     a.rp_ij = e->rp_ij; a.ix_ij = e->ix_ij; a.srp_ij = e->num_ij + 1; a.six_ij = (long)e->N * e->N;
     a.degen = e->degen;
+    a.num_b = e->num_b; a.num_bb = e->num_bb; a.num_bs = e->num_bs; a.bond = e->bond;
+    a.bonds = e->bonds; a.sbonds = 2L * e->num_b;
+    a.rp_bb = e->rp_bb; a.ix_bb = e->ix_bb; a.srp_bb = e->num_bb + 1; a.six_bb = (long)e->num_b * e->num_b;
+    a.rp_bs = e->rp_bs; a.ix_bs = e->ix_bs; a.srp_bs = e->num_bs + 1; a.six_bs = (long)e->num_b * e->N;
     a.peierlsu = dp<T>(e->peierlsu); a.peierlsd = dp<T>(e->peierlsd);
     a.sP = is_cplx<T>::value ? (long)e->N * e->N : 0;
     if (ue) {
@@ -441,6 +459,41 @@ static MeasArgs<T> meas_args(dqmc_engine* e, int ue)
 
 // unequal-time Green's functions and measurements
 #include "dq_ue_host.hpp"
+
+// Metropolis sweep of slice l for every chain; the delay rank is the register
+// capacity of the kernel variant (real: 32 slots for N <= 256, 16 for N <= 512;
+// complex: half of that).
+template <typename T>
+static int launch_update(dqmc_engine* e, int l)
+{
+    const int N = e->N, nc = e->nc;
+    const long NN = (long)N * N;
+    if (N <= UPD2_MAXN) {
+        Upd2Args<T> a;
+        a.N = N; a.L = e->L; a.l = l;
+        a.G = dp<T>(e->G); a.sG = NN;
+        a.del = e->del; a.exp_lambda = e->exp_lambda; a.hs = e->hs;
+        a.order = e->order; a.unif = e->unif;
+        a.phase = dp<T>(e->phase); a.escale = e->escale; a.iscale = e->iscale;
+        a.n_acc = e->nacc; a.active = e->active;
+        constexpr bool Z = is_cplx<T>::value;
+        if (N <= 256) k_update_reg<T, 1, Z ? 16 : 32><<<nc, UPD2_THREADS, 0, e->st>>>(a);
+        else k_update_reg<T, 2, Z ? 8 : 16><<<nc, UPD2_THREADS, 0, e->st>>>(a);
+        LCHK();
+        return 0;
+    }
+    UpdArgs<T> ua;
+    ua.N = N; ua.L = e->L; ua.l = l; ua.kd = e->kd;
+    ua.G = dp<T>(e->G); ua.sG = NN;
+    ua.Ab = dp<T>(e->Ab); ua.Bb = dp<T>(e->Bb); ua.sAB = (long)N * e->kd;
+    ua.del = e->del; ua.exp_lambda = e->exp_lambda; ua.hs = e->hs;
+    ua.order = e->order; ua.unif = e->unif;
+    ua.phase = dp<T>(e->phase); ua.escale = e->escale; ua.iscale = e->iscale;
+    ua.n_acc = e->nacc; ua.active = e->active;
+    k_update<T><<<nc, UPD_THREADS, 0, e->st>>>(ua);
+    LCHK();
+    return 0;
+}
 
 template <typename T>
 static int run_t(dqmc_engine* e, int n_sweeps, const volatile int* stop)
@@ -469,16 +522,14 @@ static int run_t(dqmc_engine* e, int n_sweeps, const volatile int* stop)
                                                             e->d_nsweep, e->d_warm, e->active,
                                                             e->flag_eq, e->flag_ue);
         LCHK();
+        // the sweep's random stream: site orders and uniforms (dq_rng.hpp)
+        k_rng_raw<<<nc, 64, 0, e->st>>>(N, L, e->draws, e->rng, e->raw, 2L * L * N, e->active);
+        LCHK();
+        k_rng_orders<<<nc, 256, 0, e->st>>>(N, L, e->raw, 2L * L * N, e->del, e->draws, e->order,
+                                            e->unif, e->active);
+        LCHK();
         for (int l = 0; l < L; l++) {
-            UpdArgs<T> ua;
-            ua.N = N; ua.L = L; ua.l = l; ua.kd = e->kd;
-            ua.G = G; ua.sG = NN;
-            ua.Ab = dp<T>(e->Ab); ua.Bb = dp<T>(e->Bb); ua.sAB = (long)N * e->kd;
-            ua.del = e->del; ua.exp_lambda = e->exp_lambda; ua.hs = e->hs; ua.rng = e->rng;
-            ua.phase = dp<T>(e->phase); ua.escale = e->escale; ua.iscale = e->iscale;
-            ua.n_acc = e->nacc; ua.active = e->active;
-            k_update<T><<<nc, UPD_THREADS, 0, e->st>>>(ua);
-            LCHK();
+            RC(launch_update<T>(e, l));
             const int f = l / n;
             const bool recalc = ((l + 1) % n == 0);
             if (recalc) {
@@ -506,6 +557,11 @@ static int run_t(dqmc_engine* e, int n_sweeps, const volatile int* stop)
                 const int nb = (std::max(e->num_i, e->num_ij) + 255) / 256;
                 k_meas_eq<T><<<dim3(nb, nc), 256, 0, e->st>>>(ma);
                 LCHK();
+                if (e->energy) {
+                    const int nb2 = (std::max(e->num_bs, e->num_bb) + 255) / 256;
+                    k_meas_eq_energy<T><<<dim3(nb2, nc), 256, 0, e->st>>>(ma);
+                    LCHK();
+                }
             }
         }
         if (any_ue) RC(ue_measure<T>(e));
@@ -589,6 +645,10 @@ static int engine_setup(dqmc_engine* e, const dqmc_params* const* P)
     DALLOC(e->phb2, B * es);
     DALLOC(e->phase, nc * es);
     DALLOC(e->nacc, nc * 8);
+    DALLOC(e->raw, (long)nc * 2 * L * N * 8);
+    DALLOC(e->order, (long)nc * L * N * 4);
+    DALLOC(e->unif, (long)nc * L * N * 8);
+    DALLOC(e->draws, nc * 4);
     DALLOC(e->active, nc * 4);
     DALLOC(e->flag_eq, nc * 4);
     DALLOC(e->flag_ue, nc * 4);
@@ -619,7 +679,12 @@ static int engine_setup(dqmc_engine* e, const dqmc_params* const* P)
     DALLOC(e->ix_i, (long)nc * N * 4);
     DALLOC(e->rp_ij, (long)nc * (e->num_ij + 1) * 4);
     DALLOC(e->ix_ij, (long)nc * NN * 4);
-    DALLOC(e->degen, nc * 3 * 4);
+    DALLOC(e->degen, nc * 4 * 4);
+    DALLOC(e->bonds, (long)nc * 2 * (e->num_b > 0 ? e->num_b : 1) * 4);
+    DALLOC(e->rp_bb, (long)nc * (e->num_bb + 1) * 4);
+    DALLOC(e->ix_bb, (long)nc * ((long)e->num_b * e->num_b > 0 ? (long)e->num_b * e->num_b : 1) * 4);
+    DALLOC(e->rp_bs, (long)nc * (e->num_bs + 1) * 4);
+    DALLOC(e->ix_bs, (long)nc * ((long)e->num_b * N > 0 ? (long)e->num_b * N : 1) * 4);
     DALLOC(e->peierlsu, (e->cplx ? nc * NN : 1) * es);
     DALLOC(e->peierlsd, (e->cplx ? nc * NN : 1) * es);
     if (e->period_uneqlt > 0) {
@@ -630,7 +695,7 @@ static int engine_setup(dqmc_engine* e, const dqmc_params* const* P)
         RC(ue_alloc(e));
     }
     // per-chain parameters
-    std::vector<int> rp, ix, deg(3 * nc);
+    std::vector<int> rp, ix, deg(4 * nc);
     for (int c = 0; c < nc; c++) {
         const dqmc_params* p = P[c];
         const size_t mb = NN * es;
@@ -654,13 +719,32 @@ static int engine_setup(dqmc_engine* e, const dqmc_params* const* P)
         csr_of_map(p->map_ij, (int)NN, e->num_ij, rp, ix);
         HIPCHK(hipMemcpy(e->rp_ij + c * (e->num_ij + 1L), rp.data(), rp.size() * 4, hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(e->ix_ij + c * NN, ix.data(), ix.size() * 4, hipMemcpyHostToDevice));
-        deg[3 * c] = p->degen_i;
-        deg[3 * c + 1] = p->degen_ij;
-        deg[3 * c + 2] = p->degen_bb;
+        if (e->num_b > 0) {
+            const long nb = e->num_b;
+            HIPCHK(hipMemcpy(e->bonds + c * 2 * nb, p->bonds, 2 * nb * 4, hipMemcpyHostToDevice));
+            csr_of_map(p->map_bb, (int)(nb * nb), e->num_bb, rp, ix);
+            HIPCHK(hipMemcpy(e->rp_bb + c * (e->num_bb + 1L), rp.data(), rp.size() * 4, hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(e->ix_bb + c * nb * nb, ix.data(), ix.size() * 4, hipMemcpyHostToDevice));
+            csr_of_map(p->map_bs, (int)(nb * N), e->num_bs, rp, ix);
+            HIPCHK(hipMemcpy(e->rp_bs + c * (e->num_bs + 1L), rp.data(), rp.size() * 4, hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(e->ix_bs + c * nb * N, ix.data(), ix.size() * 4, hipMemcpyHostToDevice));
+        }
+        deg[4 * c] = p->degen_i;
+        deg[4 * c + 1] = p->degen_ij;
+        deg[4 * c + 2] = p->degen_bb;
+        deg[4 * c + 3] = p->degen_bs;
         e->n_sweep[c] = p->n_sweep;
         e->n_sweep_warm[c] = p->n_sweep_warm;
     }
     HIPCHK(hipMemcpy(e->degen, deg.data(), deg.size() * 4, hipMemcpyHostToDevice));
+    // sites that consume a uniform (Delta's not both zero, src/updates.c:21)
+    std::vector<int> nd(nc);
+    for (int c = 0; c < nc; c++) {
+        int n = 0;
+        for (int i = 0; i < N; i++) n += !(P[c]->del[i] == 0.0 && P[c]->del[i + N] == 0.0);
+        nd[c] = n;
+    }
+    HIPCHK(hipMemcpy(e->draws, nd.data(), nc * 4, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(e->d_nsweep, e->n_sweep.data(), nc * 4, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(e->d_warm, e->n_sweep_warm.data(), nc * 4, hipMemcpyHostToDevice));
     HIPCHK(hipStreamSynchronize(e->st));
@@ -692,6 +776,8 @@ extern "C" int dqmc_engine_create(dqmc_engine** out, const dqmc_params* const* P
     e->period_eqlt = p0->period_eqlt; e->period_uneqlt = p0->period_uneqlt;
     e->energy = p0->meas_energy_corr;
     e->num_i = p0->num_i; e->num_ij = p0->num_ij;
+    e->num_b = p0->num_b; e->num_bb = p0->num_bb; e->num_bs = p0->num_bs;
+    e->bond = p0->meas_bond_corr;
     e->esz = e->cplx ? 16 : 8;
     e->kd = (tune && tune->delay_rank > 0) ? tune->delay_rank : (p0->n_delay > 0 ? p0->n_delay : 16);
     if (e->kd > e->N) e->kd = e->N;

@@ -1117,3 +1117,20 @@ void SYM(rand_uint)(uint64_t *rng, int n, uint64_t *out)
 }
 
 void SYM(shuffle)(uint64_t *rng, int n, int *a) { shuffle(rng, n, a); }
+
+/* CHECK_G_ACC of src/dqmc.c:283-286: G at slice boundary l from the B's with a
+ * QR after every single slice (calc_eq_g(l, N, L, 1, B, ...)) -- the most
+ * stable variant of the stabilized product, used to referee accuracy. */
+int SYM(calc_eq_g_fine)(const dqmc_params *p, const int *hs, int spin, int l, double *G,
+                        double *phase)
+{
+    ws_t w;
+    if (ws_alloc(&w, p, 0)) { ws_free(&w); return -1; }
+    build_BC(p, hs, &w, spin);
+    num ph = calc_eq_g(l, p->N, p->L, 1, w.B[spin], w.g[spin], w.NN1[spin], w.NN2[spin],
+                       w.N1[spin], w.N2[spin], w.N3[spin], w.pvt[spin], w.work[spin], w.lwork);
+    memcpy(G, w.g[spin], (size_t)p->N * p->N * sizeof(num));
+    store_phase(ph, phase);
+    ws_free(&w);
+    return 0;
+}

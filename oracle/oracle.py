@@ -40,6 +40,10 @@ def lib(cplx):
         f.argtypes = [P(capi.DqmcParams), P(C.c_int32), C.c_int, C.c_int, P(C.c_double),
                       P(C.c_double)]
         f.restype = C.c_int
+        f = getattr(L, pre + "calc_eq_g_fine")
+        f.argtypes = [P(capi.DqmcParams), P(C.c_int32), C.c_int, C.c_int, P(C.c_double),
+                      P(C.c_double)]
+        f.restype = C.c_int
         f = getattr(L, pre + "calc_ue_g")
         f.argtypes = [P(capi.DqmcParams), P(C.c_int32), C.c_int, P(C.c_double), P(C.c_double),
                       P(C.c_double)]
@@ -123,6 +127,19 @@ def calc_eq_g(sim, spin, f=0, hs=None):
     ph = np.zeros(2)
     _fn(cplx, "calc_eq_g")(C.byref(ps), hs.ctypes.data_as(C.POINTER(C.c_int32)), spin, f, _dp(G),
                            _dp(ph))
+    return G.T.copy(), (complex(ph[0], ph[1]) if cplx else ph[0])
+
+
+def calc_eq_g_fine(sim, spin, l=0, hs=None):
+    """G at slice boundary l with a QR after every slice (CHECK_G_ACC)."""
+    cplx = capi.is_complex(sim)
+    ps, keep = capi.make_params(sim["params"])
+    N = int(sim["params"]["N"])
+    hs = np.ascontiguousarray(sim["state"]["hs"] if hs is None else hs, dtype=np.int32)
+    G = np.zeros((N, N), dtype=np.complex128 if cplx else np.float64)
+    ph = np.zeros(2)
+    _fn(cplx, "calc_eq_g_fine")(C.byref(ps), hs.ctypes.data_as(C.POINTER(C.c_int32)), spin, l,
+                                _dp(G), _dp(ph))
     return G.T.copy(), (complex(ph[0], ph[1]) if cplx else ph[0])
 
 
