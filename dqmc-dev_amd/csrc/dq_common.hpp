@@ -80,4 +80,5 @@ __device__ inline zd wave_sum(zd v)
     return v;
 }
 __device__ inline double shfl(double v, int src) { return __shfl(v, src, 64); }
+__device__ inline int shfl(int v, int src) { return __shfl(v, src, 64); }
 __device__ inline zd shfl(zd v, int src) { return mkz(__shfl(v.x, src, 64), __shfl(v.y, src, 64)); }

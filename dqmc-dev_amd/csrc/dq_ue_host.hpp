@@ -31,7 +31,7 @@ static int ue_alloc(dqmc_engine* e)
     DALLOC(e->Wa, B * sU2 * es);
     DALLOC(e->Wb, B * sU2 * es);
     DALLOC(e->Qb, B * (E > 1 ? E - 1 : 1) * 4 * N * N * es);
-    DALLOC(e->taub, B * NE * es);
+    DALLOC(e->taub, B * (NE + N) * es);
     DALLOC(e->T2, B * (long)e->L * N * N * es);
     return 0;
 }
@@ -59,7 +59,7 @@ static int trtri_upper(dqmc_engine* e, int n, T* X, long sX, int ldx, T* W, long
 {
     k_set_identity<T><<<nblk((long)n * n * B), 256, 0, e->st>>>(n, W, sW, n, 0, B);
     LCHK();
-    RC(launch_trsm_upper<T>(e, n, n, X, sX, ldx, W, sW, n, B));
+    RC(trsm_upper_blocked<T>(e, n, n, X, ldx, sX, W, n, sW, B));
     k_store_upper<T><<<nblk((long)n * n * B), 256, 0, e->st>>>(n, W, sW, n, X, sX, ldx, B);
     LCHK();
     return 0;
@@ -113,9 +113,8 @@ static int ue_calc(dqmc_engine* e)
     } else {
         for (int l = 0; l < E - 2; l++) {
             T* Ql = Qb + (long)l * 4 * NN;
-            k_qr<T, false><<<B, QR_THREADS, 0, e->st>>>(N2, N, BLK(l, l), NL, sU2, tauB + (long)N * l,
-                                                        NE, nullptr, 0, Ql, N2, sQb);
-            LCHK();
+            RC(qr_blocked<T>(e, N2, N, BLK(l, l), NL, sU2, tauB + (long)N * l, NE, Ql, N2, sQb,
+                             nullptr, B));
             const int cols[2] = {l + 1, E - 1};
             for (int t = 0; t < 2; t++) {
                 T* P = BLK(l, cols[t]);
@@ -127,10 +126,8 @@ static int ue_calc(dqmc_engine* e)
             }
         }
         T* Qlast = Qb + (long)(E - 2) * 4 * NN;
-        k_qr<T, false><<<B, QR_THREADS, 0, e->st>>>(N2, N2, BLK(E - 2, E - 2), NL, sU2,
-                                                    tauB + (long)N * (E - 2), NE, nullptr, 0, Qlast,
-                                                    N2, sQb);
-        LCHK();
+        RC(qr_blocked<T>(e, N2, N2, BLK(E - 2, E - 2), NL, sU2, tauB + (long)N * (E - 2), NE,
+                         Qlast, N2, sQb, nullptr, B));
         // invert R
         if (E <= 2) {
             RC(trtri_upper<T>(e, NL, Gred, sU2, NL, Wa, sU2, B));

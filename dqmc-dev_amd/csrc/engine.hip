@@ -23,7 +23,9 @@
 #include "dq_factor.hpp"
 #include "dq_gemm.hpp"
 #include "dq_meas.hpp"
+#include "dq_qr_blocked.hpp"
 #include "dq_rng.hpp"
+#include "dq_ue.hpp"
 #include "dq_update.hpp"
 #include "dq_update2.hpp"
 #include "dqmc_hip.h"
@@ -171,6 +173,9 @@ struct dqmc_engine {
     void *G0t = 0, *Gtt = 0, *Gt0 = 0;                      // [B][L][NN]
     void *Gred = 0, *Wa = 0, *Wb = 0, *Qb = 0, *taub = 0, *T2 = 0;
     int gemm_adiv = 1, gemm_bdiv = 1;  // one-shot batch divisors for the next gemm()
+    // blocked QR workspace: explicit V, compact-WY T factors, GEMM scratch, dets
+    void *Vb = 0, *Tb = 0, *Wq1 = 0, *Wq2 = 0, *detq = 0, *detm = 0;
+    long sVb = 0, sTb = 0, sWq = 0;
     // GEMM timing (bench)
     bool time_gemm = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> gemm_events;
@@ -320,6 +325,110 @@ static int mul_seq_C(dqmc_engine* e, int min, int maxp1, T alpha, T* out, int ld
 }
 
 // --------------------------------------------------------------------------
+// blocked Householder QR and blocked upper-triangular solve (dq_qr_blocked.hpp)
+// --------------------------------------------------------------------------
+template <typename T, int QM, int NCW>
+static void launch_panel(dqmc_engine* e, int batch, int m, int k0, int nbp, T* A, int lda, long sA,
+                         T* tau, long stau, T* V, int ldv, long sV, T* Tk, long sT, T* det, int init)
+{
+    k_qr_panel<T, QM, NCW><<<batch, QRB_THREADS, 0, e->st>>>(m, k0, nbp, A, lda, sA, tau, stau, V,
+                                                             ldv, sV, Tk, sT, det, init);
+}
+
+// A[b] (m x n, lda) <- R (upper) + reflectors (below); tau[b][0..min(m,n));
+// Q[b] (m x m, ldq), if non-null, <- H_0 ... H_{k-1}; det[b], if non-null,
+// <- prod det(H_i).  Panels of 64 (real) / 32 (complex) columns.
+template <typename T>
+static int qr_blocked(dqmc_engine* e, int m, int n, T* A, int lda, long sA, T* tau, long stau,
+                      T* Q, int ldq, long sQ, T* det, int batch)
+{
+    constexpr bool Z = is_cplx<T>::value;
+    if (m > 1024) return set_err(DQMC_ERR_UNSUPPORTED, "QR of %d rows", m);
+    const int qm = m <= 256 ? 4 : (m <= 512 ? 8 : 16);
+    const int ncw = Z ? (qm == 16 ? 1 : 2) : (qm == 16 ? 2 : 4);
+    const int NB = 16 * ncw;
+    const int kmax = m < n ? m : n;
+    T* V = dp<T>(e->Vb);
+    const long sV = e->sVb;
+    const int ldv = m;
+    T* Tb = dp<T>(e->Tb);
+    const long sT = e->sTb;
+    T* W1 = dp<T>(e->Wq1);
+    T* W2 = dp<T>(e->Wq2);
+    const long sW = e->sWq;
+    const T one = from_re<T>(1.0), mone = from_re<T>(-1.0), zero = from_re<T>(0.0);
+    int np = 0;
+    for (int k0 = 0; k0 < kmax; k0 += NB, np++) {
+        const int nbp = (kmax - k0 < NB) ? kmax - k0 : NB;
+        T* Tk = Tb + (long)np * NB * NB;
+        if constexpr (Z) {
+            if (qm == 4) launch_panel<T, 4, 2>(e, batch, m, k0, nbp, A, lda, sA, tau, stau, V, ldv, sV, Tk, sT, det, k0 == 0);
+            else if (qm == 8) launch_panel<T, 8, 2>(e, batch, m, k0, nbp, A, lda, sA, tau, stau, V, ldv, sV, Tk, sT, det, k0 == 0);
+            else launch_panel<T, 16, 1>(e, batch, m, k0, nbp, A, lda, sA, tau, stau, V, ldv, sV, Tk, sT, det, k0 == 0);
+        } else {
+            if (qm == 4) launch_panel<T, 4, 4>(e, batch, m, k0, nbp, A, lda, sA, tau, stau, V, ldv, sV, Tk, sT, det, k0 == 0);
+            else if (qm == 8) launch_panel<T, 8, 4>(e, batch, m, k0, nbp, A, lda, sA, tau, stau, V, ldv, sV, Tk, sT, det, k0 == 0);
+            else launch_panel<T, 16, 2>(e, batch, m, k0, nbp, A, lda, sA, tau, stau, V, ldv, sV, Tk, sT, det, k0 == 0);
+        }
+        LCHK();
+        const int nt = n - k0 - nbp;
+        if (nt > 0) {
+            const int mp = m - k0;
+            T* At = A + k0 + (long)lda * (k0 + nbp);
+            T* Vk = V + k0 + (long)ldv * k0;
+            RC(gemm<T>(e, OP_C, OP_N, nbp, nt, mp, one, Vk, ldv, sV, At, lda, sA, zero, W1, NB, sW,
+                       batch));
+            RC(gemm<T>(e, OP_C, OP_N, nbp, nt, nbp, one, Tk, NB, sT, W1, NB, sW, zero, W2, NB, sW,
+                       batch));
+            RC(gemm<T>(e, OP_N, OP_N, mp, nt, nbp, mone, Vk, ldv, sV, W2, NB, sW, one, At, lda, sA,
+                       batch));
+        }
+    }
+    if (Q) {
+        k_set_identity<T><<<nblk((long)m * m * batch), 256, 0, e->st>>>(m, Q, sQ, ldq, 0, batch);
+        LCHK();
+        for (int p = np - 1; p >= 0; p--) {
+            const int k0 = p * NB;
+            const int nbp = (kmax - k0 < NB) ? kmax - k0 : NB;
+            const int mp = m - k0;
+            T* Tk = Tb + (long)p * NB * NB;
+            T* Vk = V + k0 + (long)ldv * k0;
+            T* Qs = Q + k0 + (long)ldq * k0;
+            RC(gemm<T>(e, OP_C, OP_N, nbp, mp, mp, one, Vk, ldv, sV, Qs, ldq, sQ, zero, W1, NB, sW,
+                       batch));
+            RC(gemm<T>(e, OP_N, OP_N, nbp, mp, nbp, one, Tk, NB, sT, W1, NB, sW, zero, W2, NB, sW,
+                       batch));
+            RC(gemm<T>(e, OP_N, OP_N, mp, mp, nbp, mone, Vk, ldv, sV, W2, NB, sW, one, Qs, ldq, sQ,
+                       batch));
+        }
+    }
+    return 0;
+}
+
+// X[b] (n x ncols, ldx) <- U[b]^-1 X[b] for upper-triangular U (ldu): 64-row
+// blocks from the bottom, GEMM updates off the diagonal, LDS solves on it.
+template <typename T>
+static int trsm_upper_blocked(dqmc_engine* e, int n, int ncols, const T* U, int ldu, long sU, T* X,
+                              int ldx, long sX, int batch)
+{
+    const int BS = 64;
+    const T one = from_re<T>(1.0), mone = from_re<T>(-1.0);
+    const int nblocks = (n + BS - 1) / BS;
+    for (int kb = nblocks - 1; kb >= 0; kb--) {
+        const int k0 = kb * BS;
+        const int nk = (n - k0 < BS) ? n - k0 : BS;
+        if (k0 + nk < n)
+            RC(gemm<T>(e, OP_N, OP_N, nk, ncols, n - k0 - nk, mone, U + k0 + (long)ldu * (k0 + nk),
+                       ldu, sU, X + k0 + nk, ldx, sX, one, X + k0, ldx, sX, batch));
+        dim3 grid((ncols + 4 * LU_NC - 1) / (4 * LU_NC), batch);
+        k_trsm_diag64<T><<<grid, 256, 0, e->st>>>(nk, ncols, U + k0 + (long)ldu * k0, sU, ldu,
+                                                  X + k0, sX, ldx);
+        LCHK();
+    }
+    return 0;
+}
+
+// --------------------------------------------------------------------------
 // stabilized equal-time G (calc_eq_g, src/greens.c:69-208) for all b:
 // Gout[b] <- (I + C_{l-1} ... C_l)^-1, phase_b[b] <- phase of det(I + ...)
 // --------------------------------------------------------------------------
@@ -351,39 +460,50 @@ static int calc_eq_g(dqmc_engine* e, int l, T* Gout, T* phase_b)
     const int n_mul = 2;  // N_MUL, src/greens.h:5
     T *X = dp<T>(e->X), *Y = dp<T>(e->Y), *Y2 = dp<T>(e->Y2), *Z = dp<T>(e->Z);
     T *Q = dp<T>(e->Q), *Tm = dp<T>(e->Tm), *Rs = dp<T>(e->Rs), *Tp = dp<T>(e->Tp);
+    T* Mm = dp<T>(e->Mm);
     T* tau = dp<T>(e->tau);
+    T* detq = dp<T>(e->detq);
+    T* detm = dp<T>(e->detm);
     const T one = from_re<T>(1.0), zero = from_re<T>(0.0);
     int m = (l + 1 + (F - 1) % n_mul) % F;
     RC(mul_seq_C<T>(e, l, m, one, X, N, NN));
+    // (1) column-pivoted QR of the first product (xgeqp3)
     k_qr<T, true><<<B, QR_THREADS, 0, e->st>>>(N, N, X, N, NN, tau, N, e->jpvt, N, Q, N, NN);
+    LCHK();
+    k_reflector_det<T><<<B, 64, 0, e->st>>>(N, N, X, N, NN, tau, N, detq);
     LCHK();
     k_eqg_init_T<T><<<dim3(N, B), 256, 0, e->st>>>(N, X, NN, e->jpvt, N, e->dvec, N, Tm, NN);
     LCHK();
-    T* Vlast = X;
     while (m != l) {
         const int next = (m + n_mul) % F;
         RC(mul_seq_C<T>(e, m, next, one, Y, N, NN));
         m = next;
-        // Y2 = Y * Q * diag(d)   (xunmqr R,N + column scaling)
+        // (3a) Y2 = Y * Q * diag(d)   (xunmqr R,N + column scaling)
         RC(gemm<T>(e, OP_N, OP_N, N, N, N, one, Y, N, NN, Q, N, NN, zero, Y2, N, NN, B, nullptr,
                    0, nullptr, 0, e->dvec, N));
+        // (3b) pre-pivot by column norms, (3c) blocked QR with explicit Q
         k_prepivot<T><<<B, 1024, 0, e->st>>>(N, Y2, NN, Z, NN, e->perm, N);
         LCHK();
-        k_qr<T, false><<<B, QR_THREADS, 0, e->st>>>(N, N, Z, N, NN, tau, N, nullptr, 0, Q, N, NN);
-        LCHK();
+        RC(qr_blocked<T>(e, N, N, Z, N, NN, tau, N, Q, N, NN, detq, B));
+        // (3d) T = diag(1/d) R * P^T T   (xtrmm L,U)
         k_eqg_post_qr<T><<<dim3(N, B), 256, 0, e->st>>>(N, Z, NN, e->perm, N, Tm, NN, e->dvec, N,
                                                        Rs, Tp, NN);
         LCHK();
-        // T = diag(1/d) R * P^T T   (xtrmm L,U)
         RC(gemm<T>(e, OP_N, OP_N, N, N, N, one, Rs, N, NN, Tp, N, NN, zero, Tm, N, NN, B));
-        Vlast = Z;
     }
-    k_eqg_final_prep<T><<<dim3(N, B), 256, 0, e->st>>>(N, Q, NN, e->dvec, N, Tm, NN, dp<T>(e->Mm),
-                                                      Gout, NN, e->vvec, N);
+    // Eq. 2.12: G = M'^-1 diag(v) Q^H with M' = diag(ds) T + diag(v) Q^H, solved
+    // stably through the QR of M' (M' = Q_M R_M, G = R_M^-1 Q_M^H RHS)
+    k_eqg_final_prep<T><<<dim3(N, B), 256, 0, e->st>>>(N, Q, NN, e->dvec, N, Tm, NN, Mm, Gout, NN,
+                                                      e->vvec, N);
     LCHK();
-    k_getrf<T><<<B, 1024, 0, e->st>>>(N, dp<T>(e->Mm), Gout, NN, e->ipiv, N);
+    RC(qr_blocked<T>(e, N, N, Mm, N, NN, tau, N, X, N, NN, detm, B));
+    RC(gemm<T>(e, OP_C, OP_N, N, N, N, one, X, N, NN, Gout, N, NN, zero, Y, N, NN, B));
+    RC(trsm_upper_blocked<T>(e, N, N, Mm, N, NN, Y, N, NN, B));
+    k_copy_block<T><<<nblk(NN * B), 256, 0, e->st>>>(N, Y, NN, N, one, Gout, NN, N, B);
     LCHK();
-    return launch_getrs<T>(e, Gout, Vlast, phase_b);
+    k_eqg_phase<T><<<B, 64, 0, e->st>>>(N, detq, detm, e->vvec, N, Mm, NN, phase_b);
+    LCHK();
+    return 0;
 }
 
 // --------------------------------------------------------------------------
@@ -635,7 +755,16 @@ static int engine_setup(dqmc_engine* e, const dqmc_params* const* P)
     DALLOC(e->iscale, (long)B * L * N * 8);
     void** work[] = {&e->X, &e->Y, &e->Y2, &e->Z, &e->Q, &e->Tm, &e->Rs, &e->Tp, &e->Mm, &e->T1};
     for (void** w : work) DALLOC(*w, B * NN * es);
-    DALLOC(e->tau, (long)B * N * es);
+    DALLOC(e->tau, (long)B * 2 * N * es);
+    e->sVb = (e->period_uneqlt > 0 ? 4L : 1L) * NN;
+    e->sTb = 2L * N * 64 + 64 * 64;
+    e->sWq = 64L * 2 * N;
+    DALLOC(e->Vb, B * e->sVb * es);
+    DALLOC(e->Tb, B * e->sTb * es);
+    DALLOC(e->Wq1, B * e->sWq * es);
+    DALLOC(e->Wq2, B * e->sWq * es);
+    DALLOC(e->detq, B * es);
+    DALLOC(e->detm, B * es);
     DALLOC(e->jpvt, (long)B * N * 4);
     DALLOC(e->perm, (long)B * N * 4);
     DALLOC(e->ipiv, (long)B * N * 4);
