@@ -43,13 +43,17 @@ __device__ inline void larfg(zd alpha, double xnorm2, zd* beta, zd* tau, zd* sca
 
 // QR of the m x n matrix A[b] (lda) in place: R in the upper triangle, the
 // Householder vectors below it, tau[b][0..k).  PIVOT: column pivoting on the
-// largest remaining column norm (geqp3); jpvt[b][j] = 1-based original column
-// of column j.  Q[b] (ldq, m x m), when non-null, receives Q = H_0 ... H_{k-1}.
-template <typename T, bool PIVOT>
+// largest remaining column norm (geqp3, norms recomputed exactly after every
+// step); jpvt[b][j] = 1-based original column of column j.  Q[b] (ldq, m x m),
+// when non-null, receives Q = H_0 ... H_{k-1}.
+// Row i lives in lane i%64, slot i/64 (RPL slots, m <= 64*RPL); each wave
+// updates CB of its columns at once so their loads are in flight together.
+template <typename T, bool PIVOT, int RPL>
 __global__ __launch_bounds__(QR_THREADS) void k_qr(int m, int n, T* A_, int lda, long sA, T* tau_,
                                                     long stau, int* jpvt_, long spiv, T* Q_, int ldq,
                                                     long sQ)
 {
+    constexpr int CB = (is_cplx<T>::value ? 8 : 16) / RPL > 0 ? (is_cplx<T>::value ? 8 : 16) / RPL : 1;
     __shared__ T v[QR_MAXM];
     __shared__ double nrm[QR_MAXM];
     __shared__ int piv[QR_MAXM];
@@ -64,7 +68,11 @@ __global__ __launch_bounds__(QR_THREADS) void k_qr(int m, int n, T* A_, int lda,
     if (PIVOT) {
         for (int c = wave; c < n; c += QR_WAVES) {
             double s = 0.0;
-            for (int i = lane; i < m; i += 64) s += abs2(A[i + (long)lda * c]);
+#pragma unroll
+            for (int r = 0; r < RPL; r++) {
+                const int i = lane + 64 * r;
+                if (i < m) s += abs2(A[i + (long)lda * c]);
+            }
             s = wave_sum(s);
             if (lane == 0) { nrm[c] = s; piv[c] = c; }
         }
@@ -101,16 +109,28 @@ __global__ __launch_bounds__(QR_THREADS) void k_qr(int m, int n, T* A_, int lda,
         }
         // reflector for column j (wave 0)
         if (wave == 0) {
+            T xv[RPL];
             double s = 0.0;
-            for (int i = j + 1 + lane; i < m; i += 64) s += abs2(A[i + (long)lda * j]);
+            T alpha = from_re<T>(0.0);
+#pragma unroll
+            for (int r = 0; r < RPL; r++) {
+                const int i = lane + 64 * r;
+                xv[r] = (i >= j && i < m) ? A[i + (long)lda * j] : from_re<T>(0.0);
+                if (i > j && i < m) s += abs2(xv[r]);
+                if (i == j) alpha = xv[r];
+            }
             s = wave_sum(s);
-            const T alpha = A[j + (long)lda * j];
+            alpha = shfl(alpha, j & 63);
             T beta, t, scal;
             larfg(alpha, s, &beta, &t, &scal);
-            for (int i = j + 1 + lane; i < m; i += 64) {
-                const T x = A[i + (long)lda * j] * scal;
-                A[i + (long)lda * j] = x;
-                v[i] = x;
+#pragma unroll
+            for (int r = 0; r < RPL; r++) {
+                const int i = lane + 64 * r;
+                if (i > j && i < m) {
+                    const T x = xv[r] * scal;
+                    A[i + (long)lda * j] = x;
+                    v[i] = x;
+                }
             }
             if (lane == 0) {
                 A[j + (long)lda * j] = beta;
@@ -121,30 +141,47 @@ __global__ __launch_bounds__(QR_THREADS) void k_qr(int m, int n, T* A_, int lda,
         }
         __syncthreads();
         const T ct = cj(s_tau);
-        if (!is_zero(ct)) {
-            for (int c = j + 1 + wave; c < n; c += QR_WAVES) {
-                T* col = A + (long)lda * c;
-                T w = from_re<T>(0.0);
-                for (int i = j + lane; i < m; i += 64) w += cj(v[i]) * col[i];
-                w = wave_sum(w);
-                const T f = ct * w;
-                double s = 0.0;
-                for (int i = j + lane; i < m; i += 64) {
-                    const T x = col[i] - v[i] * f;
-                    col[i] = x;
-                    if (PIVOT && i > j) s += abs2(x);
-                }
-                if (PIVOT) {
-                    s = wave_sum(s);
-                    if (lane == 0) nrm[c] = s;
+        for (int c0 = j + 1 + wave; c0 < n; c0 += QR_WAVES * CB) {
+            T x[CB][RPL];
+            T w[CB];
+#pragma unroll
+            for (int t = 0; t < CB; t++) {
+                const int c = c0 + QR_WAVES * t;
+                w[t] = from_re<T>(0.0);
+#pragma unroll
+                for (int r = 0; r < RPL; r++) {
+                    const int i = lane + 64 * r;
+                    x[t][r] = (c < n && i >= j && i < m) ? A[i + (long)lda * c] : from_re<T>(0.0);
                 }
             }
-        } else if (PIVOT) {
-            for (int c = j + 1 + wave; c < n; c += QR_WAVES) {
-                double s = 0.0;
-                for (int i = j + 1 + lane; i < m; i += 64) s += abs2(A[i + (long)lda * c]);
-                s = wave_sum(s);
-                if (lane == 0) nrm[c] = s;
+#pragma unroll
+            for (int t = 0; t < CB; t++) {
+#pragma unroll
+                for (int r = 0; r < RPL; r++) {
+                    const int i = lane + 64 * r;
+                    if (i >= j && i < m) w[t] += cj(v[i]) * x[t][r];
+                }
+                w[t] = wave_sum(w[t]);
+            }
+#pragma unroll
+            for (int t = 0; t < CB; t++) {
+                const int c = c0 + QR_WAVES * t;
+                if (c >= n) continue;
+                const T f = ct * w[t];
+                double sn = 0.0;
+#pragma unroll
+                for (int r = 0; r < RPL; r++) {
+                    const int i = lane + 64 * r;
+                    if (i >= j && i < m) {
+                        const T y = x[t][r] - v[i] * f;
+                        A[i + (long)lda * c] = y;
+                        if (PIVOT && i > j) sn += abs2(y);
+                    }
+                }
+                if (PIVOT) {
+                    sn = wave_sum(sn);
+                    if (lane == 0) nrm[c] = sn;
+                }
             }
         }
         __syncthreads();
@@ -166,13 +203,40 @@ __global__ __launch_bounds__(QR_THREADS) void k_qr(int m, int n, T* A_, int lda,
             __syncthreads();
             const T t = tau[j];
             if (!is_zero(t)) {
-                for (int c = j + wave; c < m; c += QR_WAVES) {
-                    T* col = Q + (long)ldq * c;
-                    T w = from_re<T>(0.0);
-                    for (int i = j + lane; i < m; i += 64) w += cj(v[i]) * col[i];
-                    w = wave_sum(w);
-                    const T f = t * w;
-                    for (int i = j + lane; i < m; i += 64) col[i] -= v[i] * f;
+                for (int c0 = j + wave; c0 < m; c0 += QR_WAVES * CB) {
+                    T x[CB][RPL];
+                    T w[CB];
+#pragma unroll
+                    for (int u = 0; u < CB; u++) {
+                        const int c = c0 + QR_WAVES * u;
+                        w[u] = from_re<T>(0.0);
+#pragma unroll
+                        for (int r = 0; r < RPL; r++) {
+                            const int i = lane + 64 * r;
+                            x[u][r] = (c < m && i >= j && i < m) ? Q[i + (long)ldq * c]
+                                                                  : from_re<T>(0.0);
+                        }
+                    }
+#pragma unroll
+                    for (int u = 0; u < CB; u++) {
+#pragma unroll
+                        for (int r = 0; r < RPL; r++) {
+                            const int i = lane + 64 * r;
+                            if (i >= j && i < m) w[u] += cj(v[i]) * x[u][r];
+                        }
+                        w[u] = wave_sum(w[u]);
+                    }
+#pragma unroll
+                    for (int u = 0; u < CB; u++) {
+                        const int c = c0 + QR_WAVES * u;
+                        if (c >= m) continue;
+                        const T f = t * w[u];
+#pragma unroll
+                        for (int r = 0; r < RPL; r++) {
+                            const int i = lane + 64 * r;
+                            if (i >= j && i < m) Q[i + (long)ldq * c] = x[u][r] - v[i] * f;
+                        }
+                    }
                 }
             }
             __syncthreads();

@@ -34,6 +34,11 @@ struct Upd2Args {
     const int* active;              // [c]
 };
 
+// G += A(:, 0:KD) B(:, 0:KD)^T for the rows this thread owns, d = diag(G).
+// B is staged through LDS in chunks of UPD2_CH rows; within a chunk each
+// thread keeps FB columns of its G row in flight at once so the global-memory
+// latency overlaps instead of being paid once per column.
+#define UPD2_FB 8
 template <typename T, int R, int KD>
 __device__ inline void upd2_flush(int N, int tid, T* G, const T (&A)[R][KD], const T (&Bm)[R][KD],
                                   T* d, T* Bst)
@@ -53,12 +58,22 @@ __device__ inline void upd2_flush(int N, int tid, T* G, const T (&A)[R][KD], con
         for (int r = 0; r < R; r++) {
             const int j = tid + UPD2_THREADS * r;
             if (j < N) {
-                for (int cc = 0; cc < cend; cc++) {
-                    T* gp = G + j + (long)N * (c0 + cc);
-                    T s = *gp;
+                T* grow = G + j + (long)N * c0;
+                for (int cb = 0; cb < cend; cb += UPD2_FB) {
+                    T g[UPD2_FB];
 #pragma unroll
-                    for (int m = 0; m < KD; m++) s += A[r][m] * Bst[cc * KD + m];
-                    *gp = s;
+                    for (int t = 0; t < UPD2_FB; t++)
+                        if (cb + t < cend) g[t] = grow[(long)N * (cb + t)];
+#pragma unroll
+                    for (int t = 0; t < UPD2_FB; t++) {
+                        T s = g[t];
+#pragma unroll
+                        for (int m = 0; m < KD; m++) s += A[r][m] * Bst[(cb + t) * KD + m];
+                        g[t] = s;
+                    }
+#pragma unroll
+                    for (int t = 0; t < UPD2_FB; t++)
+                        if (cb + t < cend) grow[(long)N * (cb + t)] = g[t];
                 }
             }
         }

@@ -468,7 +468,12 @@ static int calc_eq_g(dqmc_engine* e, int l, T* Gout, T* phase_b)
     int m = (l + 1 + (F - 1) % n_mul) % F;
     RC(mul_seq_C<T>(e, l, m, one, X, N, NN));
     // (1) column-pivoted QR of the first product (xgeqp3)
-    k_qr<T, true><<<B, QR_THREADS, 0, e->st>>>(N, N, X, N, NN, tau, N, e->jpvt, N, Q, N, NN);
+    if (N <= 256)
+        k_qr<T, true, 4><<<B, QR_THREADS, 0, e->st>>>(N, N, X, N, NN, tau, N, e->jpvt, N, Q, N, NN);
+    else if (N <= 512)
+        k_qr<T, true, 8><<<B, QR_THREADS, 0, e->st>>>(N, N, X, N, NN, tau, N, e->jpvt, N, Q, N, NN);
+    else
+        k_qr<T, true, 16><<<B, QR_THREADS, 0, e->st>>>(N, N, X, N, NN, tau, N, e->jpvt, N, Q, N, NN);
     LCHK();
     k_reflector_det<T><<<B, 64, 0, e->st>>>(N, N, X, N, NN, tau, N, detq);
     LCHK();
@@ -581,8 +586,9 @@ static MeasArgs<T> meas_args(dqmc_engine* e, int ue)
 #include "dq_ue_host.hpp"
 
 // Metropolis sweep of slice l for every chain; the delay rank is the register
-// capacity of the kernel variant (real: 32 slots for N <= 256, 16 for N <= 512;
-// complex: half of that).
+// capacity of the kernel variant (real: 16 slots for N <= 256, 8 for N <= 512;
+// complex: half of that).  The flush work per acceptance does not depend on the
+// rank (rounds cost ~rank, there are ~1/rank of them); the gemv shrinks with it.
 template <typename T>
 static int launch_update(dqmc_engine* e, int l)
 {
@@ -597,8 +603,8 @@ static int launch_update(dqmc_engine* e, int l)
         a.phase = dp<T>(e->phase); a.escale = e->escale; a.iscale = e->iscale;
         a.n_acc = e->nacc; a.active = e->active;
         constexpr bool Z = is_cplx<T>::value;
-        if (N <= 256) k_update_reg<T, 1, Z ? 16 : 32><<<nc, UPD2_THREADS, 0, e->st>>>(a);
-        else k_update_reg<T, 2, Z ? 8 : 16><<<nc, UPD2_THREADS, 0, e->st>>>(a);
+        if (N <= 256) k_update_reg<T, 1, Z ? 8 : 16><<<nc, UPD2_THREADS, 0, e->st>>>(a);
+        else k_update_reg<T, 2, Z ? 4 : 8><<<nc, UPD2_THREADS, 0, e->st>>>(a);
         LCHK();
         return 0;
     }
