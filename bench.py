@@ -68,9 +68,11 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=4)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--chains", type=int, default=32)
+    ap.add_argument("--chains", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-single", action="store_true")
+    ap.add_argument("--gemm-tile", type=int, default=0,
+                    help="GEMM C tile: 0 = autotune, 1 = 64x64, 2 = 128x128, 3 = 128x64")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (chains shard trivially across GPUs, "
@@ -79,9 +81,12 @@ def main():
     import numpy as np
     from dqmc_amd import engine
 
+    tiles = {1: "64x64", 2: "128x128", 3: "128x64"}
+
     def timed(nch, steps, warmup, time_gemm):
         sims = workload(nch)
-        eng = engine.Engine(sims, device=0, time_gemm=time_gemm)
+        eng = engine.Engine(sims, device=0, time_gemm=time_gemm, gemm_tile=args.gemm_tile)
+        tile = eng.gemm_tile()
         eng.run(0)  # initial G outside the timed region
         eng.run(warmup)
         acc0 = eng.accepts()
@@ -92,10 +97,10 @@ def main():
         acc = eng.accepts() - acc0
         gstats = eng.gemm_stats()
         eng.close()
-        return dt, acc, gstats
+        return dt, acc, gstats, tile
 
     nch = args.chains
-    dt, acc, (g_ms, g_n, g_flops) = timed(nch, args.steps, args.warmup, True)
+    dt, acc, (g_ms, g_n, g_flops), tile = timed(nch, args.steps, args.warmup, True)
     value = nch * args.steps / dt
     N, L, F, n_matmul = 256, 80, 8, 10
     acc_per_sweep = acc / (nch * args.steps)
@@ -103,7 +108,7 @@ def main():
     model_tflops = fl * value / 1e12
     single = None
     if not args.no_single:
-        dts, accs, _ = timed(1, max(2, args.steps // 2), 1, False)
+        dts, accs, _, _ = timed(1, max(2, args.steps // 2), 1, False)
         single = max(2, args.steps // 2) / dts
     achieved = g_flops / (g_ms * 1e-3) / 1e12 if g_ms > 0 else 0.0
     line = {
@@ -128,7 +133,8 @@ def main():
         "model_tflops": round(model_tflops, 3),
         "model_frac_fp64_peak": round(model_tflops / PEAK_FP64_TFLOPS, 4),
         "accepts_per_sweep": round(acc_per_sweep, 1),
-        "roofline": {"bound": "mfma", "kernel": "k_gemm_r<N,N> (N x N x N, batch 2*chains)",
+        "roofline": {"bound": "mfma",
+                     "kernel": "k_gemm<double,N,N> tile %s (N x N x N, batch 2*chains)" % tiles[tile],
                      "achieved": round(achieved, 3), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(achieved / PEAK_FP64_TFLOPS, 4), "traffic": None,
                      "avg_launch_ms": round(g_ms, 5), "launches": g_n},

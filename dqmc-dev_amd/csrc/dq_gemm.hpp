@@ -10,9 +10,10 @@
 //   rank-k flush G += A B^T, Q^H products, half-wraps, T updates.
 // B_l is never materialised: the HS-field scaling is applied on the fly.
 //
-// Tiling for CDNA4: 256 threads = 4 wave64s, 64x64 C tile per workgroup, each
-// wave owns 32x32 = 2x2 tiles of v_mfma_f64_16x16x4 (K step 16 staged through
-// LDS).  Complex (zd) uses four real MFMAs per block on planar LDS copies.
+// Tiling for CDNA4: 256 threads = 4 wave64s per C tile (64x64, 128x64 or
+// 128x128), each wave a 2-D array of v_mfma_f64_16x16x4 blocks; K is staged
+// through double-buffered LDS.  Complex (zd) uses four real MFMAs per block on
+// planar (re/im) LDS copies.
 // Operand lane layout of v_mfma_f64_16x16x4f64 (wave64):
 //   A[16x4]: lane l holds A[l%16][l/16];  B[4x16]: lane l holds B[l/16][l%16]
 //   D[16x16]: lane l, element r holds D[4*r + l/16][l%16]
@@ -38,229 +39,252 @@ struct GemmArgs {
     int adiv, bdiv;           // batch z uses A[z / adiv], B[z / bdiv] (shared operands)
 };
 
-#define GB_M 64
-#define GB_N 64
-#define GB_K 16
-
-template <int OP, typename T>
-__device__ inline T ld_op(const T* p, long idx)
-{
-    if (OP == OP_C) return cj(p[idx]);
-    return p[idx];
-}
+// Tile shapes (rows x columns of C per 256-thread workgroup).  The engine's
+// autotuner times the shapes on its own product sizes and keeps the fastest.
+enum { GT_AUTO = 0, GT_64 = 1, GT_128 = 2, GT_128x64 = 3 };
 
 __device__ inline dbl4 mfma(double a, double b, dbl4 c)
 {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
 
-// real ---------------------------------------------------------------------
-template <int OPA, int OPB>
-__global__ __launch_bounds__(256) void k_gemm_r(GemmArgs<double> g)
-{
-    __shared__ double As[GB_K][GB_M];
-    __shared__ double Bs[GB_K][GB_N];
-    const int b = blockIdx.z;
-    const int m0 = blockIdx.x * GB_M, n0 = blockIdx.y * GB_N;
-    const double* A = g.A + (b / g.adiv) * g.sA;
-    const double* B = g.B + (b / g.bdiv) * g.sB;
-    double* C = g.C + b * g.sC;
-    const double* u = g.u ? g.u + b * g.su : nullptr;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
-    dbl4 acc[2][2];
-#pragma unroll
-    for (int i = 0; i < 2; i++)
-#pragma unroll
-        for (int j = 0; j < 2; j++) acc[i][j] = (dbl4){0.0, 0.0, 0.0, 0.0};
-
-    for (int k0 = 0; k0 < g.K; k0 += GB_K) {
-#pragma unroll
-        for (int q = 0; q < 4; q++) {
-            const int idx = tid + 256 * q;
-            int i, k;
-            if (OPA == OP_N) { i = idx & 63; k = idx >> 6; }
-            else { k = idx & 15; i = idx >> 4; }
-            const int gi = m0 + i;
-            int gk = k0 + k;
-            double v = 0.0;
-            if (gi < g.M && gk < g.K)
-                v = (OPA == OP_N) ? A[gi + (long)g.lda * gk] : A[gk + (long)g.lda * gi];
-            As[k][i] = v;
-            int j;
-            if (OPB == OP_N) { k = idx & 15; j = idx >> 4; }
-            else { j = idx & 63; k = idx >> 6; }
-            const int gj = n0 + j;
-            gk = k0 + k;
-            double wv = 0.0;
-            if (gj < g.N && gk < g.K) {
-                wv = (OPB == OP_N) ? B[gk + (long)g.ldb * gj] : B[gj + (long)g.ldb * gk];
-                if (u) wv *= u[gk];
-            }
-            Bs[k][j] = wv;
-        }
-        __syncthreads();
-#pragma unroll
-        for (int ks = 0; ks < GB_K; ks += 4) {
-            const int kk = ks + (lane >> 4);
-            double a0 = As[kk][wm + (lane & 15)];
-            double a1 = As[kk][wm + 16 + (lane & 15)];
-            double b0 = Bs[kk][wn + (lane & 15)];
-            double b1 = Bs[kk][wn + 16 + (lane & 15)];
-            acc[0][0] = mfma(a0, b0, acc[0][0]);
-            acc[0][1] = mfma(a0, b1, acc[0][1]);
-            acc[1][0] = mfma(a1, b0, acc[1][0]);
-            acc[1][1] = mfma(a1, b1, acc[1][1]);
-        }
-        __syncthreads();
-    }
-    const double* rr = g.r ? g.r + b * g.sr : nullptr;
-    const double* ww = g.w ? g.w + b * g.sw : nullptr;
-#pragma unroll
-    for (int ti = 0; ti < 2; ti++)
-#pragma unroll
-        for (int tj = 0; tj < 2; tj++)
-#pragma unroll
-            for (int e = 0; e < 4; e++) {
-                const int row = m0 + wm + ti * 16 + 4 * e + (lane >> 4);
-                const int col = n0 + wn + tj * 16 + (lane & 15);
-                if (row < g.M && col < g.N) {
-                    double v = g.alpha * acc[ti][tj][e];
-                    if (rr) v *= rr[row];
-                    if (ww) v *= ww[col];
-                    const long o = row + (long)g.ldc * col;
-                    if (!g.beta_zero) v += g.beta * C[o];
-                    C[o] = v;
-                }
-            }
-}
-
-// complex ------------------------------------------------------------------
-template <int OPA, int OPB>
-__global__ __launch_bounds__(256) void k_gemm_z(GemmArgs<zd> g)
-{
-    __shared__ double Ar[GB_K][GB_M], Ai[GB_K][GB_M];
-    __shared__ double Br[GB_K][GB_N], Bi[GB_K][GB_N];
-    const int b = blockIdx.z;
-    const int m0 = blockIdx.x * GB_M, n0 = blockIdx.y * GB_N;
-    const zd* A = g.A + (b / g.adiv) * g.sA;
-    const zd* B = g.B + (b / g.bdiv) * g.sB;
-    zd* C = g.C + b * g.sC;
-    const double* u = g.u ? g.u + b * g.su : nullptr;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
-    dbl4 cr[2][2], ci[2][2];
-#pragma unroll
-    for (int i = 0; i < 2; i++)
-#pragma unroll
-        for (int j = 0; j < 2; j++) {
-            cr[i][j] = (dbl4){0.0, 0.0, 0.0, 0.0};
-            ci[i][j] = (dbl4){0.0, 0.0, 0.0, 0.0};
-        }
-    for (int k0 = 0; k0 < g.K; k0 += GB_K) {
-#pragma unroll
-        for (int q = 0; q < 4; q++) {
-            const int idx = tid + 256 * q;
-            int i, k;
-            if (OPA == OP_N) { i = idx & 63; k = idx >> 6; }
-            else { k = idx & 15; i = idx >> 4; }
-            const int gi = m0 + i;
-            int gk = k0 + k;
-            zd v = mkz(0.0, 0.0);
-            if (gi < g.M && gk < g.K)
-                v = (OPA == OP_N) ? A[gi + (long)g.lda * gk]
-                                  : ld_op<OPA>(A, gk + (long)g.lda * gi);
-            Ar[k][i] = v.x;
-            Ai[k][i] = v.y;
-            int j;
-            if (OPB == OP_N) { k = idx & 15; j = idx >> 4; }
-            else { j = idx & 63; k = idx >> 6; }
-            const int gj = n0 + j;
-            gk = k0 + k;
-            zd wv = mkz(0.0, 0.0);
-            if (gj < g.N && gk < g.K) {
-                wv = (OPB == OP_N) ? B[gk + (long)g.ldb * gj]
-                                   : ld_op<OPB>(B, gj + (long)g.ldb * gk);
-                if (u) wv = u[gk] * wv;
-            }
-            Br[k][j] = wv.x;
-            Bi[k][j] = wv.y;
-        }
-        __syncthreads();
-#pragma unroll
-        for (int ks = 0; ks < GB_K; ks += 4) {
-            const int kk = ks + (lane >> 4);
-            const int l16 = lane & 15;
-            double ar0 = Ar[kk][wm + l16], ai0 = Ai[kk][wm + l16];
-            double ar1 = Ar[kk][wm + 16 + l16], ai1 = Ai[kk][wm + 16 + l16];
-            double br0 = Br[kk][wn + l16], bi0 = Bi[kk][wn + l16];
-            double br1 = Br[kk][wn + 16 + l16], bi1 = Bi[kk][wn + 16 + l16];
-            double nai0 = -ai0, nai1 = -ai1;
-            cr[0][0] = mfma(ar0, br0, cr[0][0]); cr[0][0] = mfma(nai0, bi0, cr[0][0]);
-            ci[0][0] = mfma(ar0, bi0, ci[0][0]); ci[0][0] = mfma(ai0, br0, ci[0][0]);
-            cr[0][1] = mfma(ar0, br1, cr[0][1]); cr[0][1] = mfma(nai0, bi1, cr[0][1]);
-            ci[0][1] = mfma(ar0, bi1, ci[0][1]); ci[0][1] = mfma(ai0, br1, ci[0][1]);
-            cr[1][0] = mfma(ar1, br0, cr[1][0]); cr[1][0] = mfma(nai1, bi0, cr[1][0]);
-            ci[1][0] = mfma(ar1, bi0, ci[1][0]); ci[1][0] = mfma(ai1, br0, ci[1][0]);
-            cr[1][1] = mfma(ar1, br1, cr[1][1]); cr[1][1] = mfma(nai1, bi1, cr[1][1]);
-            ci[1][1] = mfma(ar1, bi1, ci[1][1]); ci[1][1] = mfma(ai1, br1, ci[1][1]);
-        }
-        __syncthreads();
-    }
-    const double* rr = g.r ? g.r + b * g.sr : nullptr;
-    const double* ww = g.w ? g.w + b * g.sw : nullptr;
-#pragma unroll
-    for (int ti = 0; ti < 2; ti++)
-#pragma unroll
-        for (int tj = 0; tj < 2; tj++)
-#pragma unroll
-            for (int e = 0; e < 4; e++) {
-                const int row = m0 + wm + ti * 16 + 4 * e + (lane >> 4);
-                const int col = n0 + wn + tj * 16 + (lane & 15);
-                if (row < g.M && col < g.N) {
-                    zd v = g.alpha * mkz(cr[ti][tj][e], ci[ti][tj][e]);
-                    if (rr) v = rr[row] * v;
-                    if (ww) v = ww[col] * v;
-                    const long o = row + (long)g.ldc * col;
-                    if (!g.beta_zero) v += g.beta * C[o];
-                    C[o] = v;
-                }
-            }
-}
-
-template <typename T> struct GemmLauncher;
-
-template <> struct GemmLauncher<double> {
-    static hipError_t launch(int opa, int opb, const GemmArgs<double>& g, int batch, hipStream_t s)
-    {
-        dim3 grid((g.M + GB_M - 1) / GB_M, (g.N + GB_N - 1) / GB_N, batch);
-        if (opa == OP_N && opb == OP_N) k_gemm_r<OP_N, OP_N><<<grid, 256, 0, s>>>(g);
-        else if (opa == OP_N) k_gemm_r<OP_N, OP_T><<<grid, 256, 0, s>>>(g);
-        else if (opb == OP_N) k_gemm_r<OP_T, OP_N><<<grid, 256, 0, s>>>(g);
-        else k_gemm_r<OP_T, OP_T><<<grid, 256, 0, s>>>(g);
-        return hipGetLastError();
-    }
+// LDS image of one operand tile, BO (rows of C or columns of C) x BK, in the
+// orientation of its global storage so that the global->LDS copy is linear:
+//   OFAST: the tile's o index is contiguous in global memory -> rows [k][o],
+//          pitch BO + 16 (the 4 k-rows read by one MFMA step hit disjoint banks)
+//   else:  k is contiguous -> rows [o][k], pitch BK + 1.
+template <int BO, int BK, bool OFAST>
+struct GTile {
+    static constexpr int P = OFAST ? BO + 16 : BK + 1;
+    static constexpr int SIZE = OFAST ? BK * P : BO * P;
+    __device__ static inline int at(int o, int k) { return OFAST ? k * P + o : o * P + k; }
 };
 
-template <> struct GemmLauncher<zd> {
-    static hipError_t launch(int opa, int opb, const GemmArgs<zd>& g, int batch, hipStream_t s)
-    {
-        dim3 grid((g.M + GB_M - 1) / GB_M, (g.N + GB_N - 1) / GB_N, batch);
-#define ZL(a, b) k_gemm_z<a, b><<<grid, 256, 0, s>>>(g)
+// C tile = 32TM x 32TN, 4 waves in a 2x2 arrangement, each 16TM x 16TN made
+// of TM x TN MFMA blocks; K advanced in steps of BK with two LDS buffers (the
+// next step's global loads are in flight while the current one computes).
+// The MFMA computes the transposed block (op(B)^T op(A)^T), so that a lane's
+// accumulator entries run down a column of C and 16 lanes store 128
+// contiguous bytes of the column-major result.
+template <typename T, int OPA, int OPB, int TM, int TN, int BK, bool BETA>
+__global__ __launch_bounds__(256) void k_gemm(GemmArgs<T> g, int tiles_m, int tiles_n, int nwg)
+{
+    constexpr bool Z = is_cplx<T>::value;
+    constexpr int NP = Z ? 2 : 1;
+    constexpr int BM = 32 * TM, BN = 32 * TN;
+    constexpr bool AF = (OPA == OP_N), BF = (OPB != OP_N);
+    using LA = GTile<BM, BK, AF>;
+    using LB = GTile<BN, BK, BF>;
+    constexpr int NA = BM * BK / 256, NB = BN * BK / 256;
+    static_assert(NA * 256 == BM * BK && NB * 256 == BN * BK, "tile/thread mismatch");
+    static_assert(2 * NP * (LA::SIZE + LB::SIZE) * 8 <= 65536, "LDS budget");
+    __shared__ double As[2][NP][LA::SIZE];
+    __shared__ double Bs[2][NP][LB::SIZE];
+    __shared__ double Us[2][BK];  // inner scale u of the K step
+
+    // Workgroup ids are dealt round-robin over the 8 XCDs; renumber so that
+    // each XCD works through a contiguous range of tiles, which keeps the
+    // tiles of one matrix (and their shared operand panels) in one L2.
+    const int id = blockIdx.x;
+    const int xcd = id & 7, per = nwg >> 3, rem = nwg & 7;
+    const int lid = xcd * per + (xcd < rem ? xcd : rem) + (id >> 3);
+    const int tm = lid % tiles_m;
+    const int rest = lid / tiles_m;
+    const int tn = rest % tiles_n;
+    const int b = rest / tiles_n;
+    const int m0 = tm * BM, n0 = tn * BN;
+    const T* A = g.A + (long)(b / g.adiv) * g.sA;
+    const T* B = g.B + (long)(b / g.bdiv) * g.sB;
+    T* C = g.C + (long)b * g.sC;
+    const double* u = g.u ? g.u + (long)b * g.su : nullptr;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int l16 = lane & 15, l4 = lane >> 4;
+    const int wm = (wave >> 1) * 16 * TM, wn = (wave & 1) * 16 * TN;
+
+    dbl4 acc[NP][TM][TN];
+#pragma unroll
+    for (int p = 0; p < NP; p++)
+#pragma unroll
+        for (int i = 0; i < TM; i++)
+#pragma unroll
+            for (int j = 0; j < TN; j++) acc[p][i][j] = (dbl4){0.0, 0.0, 0.0, 0.0};
+
+    // Raw global values of the next K step.  Conjugation is applied when the
+    // values go to LDS and the inner scale u when the B operand is read from
+    // LDS, so that no arithmetic waits on loads still in flight.
+    T ra[NA], rb[NB];
+    double uk = 1.0;
+    auto gload = [&](int k0) {
+#pragma unroll
+        for (int q = 0; q < NA; q++) {
+            const int idx = tid + 256 * q;
+            const int o = AF ? idx % BM : idx / BK;
+            const int k = AF ? idx / BM : idx % BK;
+            const int gm = m0 + o, gk = k0 + k;
+            T v = from_re<T>(0.0);
+            if (gm < g.M && gk < g.K) v = AF ? A[gm + (long)g.lda * gk] : A[gk + (long)g.lda * gm];
+            ra[q] = v;
+        }
+#pragma unroll
+        for (int q = 0; q < NB; q++) {
+            const int idx = tid + 256 * q;
+            const int o = BF ? idx % BN : idx / BK;
+            const int k = BF ? idx / BN : idx % BK;
+            const int gn = n0 + o, gk = k0 + k;
+            T v = from_re<T>(0.0);
+            if (gn < g.N && gk < g.K) v = BF ? B[gn + (long)g.ldb * gk] : B[gk + (long)g.ldb * gn];
+            rb[q] = v;
+        }
+        if (u && tid < BK) uk = (k0 + tid < g.K) ? u[k0 + tid] : 0.0;
+    };
+    auto sstore = [&](int buf) {
+#pragma unroll
+        for (int q = 0; q < NA; q++) {
+            const int idx = tid + 256 * q;
+            const int o = AF ? idx % BM : idx / BK;
+            const int k = AF ? idx / BM : idx % BK;
+            As[buf][0][LA::at(o, k)] = re(ra[q]);
+            if constexpr (Z) As[buf][NP - 1][LA::at(o, k)] = (OPA == OP_C) ? -im(ra[q]) : im(ra[q]);
+        }
+#pragma unroll
+        for (int q = 0; q < NB; q++) {
+            const int idx = tid + 256 * q;
+            const int o = BF ? idx % BN : idx / BK;
+            const int k = BF ? idx / BN : idx % BK;
+            Bs[buf][0][LB::at(o, k)] = re(rb[q]);
+            if constexpr (Z) Bs[buf][NP - 1][LB::at(o, k)] = (OPB == OP_C) ? -im(rb[q]) : im(rb[q]);
+        }
+        if (u && tid < BK) Us[buf][tid] = uk;
+    };
+    auto compute = [&](int buf) {
+#pragma unroll
+        for (int ks = 0; ks < BK; ks += 4) {
+            const int kk = ks + l4;
+            double ar[TM], br[TN], ai[TM], bi[TN];
+#pragma unroll
+            for (int i = 0; i < TM; i++) {
+                ar[i] = As[buf][0][LA::at(wm + 16 * i + l16, kk)];
+                if constexpr (Z) ai[i] = As[buf][NP - 1][LA::at(wm + 16 * i + l16, kk)];
+            }
+#pragma unroll
+            for (int j = 0; j < TN; j++) {
+                br[j] = Bs[buf][0][LB::at(wn + 16 * j + l16, kk)];
+                if constexpr (Z) bi[j] = Bs[buf][NP - 1][LB::at(wn + 16 * j + l16, kk)];
+            }
+            if (u) {
+                const double us = Us[buf][kk];
+#pragma unroll
+                for (int j = 0; j < TN; j++) {
+                    br[j] *= us;
+                    if constexpr (Z) bi[j] *= us;
+                }
+            }
+            // (AB)^T = B^T A^T; complex: re = Br Ar - Bi Ai, im = Bi Ar + Br Ai
+#pragma unroll
+            for (int i = 0; i < TM; i++)
+#pragma unroll
+                for (int j = 0; j < TN; j++) acc[0][i][j] = mfma(br[j], ar[i], acc[0][i][j]);
+            if constexpr (Z) {
+#pragma unroll
+                for (int i = 0; i < TM; i++)
+#pragma unroll
+                    for (int j = 0; j < TN; j++) acc[NP - 1][i][j] = mfma(bi[j], ar[i], acc[NP - 1][i][j]);
+#pragma unroll
+                for (int i = 0; i < TM; i++)
+#pragma unroll
+                    for (int j = 0; j < TN; j++) acc[0][i][j] = mfma(-bi[j], ai[i], acc[0][i][j]);
+#pragma unroll
+                for (int i = 0; i < TM; i++)
+#pragma unroll
+                    for (int j = 0; j < TN; j++) acc[NP - 1][i][j] = mfma(br[j], ai[i], acc[NP - 1][i][j]);
+            }
+        }
+    };
+
+    const int nk = (g.K + BK - 1) / BK;
+    const double* rr = g.r ? g.r + (long)b * g.sr : nullptr;
+    const double* ww = g.w ? g.w + (long)b * g.sw : nullptr;
+    gload(0);
+    sstore(0);
+    __syncthreads();
+    for (int kt = 0; kt < nk; kt++) {
+        const int cur = kt & 1;
+        if (kt + 1 < nk) gload((kt + 1) * BK);
+        compute(cur);
+        if (kt + 1 < nk) sstore(cur ^ 1);
+        __syncthreads();
+    }
+
+#pragma unroll
+    for (int i = 0; i < TM; i++)
+#pragma unroll
+        for (int j = 0; j < TN; j++)
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                const int row = m0 + wm + 16 * i + l16;
+                const int col = n0 + wn + 16 * j + 4 * e + l4;
+                if (row < g.M && col < g.N) {
+                    T v;
+                    if constexpr (Z) v = mkz(acc[0][i][j][e], acc[NP - 1][i][j][e]);
+                    else v = acc[0][i][j][e];
+                    v = g.alpha * v;
+                    if (rr) v = v * rr[row];
+                    if (ww) v = v * ww[col];
+                    const long o = row + (long)g.ldc * col;
+                    if constexpr (BETA) v += g.beta * C[o];
+                    C[o] = v;
+                }
+            }
+}
+
+template <typename T, int TM, int TN, int BK>
+static hipError_t gemm_launch_tile(int opa, int opb, const GemmArgs<T>& g, int batch, hipStream_t s)
+{
+    constexpr int BM = 32 * TM, BN = 32 * TN;
+    const int tm = (g.M + BM - 1) / BM, tn = (g.N + BN - 1) / BN;
+    const long nwg = (long)tm * tn * batch;
+    if (nwg <= 0) return hipSuccess;
+    if (nwg > 0x7fffffffL) return hipErrorInvalidValue;
+    const int n = (int)nwg;
+#define GL(a, b)                                                              \
+    (g.beta_zero ? k_gemm<T, a, b, TM, TN, BK, false><<<n, 256, 0, s>>>(g, tm, tn, n) \
+                 : k_gemm<T, a, b, TM, TN, BK, true><<<n, 256, 0, s>>>(g, tm, tn, n))
+    if constexpr (!is_cplx<T>::value) {
+        // conjugation is the identity on real data
+        const bool ta = opa != OP_N, tb = opb != OP_N;
+        if (!ta && !tb) GL(OP_N, OP_N);
+        else if (!ta) GL(OP_N, OP_T);
+        else if (!tb) GL(OP_T, OP_N);
+        else GL(OP_T, OP_T);
+    } else {
         if (opa == OP_N) {
-            if (opb == OP_N) ZL(OP_N, OP_N);
-            else if (opb == OP_T) ZL(OP_N, OP_T);
-            else ZL(OP_N, OP_C);
+            if (opb == OP_N) GL(OP_N, OP_N);
+            else if (opb == OP_T) GL(OP_N, OP_T);
+            else GL(OP_N, OP_C);
         } else if (opa == OP_T) {
-            if (opb == OP_N) ZL(OP_T, OP_N);
-            else if (opb == OP_T) ZL(OP_T, OP_T);
-            else ZL(OP_T, OP_C);
+            if (opb == OP_N) GL(OP_T, OP_N);
+            else if (opb == OP_T) GL(OP_T, OP_T);
+            else GL(OP_T, OP_C);
         } else {
-            if (opb == OP_N) ZL(OP_C, OP_N);
-            else if (opb == OP_T) ZL(OP_C, OP_T);
-            else ZL(OP_C, OP_C);
+            if (opb == OP_N) GL(OP_C, OP_N);
+            else if (opb == OP_T) GL(OP_C, OP_T);
+            else GL(OP_C, OP_C);
         }
-#undef ZL
-        return hipGetLastError();
     }
-};
+#undef GL
+    return hipGetLastError();
+}
+
+// Launch with tile shape `tile` (GT_*); GT_AUTO and shapes a product is too
+// small for fall back to 64 x 64.  GT_128 (128 x 128) is accepted as a request
+// but runs as 128 x 64: its accumulators left one wave per SIMD, and it was
+// the slowest shape at every size measured.
+template <typename T>
+static hipError_t gemm_launch(int opa, int opb, const GemmArgs<T>& g, int batch, hipStream_t s,
+                              int tile)
+{
+    constexpr bool Z = is_cplx<T>::value;
+    if ((tile == GT_128x64 || tile == GT_128) && g.M > 64)
+        return gemm_launch_tile<T, 4, 2, 8>(opa, opb, g, batch, s);
+    return gemm_launch_tile<T, 2, 2, Z ? 8 : 16>(opa, opb, g, batch, s);
+}

@@ -173,6 +173,7 @@ struct dqmc_engine {
     void *G0t = 0, *Gtt = 0, *Gt0 = 0;                      // [B][L][NN]
     void *Gred = 0, *Wa = 0, *Wb = 0, *Qb = 0, *taub = 0, *T2 = 0;
     int gemm_adiv = 1, gemm_bdiv = 1;  // one-shot batch divisors for the next gemm()
+    int gemm_tile = GT_64;             // GEMM C-tile shape (GT_*)
     // blocked QR workspace: explicit V, compact-WY T factors, GEMM scratch, dets
     void *Vb = 0, *Tb = 0, *Wq1 = 0, *Wq2 = 0, *detq = 0, *detm = 0;
     long sVb = 0, sTb = 0, sWq = 0;
@@ -238,7 +239,7 @@ static int gemm(dqmc_engine* e, int opa, int opb, int M, int N, int K, T alpha, 
         hipEventCreate(&a1);
         hipEventRecord(a0, e->st);
     }
-    hipError_t err = GemmLauncher<T>::launch(opa, opb, g, batch, e->st);
+    hipError_t err = gemm_launch<T>(opa, opb, g, batch, e->st, e->gemm_tile);
     if (timed) {
         hipEventRecord(a1, e->st);
         e->gemm_events.push_back({a0, a1});
@@ -886,6 +887,49 @@ static int engine_setup(dqmc_engine* e, const dqmc_params* const* P)
     return 0;
 }
 
+// Times every GEMM tile shape on the engine's dominant product (N x N x N over
+// all 2 n_chains matrices: the wraps and the B-chain products) and keeps the
+// fastest.  Operands are the engine's own buffers; their contents are
+// irrelevant and the output buffer is scratch at this point.
+template <typename T>
+static int autotune_gemm(dqmc_engine* e)
+{
+    const int N = e->N, B = 2 * e->nc;
+    const long NN = (long)N * N;
+    const int cand[3] = {GT_64, GT_128x64, GT_128};
+    const int ncand = is_cplx<T>::value ? 2 : 3;
+    hipEvent_t a0, a1;
+    HIPCHK(hipEventCreate(&a0));
+    HIPCHK(hipEventCreate(&a1));
+    float best = 1e30f;
+    int pick = GT_64;
+    int rc = 0;
+    for (int c = 0; c < ncand && !rc; c++) {
+        e->gemm_tile = cand[c];
+        const int reps = 3;
+        for (int r = 0; r <= reps && !rc; r++) {
+            if (r == 1) hipEventRecord(a0, e->st);
+            rc = gemm<T>(e, OP_N, OP_N, N, N, N, from_re<T>(1.0), dp<T>(e->expK), N, NN,
+                         dp<T>(e->G), N, NN, from_re<T>(0.0), dp<T>(e->X), N, NN, B);
+        }
+        if (rc) break;
+        hipEventRecord(a1, e->st);
+        if (hipEventSynchronize(a1) != hipSuccess) { rc = set_err(DQMC_ERR_HIP, "autotune sync"); break; }
+        float ms = 0.f;
+        hipEventElapsedTime(&ms, a0, a1);
+        if (ms < best) { best = ms; pick = cand[c]; }
+    }
+    hipEventDestroy(a0);
+    hipEventDestroy(a1);
+    e->gemm_tile = pick;
+    return rc;
+}
+
+extern "C" int dqmc_engine_gemm_tile(dqmc_engine* e)
+{
+    return e ? e->gemm_tile : 0;
+}
+
 extern "C" int dqmc_engine_create(dqmc_engine** out, const dqmc_params* const* P, int n_chains,
                                   int device, const dqmc_tuning* tune)
 {
@@ -925,6 +969,12 @@ extern "C" int dqmc_engine_create(dqmc_engine** out, const dqmc_params* const* P
         return set_err(DQMC_ERR_HIP, "hipStreamCreate failed");
     }
     int rc = engine_setup(e, P);
+    if (!rc) {
+        const int t = tune ? tune->gemm_tile : 0;
+        if (t < 0 || t > GT_128x64) rc = set_err(DQMC_ERR_ARG, "bad gemm_tile %d", t);
+        else if (t > 0) e->gemm_tile = t;
+        else rc = e->cplx ? autotune_gemm<zd>(e) : autotune_gemm<double>(e);
+    }
     if (rc) {
         dqmc_engine_destroy(e);
         return rc;
@@ -1133,10 +1183,12 @@ extern "C" int dqmc_engine_gemm_stats(dqmc_engine* e, double* avg_ms, int64_t* l
 template <typename T>
 static int test_gemm_t(int opa, int opb, int M, int N, int K, const double* alpha,
                        const double* A, int lda, const double* B, int ldb, const double* beta,
-                       double* Cm, int ldc, const double* r, const double* u, const double* w)
+                       double* Cm, int ldc, const double* r, const double* u, const double* w,
+                       int tile)
 {
     dqmc_engine tmp;
     dqmc_engine* e = &tmp;
+    e->gemm_tile = tile;
     HIPCHK(hipStreamCreate(&e->st));
     const int ka = (opa == OP_N) ? K : M, kb = (opb == OP_N) ? N : K;
     void *dA, *dB, *dC, *dr = 0, *du = 0, *dw = 0;
@@ -1166,9 +1218,12 @@ static int test_gemm_t(int opa, int opb, int M, int N, int K, const double* alph
 extern "C" int dqmc_test_gemm(int is_complex, int opa, int opb, int M, int N, int K,
                               const double* alpha, const double* A, int lda, const double* B,
                               int ldb, const double* beta, double* Cm, int ldc, const double* r,
-                              const double* u, const double* w)
+                              const double* u, const double* w, int tile)
 {
     if (M <= 0 || N <= 0 || K <= 0) return set_err(DQMC_ERR_ARG, "bad GEMM shape");
-    return is_complex ? test_gemm_t<zd>(opa, opb, M, N, K, alpha, A, lda, B, ldb, beta, Cm, ldc, r, u, w)
-                      : test_gemm_t<double>(opa, opb, M, N, K, alpha, A, lda, B, ldb, beta, Cm, ldc, r, u, w);
+    if (tile < 0 || tile > GT_128x64) return set_err(DQMC_ERR_ARG, "bad tile %d", tile);
+    return is_complex
+               ? test_gemm_t<zd>(opa, opb, M, N, K, alpha, A, lda, B, ldb, beta, Cm, ldc, r, u, w, tile)
+               : test_gemm_t<double>(opa, opb, M, N, K, alpha, A, lda, B, ldb, beta, Cm, ldc, r, u, w,
+                                     tile);
 }

@@ -18,7 +18,7 @@ _lib = None
 
 
 class DqmcTuning(C.Structure):
-    _fields_ = [("delay_rank", C.c_int), ("flags", C.c_int)]
+    _fields_ = [("delay_rank", C.c_int), ("flags", C.c_int), ("gemm_tile", C.c_int)]
 
 
 def lib():
@@ -51,6 +51,8 @@ def lib():
         L.dqmc_version.restype = C.c_char_p
         L.dqmc_acc_len.argtypes = [P(capi.DqmcParams), C.c_int, C.c_int]
         L.dqmc_acc_len.restype = C.c_long
+        L.dqmc_engine_gemm_tile.argtypes = [C.c_void_p]
+        L.dqmc_engine_gemm_tile.restype = C.c_int
         for f in ("set_state", "get_state", "set_meas", "get_meas", "run", "green", "calc_eq_g",
                   "calc_ue_g", "gemm_stats"):
             getattr(L, "dqmc_engine_" + f).restype = C.c_int
@@ -79,7 +81,7 @@ class Engine:
     "meas_uneqlt"}) -- all with the same N, L, n_matmul, dtype and
     measurement set.  State and accumulators are uploaded at construction."""
 
-    def __init__(self, sims, device=0, delay_rank=0, time_gemm=False):
+    def __init__(self, sims, device=0, delay_rank=0, time_gemm=False, gemm_tile=0):
         self.L = lib()
         self.sims = sims
         self.cplx = capi.is_complex(sims[0])
@@ -93,7 +95,7 @@ class Engine:
             structs.append(C.pointer(ps))
         arr = (C.POINTER(capi.DqmcParams) * len(sims))(*structs)
         self._arr = arr
-        tune = DqmcTuning(delay_rank, 1 if time_gemm else 0)
+        tune = DqmcTuning(delay_rank, 1 if time_gemm else 0, gemm_tile)
         h = C.c_void_p()
         _check(self.L.dqmc_engine_create(C.byref(h), arr, len(sims), device, C.byref(tune)),
                "dqmc_engine_create")
@@ -183,6 +185,10 @@ class Engine:
 
     def accepts(self):
         return int(self.L.dqmc_engine_accepts(self.h))
+
+    def gemm_tile(self):
+        """GEMM C-tile shape in use (1 = 64x64, 2 = 128x128, 3 = 128x64)."""
+        return self.L.dqmc_engine_gemm_tile(self.h)
 
     def gemm_stats(self):
         ms = C.c_double(0)

@@ -112,7 +112,9 @@ typedef struct dqmc_engine dqmc_engine;
 /* Tuning knobs picked by the autotuner; zero fields mean "engine default". */
 typedef struct dqmc_tuning {
     int delay_rank; /* rank of the delayed Metropolis update (register slots) */
-    int flags;      /* reserved */
+    int flags;      /* bit 0: time the N x N x N GEMMs with HIP events */
+    int gemm_tile;  /* C tile of the MFMA GEMM: 0 = time the shapes at creation
+                       and keep the fastest, 1 = 64x64, 2 = 128x128, 3 = 128x64 */
 } dqmc_tuning;
 
 /* Create an engine for n_chains chains on HIP device `device`.  params[c]
@@ -170,10 +172,15 @@ int dqmc_engine_gemm_stats(dqmc_engine *e, double *avg_ms, int64_t *launches,
  * the engine's FP64 MFMA kernel on host buffers,
  *   C = alpha diag(r) op(A) diag(u) op(B) diag(w) + beta C
  * op: 0 = N, 1 = T, 2 = conjugate transpose; alpha/beta are num; r/u/w may
- * be NULL.  Used by the parity tests of the GEMM kernel. */
+ * be NULL; tile as dqmc_tuning.gemm_tile (0 = 64x64).  Used by the parity
+ * tests of the GEMM kernel. */
 int dqmc_test_gemm(int is_complex, int opa, int opb, int M, int N, int K, const double *alpha,
                    const double *A, int lda, const double *B, int ldb, const double *beta,
-                   double *C, int ldc, const double *r, const double *u, const double *w);
+                   double *C, int ldc, const double *r, const double *u, const double *w,
+                   int tile);
+
+/* The GEMM tile shape the engine uses (dqmc_tuning.gemm_tile values 1..3). */
+int dqmc_engine_gemm_tile(dqmc_engine *e);
 
 const char *dqmc_last_error(void);
 const char *dqmc_version(void);

@@ -21,10 +21,11 @@ def rel(a, b):
     return np.abs(np.asarray(a) - np.asarray(b)).max() / max(np.abs(np.asarray(b)).max(), 1e-300)
 
 
-def _gemm(cplx, opa, opb, M, N, K, rs):
+def _gemm(cplx, opa, opb, M, N, K, rs, tile=0, scaled=True):
     L = engine.lib()
     P = C.POINTER(C.c_double)
-    L.dqmc_test_gemm.argtypes = [C.c_int] * 6 + [P, P, C.c_int, P, C.c_int, P, P, C.c_int, P, P, P]
+    L.dqmc_test_gemm.argtypes = ([C.c_int] * 6 + [P, P, C.c_int, P, C.c_int, P, P, C.c_int, P, P, P]
+                                 + [C.c_int])
     dt = np.complex128 if cplx else np.float64
 
     def rnd(*s):
@@ -38,25 +39,33 @@ def _gemm(cplx, opa, opb, M, N, K, rs):
     r, u, w = rs.rand(M) + .5, rs.rand(K) + .5, rs.rand(N) + .5
     al = np.array([1.3, -0.2] if cplx else [1.3])
     be = np.array([0.7, 0.1] if cplx else [0.7])
+    if not scaled:
+        r, u, w = np.ones(M), np.ones(K), np.ones(N)
+        be = np.zeros_like(be)
     alc = complex(*al) if cplx else al[0]
     bec = complex(*be) if cplx else be[0]
     ref = alc * np.diag(r) @ op(A, opa) @ np.diag(u) @ op(B, opb) @ np.diag(w) + bec * Cm
     Af, Bf, Cf = (np.asfortranarray(X.astype(dt)) for X in (A, B, Cm))
     dp = lambda a: a.ctypes.data_as(P)
+    sc = (lambda a: dp(a)) if scaled else (lambda a: None)
     rc = L.dqmc_test_gemm(int(cplx), opa, opb, M, N, K, dp(al), dp(Af), Af.shape[0], dp(Bf),
-                          Bf.shape[0], dp(be), dp(Cf), M, dp(r), dp(u), dp(w))
+                          Bf.shape[0], dp(be), dp(Cf), M, sc(r), sc(u), sc(w), tile)
     assert rc == 0
     return rel(Cf, ref)
 
 
 @pytest.mark.parametrize("cplx", [0, 1])
-def test_gemm_kernel(cplx):
+@pytest.mark.parametrize("tile", [1, 2, 3])
+def test_gemm_kernel(cplx, tile):
+    """Every tile shape, every op pair, ragged edges (sizes not multiples of
+    the tile or of the K step), and the unscaled beta = 0 path."""
     rs = np.random.RandomState(0)
     ops = (0, 1, 2) if cplx else (0, 1)
-    for (M, N, K) in [(16, 16, 16), (70, 33, 45), (256, 256, 256), (1, 130, 7)]:
+    for (M, N, K) in [(16, 16, 16), (70, 33, 45), (256, 256, 256), (1, 130, 7), (200, 144, 77)]:
         for a in ops:
             for b in ops:
-                assert _gemm(cplx, a, b, M, N, K, rs) < 1e-14
+                assert _gemm(cplx, a, b, M, N, K, rs, tile) < 1e-14, (M, N, K, a, b)
+    assert _gemm(cplx, 0, 0, 144, 144, 144, rs, tile, scaled=False) < 1e-14
 
 
 def test_golden_trajectory_on_gpu(golden):
