@@ -2,7 +2,9 @@
 
 Workload (the reference's headline configuration): 16x16 periodic square
 Hubbard model (N=256), t=1, t'=-0.25, U=6, mu=-1, beta=8, dt=0.1 (L=80),
-n_matmul=10, delay rank 32, equal-time measurements every 8 slices,
+n_matmul=10, delay rank 32 requested (the N <= 256 Metropolis kernel keeps 16
+register-resident delay slots; the rank changes only the rounding of G, not
+the Markov chain), equal-time measurements every 8 slices,
 unequal-time G(t,0)/G(0,t) + susceptibilities every 2 sweeps, real FP64.
 A "step" is one full sweep (L slices of Metropolis updates + wraps/recalcs +
 measurements) of every chain in the batch.  `value` is the aggregate
@@ -10,7 +12,9 @@ sweeps/s of the batched engine (chains x steps / seconds); the plain
 single-chain form is timed beside it.  Inputs are generated on the host by
 the package's generator (seeded, synthetic), uploaded before timing.
 
-Prints one JSON line (see the contract in the repository's instructions).
+Prints one JSON line: metric/value/unit, timing, config, the roofline of the
+dominant kernel (the N x N x N batched GEMM, timed with HIP events on the
+engine stream) and the CPU baseline (the oracle on the host cores).
 """
 import argparse
 import json
@@ -22,6 +26,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "dqmc-dev_amd"))
 
 PEAK_FP64_TFLOPS = 78.6  # MI355X dense FP64 (matrix = vector)
+# HBM bytes per 256 x 256 x 256 matrix product of the dominant GEMM, from the
+# FETCH_SIZE / WRITE_SIZE counter passes scaled by a 512 MiB copy
+# (profiles/k_gemm_N256_L80_256chains.md): 1.0012 x the 3 x 512 KiB it must move.
+GEMM_HBM_BYTES_PER_MATRIX_N256 = 1574796
 
 
 def workload(nchains, seed=7):
@@ -32,8 +40,9 @@ def workload(nchains, seed=7):
 
 
 def flop_model(N, L, F, n_matmul, accepts_per_sweep, n_delay, q_eq, ue_fraction):
-    """Algorithmic FP64 flops per sweep per chain (both spins), the SURVEY
-    model: wraps 4(L-F) N^3, recalc F[2(n-1) + Q(F)] N^3, updates, eq and ue."""
+    """Algorithmic FP64 flops per sweep per chain (both spins): standard
+    LAPACK/BLAS op counts applied to the reference's call sequence -- wraps
+    4(L-F) N^3, recalc F[2(n-1) + Q(F)] N^3, updates, eq and ue measurements."""
     N3 = float(N) ** 3
     E = 1 + (F - 1) // 2
     s0 = 1 + (F - 1) % 2
@@ -71,6 +80,9 @@ def main():
     ap.add_argument("--chains", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-single", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MB",
+                    help="first run one device-to-device copy of MB MiB (a known-size kernel that "
+                         "calibrates the FETCH_SIZE / WRITE_SIZE counters of a profiler pass)")
     ap.add_argument("--gemm-tile", type=int, default=0,
                     help="GEMM C tile: 0 = autotune, 1 = 64x64, 2 = 128x128, 3 = 128x64")
     args = ap.parse_args()
@@ -80,6 +92,15 @@ def main():
         sys.exit(2)
     import numpy as np
     from dqmc_amd import engine
+
+    if args.copy_probe > 0:
+        import torch
+        src = torch.ones(args.copy_probe << 17, dtype=torch.float64, device="cuda:0")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     tiles = {1: "64x64", 2: "128x128", 3: "128x64"}
 
@@ -125,7 +146,8 @@ def main():
         "dtype": "f64",
         "data": "synthetic (package generator, seed 7 + jump per chain)",
         "config": {"workload": "16x16 square Hubbard N=256 L=80 t'=-0.25 U=6 mu=-1 beta=8 "
-                               "n_matmul=10 delay=32 eq every 8 slices, uneq every 2 sweeps",
+                               "n_matmul=10 delay=32 (16 register slots) eq every 8 slices, "
+                               "uneq every 2 sweeps",
                    "chains": nch, "parallelism": "1 GPU, chains batched"},
         "single_chain_sweeps_per_s": None if single is None else round(single, 4),
         "batched_over_single": None if single is None else round(value / single, 3),
@@ -136,7 +158,8 @@ def main():
         "roofline": {"bound": "mfma",
                      "kernel": "k_gemm<double,N,N> tile %s (N x N x N, batch 2*chains)" % tiles[tile],
                      "achieved": round(achieved, 3), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                     "frac": round(achieved / PEAK_FP64_TFLOPS, 4), "traffic": None,
+                     "frac": round(achieved / PEAK_FP64_TFLOPS, 4),
+                     "traffic": GEMM_HBM_BYTES_PER_MATRIX_N256 * 2 * nch,
                      "avg_launch_ms": round(g_ms, 5), "launches": g_n},
     }
     if not args.no_cpu:

@@ -2,7 +2,7 @@
 reference's golden data.
 
 Tolerances: integer HS fields and RNG states bit-exact; equal-time G 1e-10
-relative max-norm; unequal-time blocks 1e-8 relative
+relative max-norm (the bound the path is held to); unequal-time blocks 1e-8 relative
 max-norm (products over up to beta/2 in FP64, both sides carry ~1e-10 of
 rounding); accumulators 1e-9 relative to the largest entry.
 """
