@@ -1,0 +1,368 @@
+/*
+ * dq_driver.c -- dqmc_wrapper (src/dqmc.c:513-671) on the GPU engine.
+ *
+ * Per file, the log carries the same milestones as the reference's wrapper:
+ * header, memory requirement, dry-run exit, open/read, checkpoint policy,
+ * progress, interruption, save and wall time.  The sweep loop is
+ * src/dqmc.c:222-459 with the engine doing the sweeps:
+ *   - before every sweep the stop flag is polled (the engine does it between
+ *     sweeps); a signal or the time limit ends the run with INTERRUPTED, and
+ *     nothing is saved (the file keeps its last checkpoint);
+ *   - a chain whose sweep is a multiple of checkpoint_every is saved before
+ *     that sweep runs (src/dqmc.c:233-244);
+ *   - after the last sweep the file is saved unless in benchmark mode.
+ * Files of one shape share an engine (one chain each); each shape group runs
+ * in its own host thread with its own engine and stream.
+ */
+#define _GNU_SOURCE
+#include "dq_driver.h"
+
+#include <limits.h>
+#include <pthread.h>
+#include <signal.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <sys/stat.h>
+#include <time.h>
+
+#include "dq_h5.h"
+#include "dqmc_hip.h"
+
+#define MAX_CHAINS_PER_ENGINE 256
+
+static volatile sig_atomic_t g_stop = 0; /* signal number, or -1 at the time limit */
+
+static void on_stop(int s) { g_stop = s; }
+
+void dq_sig_init(void)
+{
+    static int done = 0;
+    if (done) return;
+    done = 1;
+    struct sigaction sa;
+    memset(&sa, 0, sizeof sa);
+    sa.sa_handler = on_stop;
+    sigaction(SIGUSR1, &sa, NULL);
+    sigaction(SIGINT, &sa, NULL);
+    sigaction(SIGTERM, &sa, NULL);
+    sigaction(SIGHUP, &sa, NULL);
+}
+
+int64_t dq_time_ns(void)
+{
+    struct timespec t;
+    clock_gettime(CLOCK_MONOTONIC, &t);
+    return (int64_t)t.tv_sec * 1000000000LL + t.tv_nsec;
+}
+
+typedef struct job {
+    const char *file;
+    FILE *log;
+    int code;
+    int loaded, failed, interrupted;
+    int last_ckpt;
+    dq_sim sim;
+} job;
+
+typedef struct group {
+    job **j;
+    int n;
+} group;
+
+static pthread_mutex_t g_log_mu = PTHREAD_MUTEX_INITIALIZER;
+
+static void logf_(job *j, const char *fmt, ...) __attribute__((format(printf, 2, 3)));
+static void logf_(job *j, const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    pthread_mutex_lock(&g_log_mu);
+    vfprintf(j->log, fmt, ap);
+    fflush(j->log);
+    pthread_mutex_unlock(&g_log_mu);
+    va_end(ap);
+}
+
+static void now_str(char *buf, size_t n)
+{
+    time_t t = time(NULL);
+    struct tm tmv;
+    localtime_r(&t, &tmv);
+    strftime(buf, n, "%a %b %d %H:%M:%S %Y", &tmv);
+}
+
+static int same_shape(const dqmc_params *a, const dqmc_params *b)
+{
+    return a->N == b->N && a->L == b->L && a->F == b->F && a->n_matmul == b->n_matmul &&
+           a->is_complex == b->is_complex && a->period_eqlt == b->period_eqlt &&
+           a->period_uneqlt == b->period_uneqlt && a->num_i == b->num_i &&
+           a->num_ij == b->num_ij && a->meas_energy_corr == b->meas_energy_corr &&
+           a->meas_bond_corr == b->meas_bond_corr && a->meas_thermal == b->meas_thermal &&
+           a->meas_2bond_corr == b->meas_2bond_corr && a->meas_chiral == b->meas_chiral &&
+           a->meas_gen_suscept == b->meas_gen_suscept && a->meas_local_JQ == b->meas_local_JQ &&
+           a->meas_nematic_corr == b->meas_nematic_corr &&
+           a->meas_pair_bb_only == b->meas_pair_bb_only && a->num_b == b->num_b &&
+           a->num_bb == b->num_bb && a->num_bs == b->num_bs && a->num_b2 == b->num_b2 &&
+           a->num_b2b2 == b->num_b2b2 && a->num_b2b == b->num_b2b && a->num_bb2 == b->num_bb2;
+}
+
+static int fetch(dqmc_engine *e, int c, job *j)
+{
+    int rc = dqmc_engine_get_state(e, c, j->sim.rng, &j->sim.sweep, j->sim.hs);
+    if (!rc)
+        rc = dqmc_engine_get_meas(e, c, &j->sim.meq,
+                                  j->sim.p.period_uneqlt > 0 ? &j->sim.mue : NULL);
+    return rc;
+}
+
+static void *run_group(void *arg)
+{
+    group *g = (group *)arg;
+    const int n = g->n;
+    const dqmc_params **P = calloc(n, sizeof *P);
+    for (int c = 0; c < n; c++) P[c] = &g->j[c]->sim.p;
+    dqmc_engine *e = NULL;
+    dqmc_tuning tune = {0, 0, 0};
+    int rc = dqmc_engine_create(&e, P, n, 0, &tune);
+    free(P);
+    if (rc) {
+        for (int c = 0; c < n; c++) {
+            logf_(g->j[c], "engine creation failed: %s\n", dqmc_last_error());
+            g->j[c]->code = WRAP_FAIL;
+        }
+        return NULL;
+    }
+    for (int c = 0; c < n && !rc; c++) {
+        job *j = g->j[c];
+        rc = dqmc_engine_set_state(e, c, j->sim.rng, j->sim.sweep, j->sim.hs);
+        if (!rc)
+            rc = dqmc_engine_set_meas(e, c, &j->sim.meq,
+                                      j->sim.p.period_uneqlt > 0 ? &j->sim.mue : NULL);
+    }
+    if (rc) {
+        for (int c = 0; c < n; c++) {
+            logf_(g->j[c], "engine upload failed: %s\n", dqmc_last_error());
+            g->j[c]->code = WRAP_FAIL;
+        }
+        dqmc_engine_destroy(e);
+        return NULL;
+    }
+    char ts[64];
+    now_str(ts, sizeof ts);
+    for (int c = 0; c < n; c++)
+        logf_(g->j[c], "[%s] starting dqmc (MI355X engine, %d chain%s in this batch)\n", ts, n,
+              n == 1 ? "" : "s");
+
+    int interrupted = 0, failed = 0;
+    for (;;) {
+        int any = 0, kmin = INT_MAX;
+        if (g_stop) {
+            interrupted = 1;
+            break;
+        }
+        for (int c = 0; c < n; c++) {
+            job *j = g->j[c];
+            const int sw = j->sim.sweep, ns = j->sim.p.n_sweep, ce = j->sim.p.checkpoint_every;
+            if (j->failed || sw >= ns) continue;
+            any = 1;
+            if (ce > 0 && sw % ce == 0 && j->last_ckpt != sw) {
+                j->last_ckpt = sw;
+                if (fetch(e, c, j) != 0 || dq_sim_save(&j->sim) < 0) {
+                    logf_(j, "dqmc() checkpoint failed\n");
+                    j->failed = 1;
+                    j->code = WRAP_FAIL;
+                    continue;
+                }
+            }
+            int k = ns - sw;
+            if (ce > 0 && ce - sw % ce < k) k = ce - sw % ce;
+            if (k < kmin) kmin = k;
+        }
+        if (!any || kmin == INT_MAX) break;
+        rc = dqmc_engine_run(e, kmin, (const volatile int *)&g_stop);
+        for (int c = 0; c < n; c++) dqmc_engine_get_state(e, c, NULL, &g->j[c]->sim.sweep, NULL);
+        if (rc == DQMC_INTERRUPTED) {
+            interrupted = 1;
+            break;
+        }
+        if (rc < 0) {
+            failed = 1;
+            break;
+        }
+    }
+    const char *why = g_stop == -1 ? "time limit reached" : "signal received";
+    for (int c = 0; c < n; c++) {
+        job *j = g->j[c];
+        if (j->failed) continue;
+        if (failed) {
+            logf_(j, "engine run failed: %s\n", dqmc_last_error());
+            j->code = WRAP_FAIL;
+            continue;
+        }
+        if (fetch(e, c, j) != 0) {
+            logf_(j, "engine download failed: %s\n", dqmc_last_error());
+            j->code = WRAP_FAIL;
+            continue;
+        }
+        if (j->sim.sweep < j->sim.p.n_sweep && interrupted) {
+            logf_(j, "%d/%d sweeps completed (%s)\n", j->sim.sweep, j->sim.p.n_sweep,
+                  j->sim.sweep >= j->sim.p.n_sweep_warm ? "measuring" : "warming up");
+            logf_(j, "stop_flag = %d (%s), stopping\n", (int)g_stop, why);
+            logf_(j, "dqmc() interrupted\n");
+            j->interrupted = 1;
+            j->code = WRAP_INCOMPLETE;
+        }
+    }
+    now_str(ts, sizeof ts);
+    for (int c = 0; c < n; c++) {
+        job *j = g->j[c];
+        if (j->failed || j->interrupted || j->code == WRAP_FAIL) continue;
+        logf_(j, "[%s] %d/%d sweeps completed\n", ts, j->sim.sweep, j->sim.p.n_sweep);
+    }
+    dqmc_engine_destroy(e);
+    return NULL;
+}
+
+void dqmc_wrapper_batch(int n, const char *const *files, const char *const *logs,
+                        int64_t max_time_ns, bool dry, bool bench, int *codes)
+{
+    const int64_t wall_start = dq_time_ns();
+    job *jobs = calloc(n, sizeof *jobs);
+    for (int i = 0; i < n; i++) {
+        job *j = &jobs[i];
+        j->file = files[i];
+        j->code = 0;
+        j->last_ckpt = -1;
+        j->log = (logs && logs[i]) ? fopen(logs[i], "a") : stdout;
+        if (!j->log) {
+            fprintf(stderr, "fopen() failed to open: %s\n", logs[i]);
+            j->log = NULL;
+            j->code = WRAP_FAIL;
+            continue;
+        }
+        logf_(j, "executable %s\n", dqmc_version());
+        char commit[256];
+        if (dq_sim_commit(j->file, commit, sizeof commit) == 0)
+            logf_(j, "hdf5 generation script commit id %s\n", commit);
+        logf_(j, "integer size on this system: %zu bytes\n", sizeof(int));
+        const int64_t mem = dq_sim_memory_req(j->file);
+        if (mem < 0) logf_(j, "minimum RAM check failed: %lld\n", (long long)mem);
+        else logf_(j, "minimum RAM requirement: %.2f MB\n", (double)mem * 1e-6);
+        if (dry) {
+            logf_(j, "toggled dry run for mem check only\n");
+            logf_(j, "Not performing: read sim file, alloc mem, run core DQMC logic, or save data\n");
+            continue;
+        }
+        logf_(j, "opening %s\n", j->file);
+        struct stat info;
+        if (stat(j->file, &info) == 0)
+            logf_(j, "size of this hdf5 file: %.2f MB\n", (double)info.st_size * 1e-6);
+        const int rs = dq_sim_read(j->file, &j->sim);
+        if (rs) {
+            logf_(j, "sim_data_read_alloc() failed: %d\n", rs);
+            j->code = WRAP_FAIL;
+            dq_sim_free(&j->sim);
+            continue;
+        }
+        j->loaded = 1;
+        if (j->sim.p.checkpoint_every)
+            logf_(j, "checkpointing every %d sweeps\n", j->sim.p.checkpoint_every);
+        else
+            logf_(j, "warning: running with no checkpoints\n");
+        logf_(j, "%d/%d sweeps completed\n", j->sim.sweep, j->sim.p.n_sweep);
+        if (j->sim.sweep > j->sim.p.n_sweep) {
+            logf_(j, "sweep > n_sweep in %s\n", j->file);
+            j->code = WRAP_FAIL;
+        } else if (j->sim.sweep == j->sim.p.n_sweep) {
+            logf_(j, "already finished so don't do anything\n");
+        }
+    }
+
+    if (!dry) {
+        dq_sig_init();
+        /* group the runnable files by shape */
+        group *gs = calloc(n > 0 ? n : 1, sizeof *gs);
+        int ng = 0;
+        for (int i = 0; i < n; i++) {
+            job *j = &jobs[i];
+            if (!j->loaded || j->code != 0 || j->sim.sweep >= j->sim.p.n_sweep) continue;
+            int gi = -1;
+            for (int q = 0; q < ng; q++)
+                if (gs[q].n < MAX_CHAINS_PER_ENGINE && same_shape(&gs[q].j[0]->sim.p, &j->sim.p)) {
+                    gi = q;
+                    break;
+                }
+            if (gi < 0) {
+                gi = ng++;
+                gs[gi].j = calloc(n, sizeof(job *));
+            }
+            gs[gi].j[gs[gi].n++] = j;
+        }
+        pthread_t *th = calloc(ng > 0 ? ng : 1, sizeof *th);
+        int *joined = calloc(ng > 0 ? ng : 1, sizeof *joined);
+        const int64_t deadline = max_time_ns > 0 ? wall_start + max_time_ns : 0;
+        for (int q = 0; q < ng; q++) pthread_create(&th[q], NULL, run_group, &gs[q]);
+        if (deadline) {
+            /* the time limit is a stop flag raised here; the engine polls the
+             * flag before each sweep (sig_check_state, src/sig.c:70-75) */
+            int left = ng;
+            while (left > 0) {
+                for (int q = 0; q < ng; q++)
+                    if (!joined[q] && pthread_tryjoin_np(th[q], NULL) == 0) {
+                        joined[q] = 1;
+                        left--;
+                    }
+                if (left == 0) break;
+                if (dq_time_ns() >= deadline && !g_stop) g_stop = -1;
+                struct timespec ts = {0, 20 * 1000 * 1000};
+                nanosleep(&ts, NULL);
+            }
+        } else {
+            for (int q = 0; q < ng; q++) pthread_join(th[q], NULL);
+        }
+        free(joined);
+        for (int q = 0; q < ng; q++) free(gs[q].j);
+        free(gs);
+        free(th);
+
+        for (int i = 0; i < n; i++) {
+            job *j = &jobs[i];
+            if (!j->loaded || j->code != 0 || j->interrupted) continue;
+            if (bench) {
+                logf_(j, "benchmark mode enabled; not saving data to disk\n");
+                continue;
+            }
+            logf_(j, "saving data to disk\n");
+            const int ss = dq_sim_save(&j->sim);
+            if (ss < 0) {
+                logf_(j, "sim_data_save() failed: %d\n", ss);
+                j->code = WRAP_FAIL;
+            } else {
+                logf_(j, "sim_data_save() succeeded\n");
+            }
+        }
+    }
+
+    const double wall = (double)(dq_time_ns() - wall_start) * 1e-9;
+    for (int i = 0; i < n; i++) {
+        job *j = &jobs[i];
+        codes[i] = j->code;
+        if (j->loaded) dq_sim_free(&j->sim);
+        if (!j->log) continue;
+        logf_(j, "wall time: %.3f\n", wall);
+        if (j->log != stdout) fclose(j->log);
+        else fflush(stdout);
+    }
+    free(jobs);
+}
+
+int dqmc_wrapper(const char *sim_file, const char *log_file, int64_t max_time_ns, bool dry,
+                 bool bench)
+{
+    int code = 0;
+    const char *logs[1] = {log_file};
+    dqmc_wrapper_batch(1, &sim_file, logs, max_time_ns, dry, bench, &code);
+    return code;
+}

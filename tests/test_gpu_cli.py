@@ -1,0 +1,96 @@
+"""The process seam on the GPU: dqmc_1 / dqmc_stack over .h5 files, checked
+against the oracle run of the same file (src/main_1.c, src/main_stack.c,
+src/dqmc.c:513-671 semantics: checkpoints, time limit, resume, batching)."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from dqmc_amd import gen, h5
+import oracle
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+BIN = os.path.join(ROOT, "dqmc-dev_amd", "bin")
+
+
+def rel(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+def _run(args, timeout=120):
+    print("run:", " ".join(args), flush=True)
+    return subprocess.run(args, capture_output=True, text=True, timeout=timeout)
+
+
+def _check_against_oracle(fn, sim, eq_keys=("density", "double_occ", "nn", "zz", "pair_sw"),
+                          ue_keys=()):
+    r = h5.read_sim(fn)
+    out = oracle.run(sim)
+    assert np.array_equal(r["state"]["hs"], out["state"]["hs"]), fn
+    assert [int(v) for v in r["state"]["rng"]] == [int(v) for v in out["state"]["rng"]], fn
+    assert int(r["state"]["sweep"]) == int(sim["params"]["n_sweep"])
+    assert int(r["state"]["partial_write"]) == 0
+    assert int(r["meas_eqlt"]["n_sample"]) == int(out["meas_eqlt"]["n_sample"])
+    for k in eq_keys:
+        assert rel(r["meas_eqlt"][k], out["meas_eqlt"][k]) < 1e-9, (fn, k)
+    for k in ue_keys:
+        assert rel(r["meas_uneqlt"][k], out["meas_uneqlt"][k]) < 1e-8, (fn, k)
+    return r
+
+
+def test_dqmc_1_golden_file_with_checkpoints(tmp_path, golden):
+    sim = gen.create_batch(seed=1234, n_sweep_warm=20, n_sweep_meas=20, checkpoint_every=7)[0]
+    fn = str(tmp_path / "golden.h5")
+    h5.write_sim(fn, sim)
+    r = _run([os.path.join(BIN, "dqmc_1"), "-l", fn + ".log", fn])
+    log = open(fn + ".log").read()
+    assert r.returncode == 0, log
+    assert "checkpointing every 7 sweeps" in log and "sim_data_save() succeeded" in log
+    out = _check_against_oracle(fn, sim)
+    assert np.array_equal(out["state"]["hs"], np.array(golden["hs_end"]["square"]))
+
+
+def test_dqmc_stack_batches_two_shapes(tmp_path):
+    a = gen.create_batch(Nfiles=3, seed=31, Nx=4, Ny=4, U=5.0, mu=-0.4, L=20, n_matmul=5,
+                         period_eqlt=5, period_uneqlt=2, n_sweep_warm=2, n_sweep_meas=6)
+    b = gen.create_batch(Nfiles=2, seed=32, Nx=4, Ny=6, U=4.0, mu=-0.2, L=24, n_matmul=6,
+                         period_eqlt=6, nflux=1, n_sweep_warm=2, n_sweep_meas=5)
+    fa = h5.create_batch_files(str(tmp_path / "a"), a)
+    fb = h5.create_batch_files(str(tmp_path / "b"), b)
+    stack = str(tmp_path / "stack")
+    with open(stack, "w") as f:
+        f.write("\n".join(fa + fb) + "\n")
+    r = _run([os.path.join(BIN, "dqmc_stack"), "-B", "8", stack])
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert r.stdout.count("completed:") == 5, r.stdout
+    assert open(stack).read().strip() == ""
+    for fn, s in zip(fa, a):
+        _check_against_oracle(fn, s, ue_keys=("gt0", "nn", "zz"))
+    for fn, s in zip(fb, b):
+        _check_against_oracle(fn, s)
+
+
+def test_time_limit_interrupts_and_resume_matches_oracle(tmp_path):
+    sim = gen.create_batch(seed=41, Nx=4, Ny=4, U=4.0, mu=-0.3, L=40, n_matmul=8, period_eqlt=8,
+                           n_sweep_warm=10, n_sweep_meas=40000, checkpoint_every=10)[0]
+    fn = str(tmp_path / "long.h5")
+    h5.write_sim(fn, sim)
+    # 40010 sweeps cannot finish in 2 s: the limit stops the run (src/sig.c:70-75)
+    r = _run([os.path.join(BIN, "dqmc_1"), "-t", "2", "-l", fn + ".log", fn])
+    log = open(fn + ".log").read()
+    assert r.returncode == 1, log  # WRAP_INCOMPLETE
+    assert "dqmc() interrupted" in log
+    ck = h5.read_sim(fn)
+    s0 = int(ck["state"]["sweep"])
+    assert s0 % 10 == 0 and int(ck["state"]["partial_write"]) == 0
+    # resume from the checkpoint for 25 more sweeps (n_sweep edited in the file)
+    n_end = s0 + 25
+    with h5.File(fn, "r+") as f:
+        f.overwrite("/params/n_sweep", np.int32(n_end))
+    resumed = h5.read_sim(fn)
+    r = _run([os.path.join(BIN, "dqmc_1"), "-l", fn + ".log", fn])
+    assert r.returncode == 0, open(fn + ".log").read()
+    _check_against_oracle(fn, resumed)
