@@ -11,6 +11,9 @@
 #pragma once
 #include "dq_common.hpp"
 
+// extra bin maps of the optional groups (CSR per chain, like map_bb / map_bs)
+enum { PM_bb2 = 0, PM_b2b, PM_b2b2, PM_plaq, PM_b, PM_b2, PM_COUNT };
+
 template <typename T>
 struct MeasArgs {
     int N, L, num_i, num_ij, num_bb, num_b;
@@ -30,6 +33,15 @@ struct MeasArgs {
     const int* rp_bb; const int* ix_bb; long srp_bb, six_bb;  // CSR of map_bb
     const int* rp_bs; const int* ix_bs; long srp_bs, six_bs;  // CSR of map_bs
     int num_bs;
+    // optional groups (dq_meas2.hpp)
+    int thermal, twobond, pbbonly, chiral, localjq;
+    int Nx, Ny, num_b2, num_bb2, num_b2b, num_b2b2, num_plaq, num_plaq_acc, num_b_acc, num_b2_acc;
+    const int* rp2[PM_COUNT]; const int* ix2[PM_COUNT]; long srp2[PM_COUNT], six2[PM_COUNT];
+    const int* degen2;                 // [c][PM_COUNT]
+    const int* bond2s; long sbond2s;   // [c][2 num_b2]
+    const T* pp; long spp;             // [c][4][num_b2]: pp_u, ppr_u, pp_d, ppr_d
+    const int* plaqs; long splaqs;     // [c][3 num_plaq]
+    T* gs; long sgs;                   // gen_suscept partial sums [c][L][4][num_ij]
 };
 
 // accumulator indices (match enum dqmc_eq_acc / dqmc_ue_acc)
@@ -290,80 +302,3 @@ __global__ __launch_bounds__(256) void k_meas_eq_energy(MeasArgs<T> a)
     }
 }
 
-// unequal time, meas_bond_corr: pair_bb, jj, jsjs, kk, ksks
-// (src/meas.c:860-955), one thread per (bin, t)
-template <typename T>
-__global__ __launch_bounds__(256) void k_meas_ue_bond(MeasArgs<T> a)
-{
-    const int c = blockIdx.z;
-    if (!a.flag[c]) return;
-    const int N = a.N, nb = a.num_b;
-    const long NN = (long)N * N;
-    const int tt = blockIdx.y;
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= a.num_bb) return;
-    const T phase = a.phase[c];
-    T* acc = a.acc + c * a.sacc;
-    const long* off = a.off;
-    const T* pu = a.peierlsu + c * a.sP;
-    const T* pd = a.peierlsd + c * a.sP;
-    const int* bonds = a.bonds + c * a.sbonds;
-    const T* Gu0t = a.G0t + (2 * c) * a.sU + NN * tt;
-    const T* Gutt = a.Gtt + (2 * c) * a.sU + NN * tt;
-    const T* Gut0 = a.Gt0 + (2 * c) * a.sU + NN * tt;
-    const T* Gd0t = a.G0t + (2 * c + 1) * a.sU + NN * tt;
-    const T* Gdtt = a.Gtt + (2 * c + 1) * a.sU + NN * tt;
-    const T* Gdt0 = a.Gt0 + (2 * c + 1) * a.sU + NN * tt;
-    const T* Gu00 = a.Gtt + (2 * c) * a.sU;
-    const T* Gd00 = a.Gtt + (2 * c + 1) * a.sU;
-    const int* rp = a.rp_bb + c * a.srp_bb;
-    const int* ix = a.ix_bb + c * a.six_bb;
-    const T pre = phase * (1.0 / a.degen[4 * c + 2]);
-    const long o = r + (long)a.num_bb * tt;
-    T pbb = acc[off[UE_pair_bb] + o], jj = acc[off[UE_jj] + o], jsjs = acc[off[UE_jsjs] + o];
-    T kk = acc[off[UE_kk] + o], ksks = acc[off[UE_ksks] + o];
-    const double dt = (tt == 0) ? 1.0 : 0.0;
-    for (int q = rp[r]; q < rp[r + 1]; q++) {
-        const int lin = ix[q];
-        const int b = lin % nb, cc = lin / nb;
-        const int j0 = bonds[cc], j1 = bonds[cc + nb];
-        const int i0 = bonds[b], i1 = bonds[b + nb];
-        const T puj0j1 = pf(pu, j0 + (long)N * j1), puj1j0 = pf(pu, j1 + (long)N * j0);
-        const T pdj0j1 = pf(pd, j0 + (long)N * j1), pdj1j0 = pf(pd, j1 + (long)N * j0);
-        const T pui0i1 = pf(pu, i0 + (long)N * i1), pui1i0 = pf(pu, i1 + (long)N * i0);
-        const T pdi0i1 = pf(pd, i0 + (long)N * i1), pdi1i0 = pf(pd, i1 + (long)N * i0);
-        const T gui1i0 = Gutt[i1 + i0 * (long)N], gui0i1 = Gutt[i0 + i1 * (long)N];
-        const T gui0j0 = Gut0[i0 + j0 * (long)N], gui1j0 = Gut0[i1 + j0 * (long)N];
-        const T gui0j1 = Gut0[i0 + j1 * (long)N], gui1j1 = Gut0[i1 + j1 * (long)N];
-        const T guj0i0 = Gu0t[j0 + i0 * (long)N], guj1i0 = Gu0t[j1 + i0 * (long)N];
-        const T guj0i1 = Gu0t[j0 + i1 * (long)N], guj1i1 = Gu0t[j1 + i1 * (long)N];
-        const T guj1j0 = Gu00[j1 + j0 * (long)N], guj0j1 = Gu00[j0 + j1 * (long)N];
-        const T gdi1i0 = Gdtt[i1 + i0 * (long)N], gdi0i1 = Gdtt[i0 + i1 * (long)N];
-        const T gdi0j0 = Gdt0[i0 + j0 * (long)N], gdi1j0 = Gdt0[i1 + j0 * (long)N];
-        const T gdi0j1 = Gdt0[i0 + j1 * (long)N], gdi1j1 = Gdt0[i1 + j1 * (long)N];
-        const T gdj0i0 = Gd0t[j0 + i0 * (long)N], gdj1i0 = Gd0t[j1 + i0 * (long)N];
-        const T gdj0i1 = Gd0t[j0 + i1 * (long)N], gdj1i1 = Gd0t[j1 + i1 * (long)N];
-        const T gdj1j0 = Gd00[j1 + j0 * (long)N], gdj0j1 = Gd00[j0 + j1 * (long)N];
-        const double d00 = dt * (i0 == j0), d10 = dt * (i1 == j0);
-        const double d01 = dt * (i0 == j1), d11 = dt * (i1 == j1);
-        const T arr0 = (d01 - gdj1i0) * gdi1j0, arr1 = (d00 - gdj0i0) * gdi1j1;
-        const T arr2 = (d11 - gdj1i1) * gdi0j0, arr3 = (d10 - gdj0i1) * gdi0j1;
-        const T arr4 = (d01 - guj1i0) * gui1j0, arr5 = (d00 - guj0i0) * gui1j1;
-        const T arr6 = (d11 - guj1i1) * gui0j0, arr7 = (d10 - guj0i1) * gui0j1;
-        const T p0 = pdi0i1 * pdj0j1 * arr0, p1 = pdi0i1 * pdj1j0 * arr1;
-        const T p2 = pdi1i0 * pdj0j1 * arr2, p3 = pdi1i0 * pdj1j0 * arr3;
-        const T p4 = pui0i1 * puj0j1 * arr4, p5 = pui0i1 * puj1j0 * arr5;
-        const T p6 = pui1i0 * puj0j1 * arr6, p7 = pui1i0 * puj1j0 * arr7;
-        pbb += 0.5 * (pre * (gui0j0 * gdi1j1 + gui1j0 * gdi0j1 + gui0j1 * gdi1j0 + gui1j1 * gdi0j0));
-        const T x = p0 + p3 + p4 + p7;
-        const T y = p1 + p2 + p5 + p6;
-        const T t0 = pui1i0 * gui0i1, t1 = pui0i1 * gui1i0, t2 = pdi1i0 * gdi0i1, t3 = pdi0i1 * gdi1i0;
-        const T t4 = puj1j0 * guj0j1, t5 = puj0j1 * guj1j0, t6 = pdj1j0 * gdj0j1, t7 = pdj0j1 * gdj1j0;
-        jj += pre * ((t0 - t1 + t2 - t3) * (t4 - t5 + t6 - t7) + x - y);
-        jsjs += pre * ((t0 - t1 - t2 + t3) * (t4 - t5 - t6 + t7) + x - y);
-        kk += pre * ((t0 + t1 + t2 + t3) * (t4 + t5 + t6 + t7) + x + y);
-        ksks += pre * ((t0 + t1 - t2 - t3) * (t4 + t5 - t6 - t7) + x + y);
-    }
-    acc[off[UE_pair_bb] + o] = pbb; acc[off[UE_jj] + o] = jj; acc[off[UE_jsjs] + o] = jsjs;
-    acc[off[UE_kk] + o] = kk; acc[off[UE_ksks] + o] = ksks;
-}

@@ -318,11 +318,39 @@ static int ue_measure(dqmc_engine* e)
     ma.Gtt = dp<T>(e->Gtt);
     ma.Gt0 = dp<T>(e->Gt0);
     ma.sU = (long)e->L * e->N * e->N;
-    k_meas_ue<T><<<dim3((e->num_ij + 255) / 256, e->L, e->nc), 256, 0, e->st>>>(ma);
+    const int L = e->L, nc = e->nc;
+    k_meas_ue<T><<<dim3((e->num_ij + 255) / 256, L, nc), 256, 0, e->st>>>(ma);
     LCHK();
-    if (e->bond) {
-        k_meas_ue_bond<T><<<dim3((e->num_bb + 255) / 256, e->L, e->nc), 256, 0, e->st>>>(ma);
+    if (e->bond || e->thermal || e->pbbonly) {
+        k_meas_ue_pairs<T, PK_BB><<<dim3((e->num_bb + 255) / 256, L, nc), 256, 0, e->st>>>(ma);
         LCHK();
+    }
+    if (e->thermal || e->twobond) {
+        k_meas_ue_pairs<T, PK_B2B><<<dim3((e->num_bb2 + 255) / 256, L, nc), 256, 0, e->st>>>(ma);
+        LCHK();
+        k_meas_ue_pairs<T, PK_BB2><<<dim3((e->num_b2b + 255) / 256, L, nc), 256, 0, e->st>>>(ma);
+        LCHK();
+    }
+    if (e->twobond) {
+        k_meas_ue_pairs<T, PK_B2B2><<<dim3((e->num_b2b2 + 255) / 256, L, nc), 256, 0, e->st>>>(ma);
+        LCHK();
+    }
+    if (e->energy) {
+        k_meas_ue_kv<T><<<dim3((e->num_bs + 255) / 256, L, nc), 256, 0, e->st>>>(ma);
+        LCHK();
+    }
+    if (e->gensus) {
+        const long nij = e->num_ij;
+        k_gensus_ab<T><<<dim3((nij + 255) / 256, L, nc), 256, 0, e->st>>>(ma, 1);
+        LCHK();
+        k_gensus_outer<T><<<dim3((nij * nij + 255) / 256, L, nc), 256, 0, e->st>>>(ma, 1);
+        LCHK();
+    }
+    if constexpr (!is_cplx<T>::value) {
+        if (e->nematic) {
+            k_meas_ue_nematic<<<dim3((e->num_bb + 255) / 256, L, nc), 256, 0, e->st>>>(ma);
+            LCHK();
+        }
     }
     return 0;
 }

@@ -23,6 +23,7 @@
 #include "dq_factor.hpp"
 #include "dq_gemm.hpp"
 #include "dq_meas.hpp"
+#include "dq_meas2.hpp"
 #include "dq_qr_blocked.hpp"
 #include "dq_rng.hpp"
 #include "dq_ue.hpp"
@@ -119,15 +120,17 @@ extern "C" long dqmc_acc_len(const dqmc_params* p, int ue, int k)
     return 0;
 }
 
-// accumulators computed on the GPU by this version
-static bool eq_supported(int k)
+// accumulators computed on the GPU: all of them, except the nematic pair in
+// complex builds, which the reference refuses too (src/meas.c:1369-1375)
+static bool eq_supported(int k, int cplx)
 {
-    return k <= DQ_EQ_pair_sw || (k >= DQ_EQ_kk && k <= DQ_EQ_vn);
+    (void)cplx;
+    return k >= 0 && k < DQ_EQ_COUNT;
 }
-static bool ue_supported(int k)
+static bool ue_supported(int k, int cplx)
 {
-    return k <= DQ_UE_pair_sw || k == DQ_UE_vv || k == DQ_UE_vn ||
-           (k >= DQ_UE_pair_bb && k <= DQ_UE_ksks);
+    if (cplx && (k == DQ_UE_nem_nnnn || k == DQ_UE_nem_ssss)) return false;
+    return k >= 0 && k < DQ_UE_COUNT;
 }
 
 // --------------------------------------------------------------------------
@@ -174,6 +177,14 @@ struct dqmc_engine {
     void *Gred = 0, *Wa = 0, *Wb = 0, *Qb = 0, *taub = 0, *T2 = 0;
     int gemm_adiv = 1, gemm_bdiv = 1;  // one-shot batch divisors for the next gemm()
     int gemm_tile = GT_64;             // GEMM C-tile shape (GT_*)
+    // optional measurement groups (dq_meas2.hpp)
+    int thermal = 0, twobond = 0, nematic = 0, gensus = 0, chiral = 0, localjq = 0, pbbonly = 0;
+    int Nx = 0, Ny = 0, num_b2 = 0, num_bb2 = 0, num_b2b = 0, num_b2b2 = 0, num_plaq = 0;
+    int num_plaq_acc = 0, num_b_acc = 0, num_b2_acc = 0;
+    int *rp2[PM_COUNT] = {}, *ix2[PM_COUNT] = {};
+    long srp2[PM_COUNT] = {}, six2[PM_COUNT] = {};
+    int *degen2 = 0, *bond2s = 0, *plaqs = 0;
+    void *pp = 0, *gs = 0;
     // blocked QR workspace: explicit V, compact-WY T factors, GEMM scratch, dets
     void *Vb = 0, *Tb = 0, *Wq1 = 0, *Wq2 = 0, *detq = 0, *detm = 0;
     long sVb = 0, sTb = 0, sWq = 0;
@@ -573,6 +584,19 @@ static MeasArgs<T> meas_args(dqmc_engine* e, int ue)
     a.rp_bs = e->rp_bs; a.ix_bs = e->ix_bs; a.srp_bs = e->num_bs + 1; a.six_bs = (long)e->num_b * e->N;
     a.peierlsu = dp<T>(e->peierlsu); a.peierlsd = dp<T>(e->peierlsd);
     a.sP = is_cplx<T>::value ? (long)e->N * e->N : 0;
+    a.thermal = e->thermal; a.twobond = e->twobond; a.pbbonly = e->pbbonly;
+    a.chiral = e->chiral; a.localjq = e->localjq;
+    a.Nx = e->Nx; a.Ny = e->Ny; a.num_b2 = e->num_b2; a.num_bb2 = e->num_bb2;
+    a.num_b2b = e->num_b2b; a.num_b2b2 = e->num_b2b2; a.num_plaq = e->num_plaq;
+    a.num_plaq_acc = e->num_plaq_acc; a.num_b_acc = e->num_b_acc; a.num_b2_acc = e->num_b2_acc;
+    for (int m = 0; m < PM_COUNT; m++) {
+        a.rp2[m] = e->rp2[m]; a.ix2[m] = e->ix2[m]; a.srp2[m] = e->srp2[m]; a.six2[m] = e->six2[m];
+    }
+    a.degen2 = e->degen2;
+    a.bond2s = e->bond2s; a.sbond2s = 2L * e->num_b2;
+    a.pp = dp<T>(e->pp); a.spp = 4L * e->num_b2;
+    a.plaqs = e->plaqs; a.splaqs = 3L * e->num_plaq;
+    a.gs = dp<T>(e->gs); a.sgs = (long)e->L * 4 * e->num_ij;
     if (ue) {
         a.flag = e->flag_ue; a.acc = dp<T>(e->acc_ue); a.sacc = e->ue_total; a.off = e->d_ue_off;
         a.n_sample = e->ns_ue; a.sign = dp<T>(e->sign_ue);
@@ -581,6 +605,28 @@ static MeasArgs<T> meas_args(dqmc_engine* e, int ue)
         a.n_sample = e->ns_eq; a.sign = dp<T>(e->sign_eq);
     }
     return a;
+}
+
+// equal-time optional groups on the half-wrapped G of ma: gen_suscept
+// (factorised, dq_meas2.hpp), chiral and local JQ
+template <typename T>
+static int meas_eq_extra(dqmc_engine* e, const MeasArgs<T>& ma)
+{
+    const int nc = e->nc;
+    if (e->gensus) {
+        const long nij = e->num_ij;
+        k_gensus_ab<T><<<dim3((nij + 255) / 256, 1, nc), 256, 0, e->st>>>(ma, 0);
+        LCHK();
+        k_gensus_outer<T><<<dim3((nij * nij + 255) / 256, 1, nc), 256, 0, e->st>>>(ma, 0);
+        LCHK();
+    }
+    if (e->chiral || e->localjq) {
+        const int nb = std::max(e->chiral ? e->num_plaq_acc : 0,
+                                e->localjq ? std::max(e->num_b_acc, e->num_b2_acc) : 0);
+        k_meas_eq_extra<T><<<dim3((nb + 255) / 256, nc), 256, 0, e->st>>>(ma);
+        LCHK();
+    }
+    return 0;
 }
 
 // unequal-time Green's functions and measurements
@@ -689,6 +735,7 @@ static int run_t(dqmc_engine* e, int n_sweeps, const volatile int* stop)
                     k_meas_eq_energy<T><<<dim3(nb2, nc), 256, 0, e->st>>>(ma);
                     LCHK();
                 }
+                RC(meas_eq_extra<T>(e, ma));
             }
         }
         if (any_ue) RC(ue_measure<T>(e));
@@ -737,13 +784,14 @@ static int engine_setup(dqmc_engine* e, const dqmc_params* const* P)
     const size_t es = e->esz;
     // check the measurement set against what the GPU computes
     for (int k = 0; k < DQ_EQ_COUNT; k++)
-        if (dqmc_acc_len(p0, 0, k) > 0 && !eq_supported(k))
+        if (dqmc_acc_len(p0, 0, k) > 0 && !eq_supported(k, e->cplx))
             return set_err(DQMC_ERR_UNSUPPORTED,
                            "equal-time accumulator %d is enabled but not implemented on the GPU", k);
     for (int k = 0; k < DQ_UE_COUNT; k++)
-        if (dqmc_acc_len(p0, 1, k) > 0 && !ue_supported(k))
+        if (dqmc_acc_len(p0, 1, k) > 0 && !ue_supported(k, e->cplx))
             return set_err(DQMC_ERR_UNSUPPORTED,
-                           "unequal-time accumulator %d is enabled but not implemented on the GPU",
+                           "unequal-time accumulator %d is not available in complex builds "
+                           "(peierls phase not implemented for nematic measurements)",
                            k);
     DALLOC(e->expK, B * NN * es);
     DALLOC(e->iexpK, B * NN * es);
@@ -823,6 +871,31 @@ static int engine_setup(dqmc_engine* e, const dqmc_params* const* P)
     DALLOC(e->ix_bs, (long)nc * ((long)e->num_b * N > 0 ? (long)e->num_b * N : 1) * 4);
     DALLOC(e->peierlsu, (e->cplx ? nc * NN : 1) * es);
     DALLOC(e->peierlsd, (e->cplx ? nc * NN : 1) * es);
+    // optional groups: which extra maps are needed, their input counts and bins
+    const bool ueon = e->period_uneqlt > 0;
+    const long nb1 = e->num_b, nb2 = e->num_b2;
+    const bool need[PM_COUNT] = {ueon && (e->thermal || e->twobond), ueon && (e->thermal || e->twobond),
+                                 ueon && e->twobond, (bool)e->chiral, (bool)e->localjq, (bool)e->localjq};
+    const long nin[PM_COUNT] = {nb2 * nb1, nb1 * nb2, nb2 * nb2, e->num_plaq, nb1, nb2};
+    const int nbin[PM_COUNT] = {e->num_bb2, e->num_b2b, e->num_b2b2, e->num_plaq_acc, e->num_b_acc,
+                                e->num_b2_acc};
+    const int32_t* const* pmaps[PM_COUNT] = {&p0->map_bb2, &p0->map_b2b, &p0->map_b2b2, &p0->map_plaq,
+                                             &p0->map_b, &p0->map_b2};
+    (void)pmaps;
+    for (int m = 0; m < PM_COUNT; m++) {
+        if (!need[m]) continue;
+        if (nbin[m] <= 0 || nin[m] <= 0)
+            return set_err(DQMC_ERR_ARG, "optional measurement map %d is empty", m);
+        e->srp2[m] = nbin[m] + 1L;
+        e->six2[m] = nin[m];
+        DALLOC(e->rp2[m], (long)nc * e->srp2[m] * 4);
+        DALLOC(e->ix2[m], (long)nc * e->six2[m] * 4);
+    }
+    DALLOC(e->degen2, (long)nc * PM_COUNT * 4);
+    DALLOC(e->bond2s, (long)nc * 2 * (nb2 > 0 ? nb2 : 1) * 4);
+    DALLOC(e->pp, (long)nc * 4 * (nb2 > 0 ? nb2 : 1) * es);
+    DALLOC(e->plaqs, (long)nc * 3 * (e->num_plaq > 0 ? e->num_plaq : 1) * 4);
+    if (e->gensus) DALLOC(e->gs, (long)nc * L * 4 * e->num_ij * es);
     if (e->period_uneqlt > 0) {
         const long LNN = (long)L * NN;
         DALLOC(e->G0t, B * LNN * es);
@@ -864,6 +937,31 @@ static int engine_setup(dqmc_engine* e, const dqmc_params* const* P)
             csr_of_map(p->map_bs, (int)(nb * N), e->num_bs, rp, ix);
             HIPCHK(hipMemcpy(e->rp_bs + c * (e->num_bs + 1L), rp.data(), rp.size() * 4, hipMemcpyHostToDevice));
             HIPCHK(hipMemcpy(e->ix_bs + c * nb * N, ix.data(), ix.size() * 4, hipMemcpyHostToDevice));
+        }
+        {
+            const int32_t* maps[PM_COUNT] = {p->map_bb2, p->map_b2b, p->map_b2b2, p->map_plaq,
+                                             p->map_b, p->map_b2};
+            for (int m = 0; m < PM_COUNT; m++) {
+                if (!need[m]) continue;
+                if (!maps[m]) return set_err(DQMC_ERR_ARG, "chain %d: optional map %d missing", c, m);
+                csr_of_map(maps[m], (int)nin[m], nbin[m], rp, ix);
+                HIPCHK(hipMemcpy(e->rp2[m] + c * e->srp2[m], rp.data(), rp.size() * 4, hipMemcpyHostToDevice));
+                HIPCHK(hipMemcpy(e->ix2[m] + c * e->six2[m], ix.data(), ix.size() * 4, hipMemcpyHostToDevice));
+            }
+            const int d2[PM_COUNT] = {p->degen_bb2, p->degen_b2b, p->degen_b2b2, p->degen_plaq,
+                                      p->degen_b, p->degen_b2};
+            HIPCHK(hipMemcpy(e->degen2 + c * PM_COUNT, d2, sizeof d2, hipMemcpyHostToDevice));
+            if (nb2 > 0 && p->bond2s)
+                HIPCHK(hipMemcpy(e->bond2s + c * 2 * nb2, p->bond2s, 2 * nb2 * 4, hipMemcpyHostToDevice));
+            if (nb2 > 0 && p->pp_u && p->ppr_u && p->pp_d && p->ppr_d) {
+                const double* src[4] = {p->pp_u, p->ppr_u, p->pp_d, p->ppr_d};
+                for (int q = 0; q < 4; q++)
+                    HIPCHK(hipMemcpy((char*)e->pp + ((long)c * 4 + q) * nb2 * es, src[q], nb2 * es,
+                                     hipMemcpyHostToDevice));
+            }
+            if (e->num_plaq > 0 && p->plaqs)
+                HIPCHK(hipMemcpy(e->plaqs + c * 3L * e->num_plaq, p->plaqs, 3L * e->num_plaq * 4,
+                                 hipMemcpyHostToDevice));
         }
         deg[4 * c] = p->degen_i;
         deg[4 * c + 1] = p->degen_ij;
@@ -957,6 +1055,14 @@ extern "C" int dqmc_engine_create(dqmc_engine** out, const dqmc_params* const* P
     e->num_i = p0->num_i; e->num_ij = p0->num_ij;
     e->num_b = p0->num_b; e->num_bb = p0->num_bb; e->num_bs = p0->num_bs;
     e->bond = p0->meas_bond_corr;
+    e->thermal = p0->meas_thermal; e->twobond = p0->meas_2bond_corr;
+    e->nematic = p0->meas_nematic_corr; e->gensus = p0->meas_gen_suscept;
+    e->chiral = p0->meas_chiral; e->localjq = p0->meas_local_JQ;
+    e->pbbonly = p0->meas_pair_bb_only;
+    e->Nx = p0->Nx; e->Ny = p0->Ny; e->num_b2 = p0->num_b2; e->num_bb2 = p0->num_bb2;
+    e->num_b2b = p0->num_b2b; e->num_b2b2 = p0->num_b2b2; e->num_plaq = p0->num_plaq;
+    e->num_plaq_acc = p0->num_plaq_accum; e->num_b_acc = p0->num_b_accum;
+    e->num_b2_acc = p0->num_b2_accum;
     e->esz = e->cplx ? 16 : 8;
     e->kd = (tune && tune->delay_rank > 0) ? tune->delay_rank : (p0->n_delay > 0 ? p0->n_delay : 16);
     if (e->kd > e->N) e->kd = e->N;

@@ -17,6 +17,8 @@
  *   update_delayed ...................... src/updates.c:7-84
  *   measure_eqlt ........................ src/meas.c:66-407
  *   measure_uneqlt (default, energy vv/vn, bond_corr) src/meas.c:728-858, 860-955
+ *   optional eq/uneqlt groups ........... meas_ext.inc (thermal, 2-bond, energy
+ *                                         kv/kn, gen_suscept, nematic, chiral, local JQ)
  *   dqmc() sweep loop ................... src/dqmc.c:84-459
  * Compiled twice: without and with -DORACLE_CPLX (the reference's -DUSE_CPLX).
  * Parity pins: see tests/test_oracle_*.py.
@@ -613,6 +615,8 @@ static inline num load_num(const double *p, long i)
 #endif
 }
 
+#include "meas_ext.inc"
+
 
 
 /* measure_eqlt (src/meas.c:66-407) */
@@ -671,6 +675,7 @@ static void measure_eqlt(const dqmc_params *p, num phase, const num *gu, const n
             }
         }
     }
+    measure_eqlt_ext(p, phase, gu, gd, m);
     if (!p->meas_energy_corr) return;
 
     /* 1 bond - 1 site (src/meas.c:290-330) */
@@ -809,6 +814,7 @@ static void measure_uneqlt(const dqmc_params *p, num phase, const num *Gu0t, con
             }
         }
     }
+    measure_uneqlt_ext(p, phase, Gu0t, Gutt, Gut0, Gd0t, Gdtt, Gdt0, m);
     if (!p->meas_bond_corr) return;
 
     for (int t = 0; t < L; t++) {

@@ -199,3 +199,39 @@ def test_full_size_green_consistency():
             assert abs(ph0 - phf) < 1e-10
     finally:
         eng.close()
+
+
+@pytest.mark.parametrize("nflux", [0, 1])
+@pytest.mark.parametrize("trans_sym", [1, 0])
+def test_optional_measurement_groups(nflux, trans_sym):
+    """Thermal, 2-bond, energy, bond, gen_suscept / local JQ, chiral and (real)
+    nematic accumulators after a short run, against the oracle's term-by-term
+    restatement of src/meas.c."""
+    tog = dict(meas_thermal=1, meas_2bond_corr=1, meas_energy_corr=1, meas_bond_corr=1,
+               meas_chiral=1)
+    if nflux == 0:
+        tog["meas_nematic_corr"] = 1
+    if trans_sym:
+        tog["meas_gen_suscept"] = 1
+    else:
+        tog["meas_local_JQ"] = 1
+    sim = gen.create_batch(seed=5, Nx=4, Ny=4, nflux=nflux, U=4.0, mu=-0.3, tp=-0.2, L=16,
+                           n_matmul=4, period_eqlt=4, period_uneqlt=1, n_sweep_warm=1,
+                           n_sweep_meas=2, trans_sym=trans_sym, **tog)[0]
+    eng = engine.Engine([sim])
+    try:
+        eng.run(3)
+        out = oracle.run(sim)
+        assert np.array_equal(eng.get_state(0)["hs"], out["state"]["hs"])
+        eq, ue = eng.get_meas(0)
+        checked = 0
+        for grp, mine, ref, tol in (("eq", eq, out["meas_eqlt"], 1e-9), ("ue", ue, out["meas_uneqlt"], 1e-8)):
+            for k, v in ref.items():
+                if k in ("n_sample", "sign"):
+                    continue
+                assert k in mine, (grp, k)
+                assert rel(mine[k], v) < tol, (grp, k, rel(mine[k], v))
+                checked += 1
+        assert checked >= 40
+    finally:
+        eng.close()
