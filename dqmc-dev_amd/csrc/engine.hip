@@ -24,6 +24,7 @@
 #include "dq_gemm.hpp"
 #include "dq_meas.hpp"
 #include "dq_meas2.hpp"
+#include "dq_adjoint.hpp"
 #include "dq_qr_blocked.hpp"
 #include "dq_rng.hpp"
 #include "dq_ue.hpp"
@@ -185,6 +186,9 @@ struct dqmc_engine {
     long srp2[PM_COUNT] = {}, six2[PM_COUNT] = {};
     int *degen2 = 0, *bond2s = 0, *plaqs = 0;
     void *pp = 0, *gs = 0;
+    // adjoint outputs (allocated on first use)
+    void *adjW = 0, *adjT = 0;
+    double* adjL = 0;
     // blocked QR workspace: explicit V, compact-WY T factors, GEMM scratch, dets
     void *Vb = 0, *Tb = 0, *Wq1 = 0, *Wq2 = 0, *detq = 0, *detm = 0;
     long sVb = 0, sTb = 0, sWq = 0;
@@ -564,6 +568,59 @@ static int init_green(dqmc_engine* e)
     LCHK();
     HIPCHK(hipStreamSynchronize(e->st));
     e->stale = false;
+    return 0;
+}
+
+// Adjoint pass for the current (fixed) fields of every chain: log|det M| per
+// (chain, spin) and the sums W, tr G, lambda term of dq_adjoint.hpp.  The
+// slices are walked like a sweep without updates: wraps, and a fresh
+// stabilized G from the stored group products every n_matmul slices.
+template <typename T>
+static int adjoint_t(dqmc_engine* e, double* logdet, double* Wout, double* trG, double* sGl)
+{
+    const int N = e->N, L = e->L, nc = e->nc, B = 2 * nc, n = e->n_matmul, F = e->F;
+    const long NN = (long)N * N, LN = (long)L * N;
+    const T one = from_re<T>(1.0), zero = from_re<T>(0.0);
+    if (!e->adjW) {
+        DALLOC(e->adjW, B * NN * e->esz);
+        DALLOC(e->adjT, 2L * B * e->esz);
+        DALLOC(e->adjL, B * 8);
+    }
+    T* W = dp<T>(e->adjW);
+    T* tr = dp<T>(e->adjT);
+    T* sg = tr + B;
+    HIPCHK(hipMemsetAsync(W, 0, B * NN * sizeof(T), e->st));
+    HIPCHK(hipMemsetAsync(tr, 0, 2L * B * sizeof(T), e->st));
+    T* G = dp<T>(e->G);
+    T* T1 = dp<T>(e->T1);
+    k_scales<<<nblk((long)nc * L * N), 256, 0, e->st>>>(N, L, nc, e->hs, e->exp_lambda, e->escale,
+                                                        e->iscale);
+    LCHK();
+    RC(build_C<T>(e, 0, F));
+    RC(calc_eq_g<T>(e, 0, G, dp<T>(e->phb)));
+    k_eqg_logdet<T><<<B, 64, 0, e->st>>>(N, e->vvec, N, dp<T>(e->Mm), NN, e->adjL);
+    LCHK();
+    for (int l = 0; l < L; l++) {
+        k_adj_accum_W<T><<<nblk(B * NN), 256, 0, e->st>>>(N, G, NN, e->escale, e->iscale, LN, l, W,
+                                                           NN, B);
+        LCHK();
+        k_adj_accum_diag<T><<<B, 64, 0, e->st>>>(N, L, G, NN, e->hs, l, tr, sg);
+        LCHK();
+        if ((l + 1) % n == 0) {
+            RC(calc_eq_g<T>(e, ((l / n) + 1) % F, G, dp<T>(e->phb)));
+        } else {
+            RC(gemm<T>(e, OP_N, OP_N, N, N, N, one, G, N, NN, dp<T>(e->iexpK), N, NN, zero, T1, N, NN,
+                       B, nullptr, 0, e->iscale + (long)N * l, LN));
+            RC(gemm<T>(e, OP_N, OP_N, N, N, N, one, dp<T>(e->expK), N, NN, T1, N, NN, zero, G, N, NN,
+                       B, nullptr, 0, e->escale + (long)N * l, LN));
+        }
+    }
+    HIPCHK(hipStreamSynchronize(e->st));
+    if (logdet) HIPCHK(hipMemcpy(logdet, e->adjL, B * 8, hipMemcpyDeviceToHost));
+    if (Wout) HIPCHK(hipMemcpy(Wout, W, B * NN * sizeof(T), hipMemcpyDeviceToHost));
+    if (trG) HIPCHK(hipMemcpy(trG, tr, B * sizeof(T), hipMemcpyDeviceToHost));
+    if (sGl) HIPCHK(hipMemcpy(sGl, sg, B * sizeof(T), hipMemcpyDeviceToHost));
+    e->stale = true;  // G was walked through all slices; the next run starts fresh
     return 0;
 }
 
@@ -1021,6 +1078,14 @@ static int autotune_gemm(dqmc_engine* e)
     hipEventDestroy(a1);
     e->gemm_tile = pick;
     return rc;
+}
+
+extern "C" int dqmc_engine_adjoint(dqmc_engine* e, double* logdet, double* W, double* trG,
+                                   double* sG)
+{
+    if (!e) return set_err(DQMC_ERR_ARG, "null engine");
+    HIPCHK(hipSetDevice(e->device));
+    return e->cplx ? adjoint_t<zd>(e, logdet, W, trG, sG) : adjoint_t<double>(e, logdet, W, trG, sG);
 }
 
 extern "C" int dqmc_engine_gemm_tile(dqmc_engine* e)

@@ -1,0 +1,154 @@
+"""Differentiable log|det M| of the Hubbard DQMC weight for fixed HS fields.
+
+No counterpart in the reference (it computes only the phase of det M,
+src/greens.c:180-191).  For a batch of fixed fields s_{l,i} = +-1 the weight
+of field f is det M_up det M_dn with
+    M_s = I + B_{L-1} ... B_0,   B_l = exp(-dt K_s) diag(exp(sigma_s lambda s_l)),
+    K_s = K - mu I,  sigma_up = +1, sigma_dn = -1,
+K the (real) hopping matrix -- the generator's kij -- and lambda the HS coupling
+(cosh-form e^lambda = e^{U dt/2} + sqrt(e^{U dt} - 1), gen_1band_unified_hub.py
+:925-927).  The GPU engine returns per field and spin log|det M_s| and the
+adjoint sums W_s, sum_l tr G_l and the lambda term (dqmc_engine_adjoint); the
+gradient with respect to K follows from d log det M = tr(W E^-1 dE) with
+E = expm(A), A = -dt K_s, and the adjoint of the exponential's Frechet
+derivative:   d log det / dK = -dt L(A, W E^-1)^T.
+Gradients with respect to t, t' (d K / dt = the nearest-neighbour hopping
+pattern), mu (= -tr of the K gradient, also dt (2 N L - sum tr G)) and U
+(through lambda(U)) are projections of these.
+
+`LogDet` is the torch.autograd.Function; `AdjointProblem` holds the fields
+and the lattice.  The engine's matrices are column-major views of the raw
+buffers (a numpy array A is seen as A^T), so exponentials are passed
+transposed to make the engine use exp(-dt K) itself.
+"""
+import ctypes as C
+
+import numpy as np
+from scipy.linalg import expm, expm_frechet
+
+from . import engine as _eng
+from . import gen
+
+
+def lambda_of_U(U, dt):
+    """HS coupling of the reference's cosh form and d lambda / d U."""
+    a = np.exp(0.5 * U * dt) + np.sqrt(np.expm1(U * dt))
+    lam = np.log(a)
+    da = 0.5 * dt * np.exp(0.5 * U * dt) + 0.5 * dt * np.exp(U * dt) / np.sqrt(np.expm1(U * dt))
+    return lam, da / a
+
+
+class AdjointProblem:
+    """A batch of fixed HS fields on one lattice (square, periodic).
+
+    fields: int32 array [nf][L][N] of 0/1 (the file's /state/hs layout)."""
+
+    def __init__(self, Nx, Ny, L, dt, fields, n_matmul=10, device=0):
+        self.Nx, self.Ny, self.N, self.L, self.dt = Nx, Ny, Nx * Ny, L, dt
+        self.n_matmul = n_matmul
+        self.fields = np.asarray(fields, dtype=np.int32).reshape(-1, L, Nx * Ny)
+        self.device = device
+        # template simulation (maps etc.); its physical content is replaced below
+        self._tmpl = gen.create_1(init_rng=gen.splitmix64_seed(1), Nx=Nx, Ny=Ny, L=L, dt=dt,
+                                  n_matmul=n_matmul, period_eqlt=n_matmul, n_sweep_warm=0,
+                                  n_sweep_meas=1)
+        self.t_pattern = gen.hopping_square(Nx, Ny, t=1.0, tp=0.0)[0].real
+        self.tp_pattern = gen.hopping_square(Nx, Ny, t=0.0, tp=1.0)[0].real
+
+    def hopping(self, t=1.0, tp=0.0):
+        return t * self.t_pattern + tp * self.tp_pattern
+
+    def _sims(self, K, mu, lam):
+        N, dt = self.N, self.dt
+        Ks = np.asarray(K, dtype=np.float64) - mu * np.eye(N)
+        E = expm(-dt * Ks)
+        base = dict(self._tmpl["params"])
+        base["exp_Ku"] = base["exp_Kd"] = np.ascontiguousarray(E.T)
+        base["inv_exp_Ku"] = base["inv_exp_Kd"] = np.ascontiguousarray(expm(dt * Ks).T)
+        base["exp_halfKu"] = base["exp_halfKd"] = np.ascontiguousarray(expm(-0.5 * dt * Ks).T)
+        base["inv_exp_halfKu"] = base["inv_exp_halfKd"] = np.ascontiguousarray(expm(0.5 * dt * Ks).T)
+        el = np.exp(lam) * np.ones(N)
+        base["exp_lambda"] = np.array((1.0 / el, el))
+        base["del"] = np.array((el ** 2 - 1.0, el ** -2 - 1.0))
+        sims = []
+        for f in self.fields:
+            st = dict(self._tmpl["state"])
+            st["hs"] = f.copy()
+            sims.append({"params": base, "state": st, "meas_eqlt": self._tmpl["meas_eqlt"],
+                         "meas_uneqlt": {}, "metadata": self._tmpl["metadata"]})
+        return sims, E
+
+    def run(self, K, mu, lam):
+        """Forward + adjoint sums on the GPU.  Returns (logdet [nf] summed over
+        spins, dict of per-field pieces)."""
+        sims, E = self._sims(K, mu, lam)
+        eng = _eng.Engine(sims, device=self.device)
+        try:
+            nf, N = len(sims), self.N
+            L = eng.L
+            logdet = np.zeros(2 * nf)
+            W = np.zeros(2 * nf * N * N)
+            trG = np.zeros(2 * nf)
+            sG = np.zeros(2 * nf)
+            dp = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))
+            L.dqmc_engine_adjoint.argtypes = [C.c_void_p] + [C.POINTER(C.c_double)] * 4
+            rc = L.dqmc_engine_adjoint(eng.h, dp(logdet), dp(W), dp(trG), dp(sG))
+            if rc != 0:
+                raise _eng.DqmcError("dqmc_engine_adjoint: %s" % L.dqmc_last_error().decode())
+        finally:
+            eng.close()
+        # column-major buffers -> (row, col) matrices
+        Wm = W.reshape(nf, 2, N, N).transpose(0, 1, 3, 2)
+        return logdet.reshape(nf, 2).sum(axis=1), {
+            "logdet": logdet.reshape(nf, 2), "W": Wm, "trG": trG.reshape(nf, 2),
+            "sG": sG.reshape(nf, 2), "E": E}
+
+    def grad_K_from(self, K, mu, W_weighted, E):
+        """d/dK of sum_f w_f sum_s logdet, given sum_f w_f W_f,s summed over s
+        (both spins share K_s = K - mu I here)."""
+        dt = self.dt
+        A = -dt * (np.asarray(K, dtype=np.float64) - mu * np.eye(self.N))
+        Z = W_weighted @ np.linalg.inv(E)
+        Lf = expm_frechet(A, Z, compute_expm=False)
+        return -dt * Lf.T
+
+
+def logdet_and_grads(prob, K, mu, lam):
+    """Per-field log|det M_up M_dn| and the gradients of their SUM with respect
+    to K (full matrix), mu and lambda (per field for mu and lambda)."""
+    f, d = prob.run(K, mu, lam)
+    Wsum = d["W"].sum(axis=(0, 1))
+    gK = prob.grad_K_from(K, mu, Wsum, d["E"])
+    N, L, dt = prob.N, prob.L, prob.dt
+    gmu = dt * (2 * N * L - d["trG"].sum(axis=1))
+    glam = d["sG"].sum(axis=1)
+    return f, gK, gmu, glam, d
+
+
+try:
+    import torch
+
+    class LogDet(torch.autograd.Function):
+        """y[f] = log|det M_up M_dn| of field f; differentiable in K (N x N),
+        mu and lam (0-d tensors).  `prob` carries the fields and lattice."""
+
+        @staticmethod
+        def forward(ctx, K, mu, lam, prob):
+            Kn = K.detach().cpu().numpy()
+            f, d = prob.run(Kn, float(mu), float(lam))
+            ctx.prob, ctx.K, ctx.mu, ctx.d = prob, Kn, float(mu), d
+            return torch.as_tensor(f, dtype=torch.float64)
+
+        @staticmethod
+        def backward(ctx, g):
+            prob, d = ctx.prob, ctx.d
+            w = g.detach().cpu().numpy().astype(np.float64)
+            Ww = np.tensordot(w, d["W"].sum(axis=1), axes=(0, 0))
+            gK = prob.grad_K_from(ctx.K, ctx.mu, Ww, d["E"])
+            N, L, dt = prob.N, prob.L, prob.dt
+            gmu = float(np.dot(w, dt * (2 * N * L - d["trG"].sum(axis=1))))
+            glam = float(np.dot(w, d["sG"].sum(axis=1)))
+            return (torch.as_tensor(gK, dtype=torch.float64), torch.tensor(gmu, dtype=torch.float64),
+                    torch.tensor(glam, dtype=torch.float64), None)
+except ImportError:  # torch is optional for the numpy interface
+    LogDet = None

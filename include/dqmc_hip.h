@@ -156,6 +156,21 @@ int dqmc_engine_calc_eq_g(dqmc_engine *e, int chain, int spin, int f, double *G,
 int dqmc_engine_calc_ue_g(dqmc_engine *e, int chain, int spin, double *G0t, double *Gtt,
                           double *Gt0);
 
+/* Adjoint of log|det M_s| through the stabilized B-chain.  No counterpart in
+ * the reference, which computes only the phase (its determinant code is
+ * commented out, src/greens.c:180-191).  For the current HS field of every
+ * chain (no updates, state unchanged), with B_l = E e^{V_l}, E = e^{-dt K}
+ * the engine's exp_K and G_l the Green's function with B_l at the right end:
+ *   logdet[2c+s] = log|det(I + B_{L-1}..B_0)|                 (double)
+ *   W[2c+s]      = sum_l e^{V_l} (I - G_l) e^{-V_l}, N x N column-major (num);
+ *                  d logdet = Re tr(W E^-1 dE)
+ *   trG[2c+s]    = sum_l tr G_l (num);  d logdet / d mu = dt (N L - trG)
+ *   sG[2c+s]     = sum_l sum_i sigma_s s_li (1 - G_l,ii) (num), s = 2 hs - 1,
+ *                  sigma = +1 up / -1 down;  d logdet / d lambda = sG
+ * G is advanced by wraps and recomputed from the stored group products every
+ * n_matmul slices.  Any pointer may be NULL.  The next run recomputes G. */
+int dqmc_engine_adjoint(dqmc_engine *e, double *logdet, double *W, double *trG, double *sG);
+
 /* Accepted flips so far, summed over chains (for the flop model). */
 int64_t dqmc_engine_accepts(dqmc_engine *e);
 

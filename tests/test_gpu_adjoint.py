@@ -1,0 +1,93 @@
+"""Adjoint of log|det M| on the GPU (dqmc_engine_adjoint + dqmc_amd.adjoint).
+
+(1) The engine's per-field pieces -- log|det M_s|, W = sum_l e^{V_l}(I - G_l)
+    e^{-V_l}, sum_l tr G_l and the lambda term -- against explicit products on
+    a 4x4 lattice (brute force, fine at beta = 2).
+(2) The differentiation workload: 10x10 square lattice, t' = -0.2, U = 4,
+    mu = -0.3, beta = 5 (dt = 0.1, L = 50), UDT checkpoints every 10 slices,
+    64 fixed fields drawn as create_batch(seed=2024) draws them.  Gradients
+    with respect to mu, lambda, t, t' and entries of the full K against
+    central finite differences (h = 1e-6), and the tr(G dM) identity
+    d/dmu = -tr(d/dK) = dt (2 N L - sum tr G).
+(3) The torch.autograd.Function."""
+import numpy as np
+import pytest
+
+from dqmc_amd import adjoint, gen
+
+pytestmark = pytest.mark.gpu
+
+
+def _fields(Nx, Ny, L, nf, seed):
+    sims = gen.create_batch(Nfiles=nf, seed=seed, Nx=Nx, Ny=Ny, L=L, n_matmul=10, period_eqlt=10,
+                            n_sweep_warm=0, n_sweep_meas=1)
+    return np.array([s["state"]["hs"] for s in sims])
+
+
+def test_engine_pieces_match_brute_force():
+    from test_adjoint_host import _brute
+    Nx, Ny, L = 4, 4, 20
+    prob = adjoint.AdjointProblem(Nx, Ny, L, 0.1, _fields(Nx, Ny, L, 3, 11), n_matmul=10)
+    K = prob.hopping(1.0, -0.2)
+    mu, lam = -0.3, adjoint.lambda_of_U(4.0, 0.1)[0]
+    f, d = prob.run(K, mu, lam)
+    for k in range(3):
+        b, _ = _brute(prob, K, mu, lam, prob.fields[k])
+        assert abs(f[k] - b["logdet"]) < 1e-10 * abs(b["logdet"])
+        W = d["W"][k].sum(axis=0)
+        assert np.max(np.abs(W - b["W"])) < 1e-9 * np.max(np.abs(b["W"]))
+        assert abs(d["trG"][k].sum() - b["trG"]) < 1e-9 * abs(b["trG"])
+        assert abs(d["sG"][k].sum() - b["sG"]) < 1e-9 * max(1.0, abs(b["sG"]))
+
+
+@pytest.fixture(scope="module")
+def cfg5():
+    Nx = Ny = 10
+    L = 50
+    prob = adjoint.AdjointProblem(Nx, Ny, L, 0.1, _fields(Nx, Ny, L, 64, 2024), n_matmul=10)
+    return prob, -0.3, adjoint.lambda_of_U(4.0, 0.1)[0]
+
+
+def test_gradients_match_central_differences(cfg5):
+    prob, mu, lam = cfg5
+    t, tp = 1.0, -0.2
+    K = prob.hopping(t, tp)
+    f0, gK, gmu, glam, d = adjoint.logdet_and_grads(prob, K, mu, lam)
+    F = lambda K_, mu_, lam_: prob.run(K_, mu_, lam_)[0].sum()
+    h = 1e-6
+    # mu and the tr(G dM) identity
+    fd = (F(K, mu + h, lam) - F(K, mu - h, lam)) / (2 * h)
+    assert abs(gmu.sum() - fd) < 1e-6 * abs(fd)
+    assert abs(gmu.sum() + np.trace(gK)) < 1e-9 * abs(gmu.sum())
+    # lambda
+    fd = (F(K, mu, lam + h) - F(K, mu, lam - h)) / (2 * h)
+    assert abs(glam.sum() - fd) < 1e-6 * max(abs(fd), 1.0)
+    # t, t' (projections of the K gradient)
+    for pat, v in ((prob.t_pattern, "t"), (prob.tp_pattern, "t'")):
+        g = float(np.sum(gK * pat))
+        fd = (F(K + h * pat, mu, lam) - F(K - h * pat, mu, lam)) / (2 * h)
+        assert abs(g - fd) < 1e-6 * abs(fd), v
+    # individual entries of the full K
+    for (i, j) in [(0, 1), (5, 5), (17, 42)]:
+        E = np.zeros_like(K)
+        E[i, j] = 1.0
+        fd = (F(K + h * E, mu, lam) - F(K - h * E, mu, lam)) / (2 * h)
+        assert abs(gK[i, j] - fd) < 1e-6 * max(abs(fd), np.max(np.abs(gK))), (i, j, gK[i, j], fd)
+
+
+def test_torch_autograd_function(cfg5):
+    torch = pytest.importorskip("torch")
+    prob, mu, lam = cfg5
+    K = torch.tensor(prob.hopping(1.0, -0.2), dtype=torch.float64, requires_grad=True)
+    m = torch.tensor(mu, dtype=torch.float64, requires_grad=True)
+    la = torch.tensor(lam, dtype=torch.float64, requires_grad=True)
+    w = torch.linspace(0.5, 1.5, 64, dtype=torch.float64)
+    y = (adjoint.LogDet.apply(K, m, la, prob) * w).sum()
+    y.backward()
+    f0, gK, gmu, glam, d = adjoint.logdet_and_grads(prob, K.detach().numpy(), mu, lam)
+    # weighted gradient from the per-field pieces
+    Ww = np.tensordot(w.numpy(), d["W"].sum(axis=1), axes=(0, 0))
+    gKw = prob.grad_K_from(K.detach().numpy(), mu, Ww, d["E"])
+    assert np.max(np.abs(K.grad.numpy() - gKw)) < 1e-12 * np.max(np.abs(gKw))
+    assert abs(float(m.grad) - float(np.dot(w.numpy(), gmu))) < 1e-9 * abs(float(m.grad))
+    assert abs(float(la.grad) - float(np.dot(w.numpy(), glam))) < 1e-9 * max(1.0, abs(float(la.grad)))
