@@ -42,6 +42,8 @@ struct MeasArgs {
     const T* pp; long spp;             // [c][4][num_b2]: pp_u, ppr_u, pp_d, ppr_d
     const int* plaqs; long splaqs;     // [c][3 num_plaq]
     T* gs; long sgs;                   // gen_suscept partial sums [c][L][4][num_ij]
+    const int* invij; long sinv;       // [c][j][num_ij]: the row i that column j adds to bin r
+                                       // (when every column feeds every bin once)
 };
 
 // accumulator indices (match enum dqmc_eq_acc / dqmc_ue_acc)
@@ -208,6 +210,109 @@ __global__ __launch_bounds__(256) void k_meas_ue(MeasArgs<T> a)
     acc[off[UE_nn] + o] = nn; acc[off[UE_xx] + o] = xx; acc[off[UE_zz] + o] = zz;
     acc[off[UE_pair_sw] + o] = psw;
     if (a.energy) { acc[off[UE_vv] + o] = vv; acc[off[UE_vn] + o] = vn; }
+}
+
+// The same contraction for maps in which every column j adds exactly one term
+// to every bin (translation-symmetric maps, num_ij = N): one workgroup per
+// (t, chain), one thread per bin.  Row strips of G(0,t) (and of the Peierls
+// factors) are staged through LDS with line-coalesced reads, and the diagonals
+// of G(t,t) and G(0,0) are kept in LDS; each bin still adds its terms in
+// increasing j, the reference's order.
+#define UES_NMAX 256
+template <typename T, int CH>
+__global__ __launch_bounds__(256) void k_meas_ue_strip(MeasArgs<T> a)
+{
+    constexpr bool Z = is_cplx<T>::value;
+    __shared__ T strip[2][CH][UES_NMAX];               // G0t_s[j0 + jj, i]
+    __shared__ T pstrip[Z ? 2 : 1][Z ? CH : 1][Z ? UES_NMAX : 1];  // P_s[j0 + jj + N i]
+    __shared__ T dtt[2][UES_NMAX], doo[2][UES_NMAX];
+    const int c = blockIdx.y;
+    if (!a.flag[c]) return;
+    const int tt = blockIdx.x;
+    const int N = a.N, nij = a.num_ij;
+    const long NN = (long)N * N;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    const T phase = a.phase[c];
+    if (tid == 0 && tt == 0) {
+        a.n_sample[c] += 1;
+        a.sign[c] += phase;
+    }
+    const T* G0t[2] = {a.G0t + (2 * c) * a.sU + NN * tt, a.G0t + (2 * c + 1) * a.sU + NN * tt};
+    const T* Gt0[2] = {a.Gt0 + (2 * c) * a.sU + NN * tt, a.Gt0 + (2 * c + 1) * a.sU + NN * tt};
+    const T* P[2] = {a.peierlsu + c * a.sP, a.peierlsd + c * a.sP};
+    for (int i = tid; i < N; i += nt) {
+#pragma unroll
+        for (int s = 0; s < 2; s++) {
+            dtt[s][i] = a.Gtt[(2 * c + s) * a.sU + NN * tt + i + (long)N * i];
+            doo[s][i] = a.Gtt[(2 * c + s) * a.sU + i + (long)N * i];
+        }
+    }
+    const int* inv = a.invij + c * a.sinv;
+    const int r = tid;
+    const T pre = phase * (1.0 / a.degen[4 * c + 1]);
+    const T zero = from_re<T>(0.0), one = from_re<T>(1.0), two = from_re<T>(2.0);
+    T* acc = a.acc + c * a.sacc;
+    const long* off = a.off;
+    const long o = r + (long)nij * tt;
+    T gt0 = zero, gt0u = zero, gt0d = zero, nn = zero, xx = zero, zz = zero, psw = zero;
+    T vv = zero, vn = zero;
+    if (r < nij) {   // terms add straight into the running sums, as the reference's do
+        gt0 = acc[off[UE_gt0] + o]; gt0u = acc[off[UE_gt0_u] + o]; gt0d = acc[off[UE_gt0_d] + o];
+        nn = acc[off[UE_nn] + o]; xx = acc[off[UE_xx] + o]; zz = acc[off[UE_zz] + o];
+        psw = acc[off[UE_pair_sw] + o];
+        if (a.energy) { vv = acc[off[UE_vv] + o]; vn = acc[off[UE_vn] + o]; }
+    }
+    const double delta_t = (tt == 0) ? 1.0 : 0.0;
+    for (int j0 = 0; j0 < N; j0 += CH) {
+        const int ch = (N - j0 < CH) ? N - j0 : CH;
+        __syncthreads();
+        for (int e = tid; e < CH * N; e += nt) {
+            const int jj = e % CH, i = e / CH;
+            if (jj < ch) {
+#pragma unroll
+                for (int s = 0; s < 2; s++) {
+                    strip[s][jj][i] = G0t[s][(j0 + jj) + (long)N * i];
+                    if constexpr (Z) pstrip[s][jj][i] = P[s][(j0 + jj) + (long)N * i];
+                }
+            }
+        }
+        __syncthreads();
+        if (r < nij) {
+            for (int jj = 0; jj < ch; jj++) {
+                const int j = j0 + jj;
+                const int i = inv[(long)j * nij + r];
+                const double dtij = delta_t * ((i == j) ? 1.0 : 0.0);
+                const T guii = dtt[0][i], gdii = dtt[1][i], gujj = doo[0][j], gdjj = doo[1][j];
+                const T guij = Gt0[0][i + (long)N * j], gdij = Gt0[1][i + (long)N * j];
+                const T guji = strip[0][jj][i], gdji = strip[1][jj][i];
+                T pgu = guij, pgd = gdij;
+                if constexpr (Z) {   // g * p, the product order of peierls_mul
+                    pgu = guij * pstrip[0][jj][i];
+                    pgd = gdij * pstrip[1][jj][i];
+                }
+                gt0 += 0.5 * (pre * (pgu + pgd));
+                gt0u += pre * pgu;
+                gt0d += pre * pgd;
+                const T x = dtij * (guii + gdii) - (guji * guij + gdji * gdij);
+                nn += pre * ((two - guii - gdii) * (two - gujj - gdjj) + x);
+                xx += 0.25 * (pre * (dtij * (guii + gdii) - (guji * gdij + gdji * guij)));
+                zz += 0.25 * (pre * ((gdii - guii) * (gdjj - gujj) + x));
+                psw += pre * guij * gdij;
+                if (a.energy) {
+                    const T nuinuj = (one - guii) * (one - gujj) + (dtij - guji) * guij;
+                    const T ndindj = (one - gdii) * (one - gdjj) + (dtij - gdji) * gdij;
+                    vv += pre * nuinuj * ndindj;
+                    vn += pre * (nuinuj * (one - gdii) + (one - guii) * ndindj);
+                }
+            }
+        }
+    }
+    if (r < nij) {
+        acc[off[UE_gt0] + o] = gt0; acc[off[UE_gt0_u] + o] = gt0u; acc[off[UE_gt0_d] + o] = gt0d;
+        acc[off[UE_nn] + o] = nn; acc[off[UE_xx] + o] = xx; acc[off[UE_zz] + o] = zz;
+        acc[off[UE_pair_sw] + o] = psw;
+        if (a.energy) { acc[off[UE_vv] + o] = vv; acc[off[UE_vn] + o] = vn; }
+    }
 }
 
 // Peierls factor p[i + N j] (complex builds; 1 in real builds)

@@ -319,7 +319,12 @@ static int ue_measure(dqmc_engine* e)
     ma.Gt0 = dp<T>(e->Gt0);
     ma.sU = (long)e->L * e->N * e->N;
     const int L = e->L, nc = e->nc;
-    k_meas_ue<T><<<dim3((e->num_ij + 255) / 256, L, nc), 256, 0, e->st>>>(ma);
+    if (e->ij_percol) {
+        const int thr = (e->num_ij + 63) / 64 * 64;
+        k_meas_ue_strip<T, is_cplx<T>::value ? 8 : 16><<<dim3(L, nc), thr, 0, e->st>>>(ma);
+    } else {
+        k_meas_ue<T><<<dim3((e->num_ij + 255) / 256, L, nc), 256, 0, e->st>>>(ma);
+    }
     LCHK();
     if (e->bond || e->thermal || e->pbbonly) {
         k_meas_ue_pairs<T, PK_BB><<<dim3((e->num_bb + 255) / 256, L, nc), 256, 0, e->st>>>(ma);

@@ -189,6 +189,8 @@ struct dqmc_engine {
     // adjoint outputs (allocated on first use)
     void *adjW = 0, *adjT = 0;
     double* adjL = 0;
+    int* invij = 0;      // [c][j][num_ij] row of column j in bin r (when ij_percol)
+    bool ij_percol = false;
     // blocked QR workspace: explicit V, compact-WY T factors, GEMM scratch, dets
     void *Vb = 0, *Tb = 0, *Wq1 = 0, *Wq2 = 0, *detq = 0, *detm = 0;
     long sVb = 0, sTb = 0, sWq = 0;
@@ -654,6 +656,7 @@ static MeasArgs<T> meas_args(dqmc_engine* e, int ue)
     a.pp = dp<T>(e->pp); a.spp = 4L * e->num_b2;
     a.plaqs = e->plaqs; a.splaqs = 3L * e->num_plaq;
     a.gs = dp<T>(e->gs); a.sgs = (long)e->L * 4 * e->num_ij;
+    a.invij = e->invij; a.sinv = (long)e->N * e->num_ij;
     if (ue) {
         a.flag = e->flag_ue; a.acc = dp<T>(e->acc_ue); a.sacc = e->ue_total; a.off = e->d_ue_off;
         a.n_sample = e->ns_ue; a.sign = dp<T>(e->sign_ue);
@@ -949,6 +952,21 @@ static int engine_setup(dqmc_engine* e, const dqmc_params* const* P)
         DALLOC(e->ix2[m], (long)nc * e->six2[m] * 4);
     }
     DALLOC(e->degen2, (long)nc * PM_COUNT * 4);
+    // does every column j of map_ij feed every bin exactly once (translation
+    // symmetry)?  Then the strip kernel can walk columns with an inverse table.
+    e->ij_percol = e->period_uneqlt > 0 && e->num_ij == N && N <= UES_NMAX;
+    for (int c = 0; c < nc && e->ij_percol; c++) {
+        std::vector<char> seen(N);
+        for (int j = 0; j < N && e->ij_percol; j++) {
+            std::fill(seen.begin(), seen.end(), 0);
+            for (int i = 0; i < N; i++) {
+                const int r = P[c]->map_ij[i + (long)N * j];
+                if (r < 0 || r >= N || seen[r]) { e->ij_percol = false; break; }
+                seen[r] = 1;
+            }
+        }
+    }
+    if (e->ij_percol) DALLOC(e->invij, (long)nc * N * N * 4);
     DALLOC(e->bond2s, (long)nc * 2 * (nb2 > 0 ? nb2 : 1) * 4);
     DALLOC(e->pp, (long)nc * 4 * (nb2 > 0 ? nb2 : 1) * es);
     DALLOC(e->plaqs, (long)nc * 3 * (e->num_plaq > 0 ? e->num_plaq : 1) * 4);
@@ -982,6 +1000,12 @@ static int engine_setup(dqmc_engine* e, const dqmc_params* const* P)
         csr_of_map(p->map_i, N, e->num_i, rp, ix);
         HIPCHK(hipMemcpy(e->rp_i + c * (e->num_i + 1L), rp.data(), rp.size() * 4, hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(e->ix_i + c * (long)N, ix.data(), ix.size() * 4, hipMemcpyHostToDevice));
+        if (e->ij_percol) {
+            std::vector<int> inv((size_t)N * N);
+            for (int j = 0; j < N; j++)
+                for (int i = 0; i < N; i++) inv[(size_t)j * N + p->map_ij[i + (long)N * j]] = i;
+            HIPCHK(hipMemcpy(e->invij + c * NN, inv.data(), NN * 4, hipMemcpyHostToDevice));
+        }
         csr_of_map(p->map_ij, (int)NN, e->num_ij, rp, ix);
         HIPCHK(hipMemcpy(e->rp_ij + c * (e->num_ij + 1L), rp.data(), rp.size() * 4, hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(e->ix_ij + c * NN, ix.data(), ix.size() * 4, hipMemcpyHostToDevice));
