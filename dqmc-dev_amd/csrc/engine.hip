@@ -25,6 +25,7 @@
 #include "dq_meas.hpp"
 #include "dq_meas2.hpp"
 #include "dq_adjoint.hpp"
+#include "dq_qrp.hpp"
 #include "dq_qr_blocked.hpp"
 #include "dq_rng.hpp"
 #include "dq_ue.hpp"
@@ -356,6 +357,41 @@ static void launch_panel(dqmc_engine* e, int batch, int m, int k0, int nbp, T* A
 // A[b] (m x n, lda) <- R (upper) + reflectors (below); tau[b][0..min(m,n));
 // Q[b] (m x m, ldq), if non-null, <- H_0 ... H_{k-1}; det[b], if non-null,
 // <- prod det(H_i).  Panels of 64 (real) / 32 (complex) columns.
+// Q = H_0 H_1 ... H_{kmax-1} (m x m, ldq) from the explicit reflectors in the
+// V workspace (ld m) and the compact-WY T factors of their NB-column panels in
+// the T workspace: Q <- (I - V_p T_p V_p^H) Q, panels from the last.
+template <typename T>
+static int accumulate_q(dqmc_engine* e, int m, int kmax, int NB, T* Q, int ldq, long sQ, int batch)
+{
+    T* V = dp<T>(e->Vb);
+    const long sV = e->sVb;
+    const int ldv = m;
+    T* Tb = dp<T>(e->Tb);
+    const long sT = e->sTb;
+    T* W1 = dp<T>(e->Wq1);
+    T* W2 = dp<T>(e->Wq2);
+    const long sW = e->sWq;
+    const T one = from_re<T>(1.0), mone = from_re<T>(-1.0), zero = from_re<T>(0.0);
+    const int np = (kmax + NB - 1) / NB;
+    k_set_identity<T><<<nblk((long)m * m * batch), 256, 0, e->st>>>(m, Q, sQ, ldq, 0, batch);
+    LCHK();
+    for (int p = np - 1; p >= 0; p--) {
+        const int k0 = p * NB;
+        const int nbp = (kmax - k0 < NB) ? kmax - k0 : NB;
+        const int mp = m - k0;
+        T* Tk = Tb + (long)p * NB * NB;
+        T* Vk = V + k0 + (long)ldv * k0;
+        T* Qs = Q + k0 + (long)ldq * k0;
+        RC(gemm<T>(e, OP_C, OP_N, nbp, mp, mp, one, Vk, ldv, sV, Qs, ldq, sQ, zero, W1, NB, sW,
+                   batch));
+        RC(gemm<T>(e, OP_N, OP_N, nbp, mp, nbp, one, Tk, NB, sT, W1, NB, sW, zero, W2, NB, sW,
+                   batch));
+        RC(gemm<T>(e, OP_N, OP_N, mp, mp, nbp, mone, Vk, ldv, sV, W2, NB, sW, one, Qs, ldq, sQ,
+                   batch));
+    }
+    return 0;
+}
+
 template <typename T>
 static int qr_blocked(dqmc_engine* e, int m, int n, T* A, int lda, long sA, T* tau, long stau,
                       T* Q, int ldq, long sQ, T* det, int batch)
@@ -402,25 +438,59 @@ static int qr_blocked(dqmc_engine* e, int m, int n, T* A, int lda, long sA, T* t
                        batch));
         }
     }
-    if (Q) {
-        k_set_identity<T><<<nblk((long)m * m * batch), 256, 0, e->st>>>(m, Q, sQ, ldq, 0, batch);
-        LCHK();
-        for (int p = np - 1; p >= 0; p--) {
-            const int k0 = p * NB;
-            const int nbp = (kmax - k0 < NB) ? kmax - k0 : NB;
-            const int mp = m - k0;
-            T* Tk = Tb + (long)p * NB * NB;
-            T* Vk = V + k0 + (long)ldv * k0;
-            T* Qs = Q + k0 + (long)ldq * k0;
-            RC(gemm<T>(e, OP_C, OP_N, nbp, mp, mp, one, Vk, ldv, sV, Qs, ldq, sQ, zero, W1, NB, sW,
-                       batch));
-            RC(gemm<T>(e, OP_N, OP_N, nbp, mp, nbp, one, Tk, NB, sT, W1, NB, sW, zero, W2, NB, sW,
-                       batch));
-            RC(gemm<T>(e, OP_N, OP_N, mp, mp, nbp, mone, Vk, ldv, sV, W2, NB, sW, one, Qs, ldq, sQ,
-                       batch));
-        }
-    }
+    if (Q) RC(accumulate_q<T>(e, m, kmax, NB, Q, ldq, sQ, batch));
     return 0;
+}
+
+// Q column-pivoted QR of the first product with the matrix resident on one CU
+// (k_qrp_res), when it fits: real n <= 256, complex n <= 144.
+template <typename T>
+static bool qrp_resident_fits(int n)
+{
+    return is_cplx<T>::value ? n <= 144 : n <= 256;
+}
+
+template <typename T>
+static int qrp_resident(dqmc_engine* e, int n, T* A, long sA, T* tau, int* jpvt, T* Q, long sQ,
+                        int batch)
+{
+    constexpr bool Z = is_cplx<T>::value;
+    T* V = dp<T>(e->Vb);
+    const long sV = e->sVb;
+#define QRP(R, SR, SL, CB)                                                                  \
+    k_qrp_res<T, R, SR, SL, CB><<<batch, QRP_THREADS, 0, e->st>>>(n, n, A, n, sA, tau, n, jpvt, \
+                                                                  n, V, n, sV)
+    if constexpr (Z) {
+        if (n <= 64) QRP(1, 8, 0, 2);
+        else if (n <= 128) QRP(2, 14, 2, 2);
+        else if (n <= 144) QRP(3, 14, 4, 2);
+        else return set_err(DQMC_ERR_UNSUPPORTED, "resident pivoted QR of order %d", n);
+    } else {
+        if (n <= 64) QRP(1, 8, 0, 4);
+        else if (n <= 128) QRP(2, 16, 0, 4);
+        else if (n <= 192) QRP(3, 24, 0, 4);
+        else if (n <= 256) QRP(4, 24, 8, 4);
+        else return set_err(DQMC_ERR_UNSUPPORTED, "resident pivoted QR of order %d", n);
+    }
+#undef QRP
+    LCHK();
+    // explicit Q: compact-WY factors of 64-column panels from their Gram
+    // matrices, then the backward three-GEMM accumulation
+    const int NB = 64;
+    const int np = (n + NB - 1) / NB;
+    T* Tb = dp<T>(e->Tb);
+    const long sT = e->sTb;
+    const T one = from_re<T>(1.0), zero = from_re<T>(0.0);
+    for (int p = 0; p < np; p++) {
+        const int k0 = p * NB;
+        const int nbp = (n - k0 < NB) ? n - k0 : NB;
+        T* Vk = V + k0 + (long)n * k0;
+        RC(gemm<T>(e, OP_C, OP_N, nbp, nbp, n - k0, one, Vk, n, sV, Vk, n, sV, zero,
+                   Tb + (long)p * NB * NB, NB, sT, batch));
+    }
+    k_larft_gram<T><<<dim3(np, batch), 64, 0, e->st>>>(NB, n, tau, n, Tb, sT);
+    LCHK();
+    return accumulate_q<T>(e, n, n, NB, Q, n, sQ, batch);
 }
 
 // X[b] (n x ncols, ldx) <- U[b]^-1 X[b] for upper-triangular U (ldu): 64-row
@@ -486,7 +556,9 @@ static int calc_eq_g(dqmc_engine* e, int l, T* Gout, T* phase_b)
     int m = (l + 1 + (F - 1) % n_mul) % F;
     RC(mul_seq_C<T>(e, l, m, one, X, N, NN));
     // (1) column-pivoted QR of the first product (xgeqp3)
-    if (N <= 256)
+    if (qrp_resident_fits<T>(N))
+        RC(qrp_resident<T>(e, N, X, NN, tau, e->jpvt, Q, NN, B));
+    else if (N <= 256)
         k_qr<T, true, 4><<<B, QR_THREADS, 0, e->st>>>(N, N, X, N, NN, tau, N, e->jpvt, N, Q, N, NN);
     else if (N <= 512)
         k_qr<T, true, 8><<<B, QR_THREADS, 0, e->st>>>(N, N, X, N, NN, tau, N, e->jpvt, N, Q, N, NN);
