@@ -72,6 +72,175 @@ def cpu_baseline(seconds_budget=20.0):
                       % (out["sweeps_run"], dt)}
 
 
+# --------------------------------------------------------------------------
+# the other configurations (parity workloads of the test suite; each prints
+# its own JSON line when selected with --config)
+# --------------------------------------------------------------------------
+def _run_engines(groups, steps, warmup, threads):
+    """Run each engine (one per shape group) for warmup + steps sweeps, the
+    engines concurrently from `threads` host threads (each engine has its own
+    HIP stream); returns (seconds for the timed sweeps, total chain-sweeps)."""
+    import threading
+
+    def pool(fn, what):
+        todo = list(groups)
+        lock = threading.Lock()
+        done = [0]
+
+        def worker():
+            while True:
+                with lock:
+                    if not todo:
+                        return
+                    eng = todo.pop()
+                fn(eng)
+                with lock:
+                    done[0] += 1
+                    if len(groups) > 1 and done[0] % 16 == 0:
+                        print("%s: %d/%d engines" % (what, done[0], len(groups)),
+                              file=sys.stderr, flush=True)
+
+        ths = [threading.Thread(target=worker) for _ in range(min(threads, len(groups)))]
+        for t in ths:
+            t.start()
+        for t in ths:
+            t.join()
+
+    pool(lambda eng: (eng.run(0), eng.run(warmup)), "warmup")  # initial G + warmup, untimed
+    t0 = time.perf_counter()
+    pool(lambda eng: eng.run(steps), "timed")
+    dt = time.perf_counter() - t0
+    return dt, steps * sum(e.nc for e in groups)
+
+
+def _line(metric, value, unit, steps, warmup, dt, workload_desc, extra, nchains):
+    out = {"metric": metric, "value": round(value, 4), "unit": unit, "n_gpus": 1,
+           "steps": steps, "warmup": warmup, "ms_per_step": round(dt / steps * 1e3, 3),
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+           "data": "synthetic (package generator)",
+           "config": {"workload": workload_desc, "chains": nchains,
+                      "parallelism": "1 GPU, chains batched"}}
+    out.update(extra)
+    return out
+
+
+def bench_config1(args):
+    """4x4 N=16, t'=0, U=4, mu=0, beta=4 (L=40), n_matmul=8, delay 8, seed 1234,
+    100 warmup + 400 measurement sweeps, equal-time measurements."""
+    from dqmc_amd import engine, gen
+    kw = dict(Nx=4, Ny=4, tp=0.0, U=4.0, mu=0.0, dt=0.1, L=40, n_matmul=8, n_delay=8,
+              period_eqlt=8, n_sweep_warm=100, n_sweep_meas=400)
+    sim = gen.create_batch(Nfiles=1, seed=1234, **kw)
+    eng = engine.Engine(sim)
+    dt1, n1 = _run_engines([eng], 400, 100, 1)
+    eng.close()
+    nb = args.chains if args.chains != 256 else 1024
+    sims = gen.create_batch(Nfiles=nb, seed=1234, **kw)
+    engb = engine.Engine(sims)
+    dtb, nbs = _run_engines([engb], args.steps * 25, args.warmup, 1)
+    engb.close()
+    return _line("DQMC sweeps/s, 4x4 Hubbard N=16 L=40 FP64 (single chain)", n1 / dt1,
+                 "sweeps/s", 400, 100, dt1,
+                 "4x4 square Hubbard N=16 L=40 U=4 mu=0 beta=4 n_matmul=8 delay=8 seed 1234, "
+                 "100 warmup + 400 measurement sweeps, eq every 8 slices", 
+                 {"batched_chains": nb, "batched_sweeps_per_s": round(nbs / dtb, 2)}, 1)
+
+
+def bench_config3(args):
+    """Complex Peierls 12x12 N=144, nflux=1, t'=0, U=4, mu=-0.5, beta=6 (L=60),
+    n_matmul=10, delay 16, complex128, eq + unequal-time with the bond
+    (current-current jj, jsjs, kk, ksks) correlators."""
+    from dqmc_amd import engine, gen
+    kw = dict(Nx=12, Ny=12, tp=0.0, U=4.0, mu=-0.5, dt=0.1, L=60, nflux=1, n_matmul=10,
+              n_delay=16, period_eqlt=10, period_uneqlt=2, n_sweep_warm=0,
+              n_sweep_meas=100000, meas_bond_corr=1)
+    eng = engine.Engine(gen.create_batch(Nfiles=1, seed=3, **kw))
+    dt1, n1 = _run_engines([eng], max(2, args.steps), 1, 1)
+    eng.close()
+    nb = args.chains if args.chains != 256 else 128
+    engb = engine.Engine(gen.create_batch(Nfiles=nb, seed=3, **kw))
+    dtb, nbs = _run_engines([engb], args.steps, args.warmup, 1)
+    engb.close()
+    return _line("DQMC sweeps/s, complex Peierls 12x12 N=144 L=60 (batched chains, aggregate)",
+                 nbs / dtb, "sweeps/s", args.steps, args.warmup, dtb,
+                 "12x12 square Hubbard, nflux=1 (Landau gauge Peierls phases), complex128, "
+                 "N=144 L=60 U=4 mu=-0.5 beta=6 n_matmul=10 delay=16, eq every 10 slices, "
+                 "uneq + bond correlators every 2 sweeps",
+                 {"single_chain_sweeps_per_s": round(n1 / dt1, 4),
+                  "batched_over_single": round(nbs / dtb / (n1 / dt1), 3), "dtype": "c128"}, nb)
+
+
+def bench_config4(args):
+    """128 chains of uneven sizes: side 6..20, beta in [2, 10] (L = 20..100),
+    U in {2, 4, 8}, mu in [-1.5, 0], t'=0, seed 99 + chain (host draws from
+    numpy default_rng(99)).  Chains go to five size buckets (sides 6-8, 9-11,
+    12-14, 15-17, 18-20), each one engine padded to its largest N and L
+    (dqmc_amd.buckets: inert padding sites, identity padding slices); the
+    bucket's n_matmul is 10 (8 for N > 200) with a partial last group where it
+    does not divide a chain's L, delay rank min(32, N), eq measurements every
+    n_matmul slices.  The engines run concurrently."""
+    from dqmc_amd import buckets, gen
+    nch = args.chains if args.chains != 256 else 128
+    r = __import__("numpy").random.default_rng(99)
+    edges = ((6, 8), (9, 11), (12, 14), (15, 17), (18, 20))
+    groups = {}
+    for c in range(nch):
+        side = int(r.integers(6, 21))
+        L = int(round(float(r.uniform(2.0, 10.0)) / 0.1))
+        U = float(r.choice([2.0, 4.0, 8.0]))
+        mu = float(r.uniform(-1.5, 0.0))
+        N = side * side
+        # the chain's own file must satisfy the reference's L % n_matmul
+        # (and L % period_eqlt) rule; the bucket engine regroups it
+        nm = max(d for d in range(1, 11) if L % d == 0)
+        kw = dict(Nx=side, Ny=side, tp=0.0, U=U, mu=mu, dt=0.1, L=L, n_matmul=nm,
+                  n_delay=min(32, N), period_eqlt=nm, n_sweep_warm=0, n_sweep_meas=100000)
+        sim = gen.create_batch(Nfiles=1, seed=99 + c, **kw)[0]
+        bk = next(i for i, (lo, hi) in enumerate(edges) if lo <= side <= hi)
+        groups.setdefault(bk, []).append(sim)
+    engs = []
+    for bk in sorted(groups):
+        npad = edges[bk][1] ** 2
+        engs.append(buckets.PaddedBatch(groups[bk], 10 if npad < 200 else 8))
+        print("config 4: bucket %d: %d chains padded to N=%d L=%d" %
+              (bk, len(groups[bk]), engs[-1].N, engs[-1].L), file=sys.stderr, flush=True)
+    dt, n = _run_engines(engs, args.steps, args.warmup, len(engs))
+    for e in engs:
+        e.close()
+    return _line("DQMC sweeps/s, 128 uneven real chains (aggregate)", n / dt, "sweeps/s",
+                 args.steps, args.warmup, dt,
+                 "%d real chains, side 6..20, L 20..100, U in {2,4,8}, mu in [-1.5,0], "
+                 "seed 99+chain; %d size buckets (one padded engine each), run concurrently"
+                 % (nch, len(engs)), {"buckets": [[e.nc, e.N, e.L] for e in engs]}, nch)
+
+
+def bench_config5(args):
+    """Adjoint: 10x10 N=100, t'=-0.2, U=4, mu=-0.3, beta=5 (L=50), 64 fixed
+    HS fields from seed 2024, UDT checkpoints every 10 slices: forward
+    log|det M_up M_dn| and G, backward gradients for t, t', mu, lambda and the
+    full hopping matrix K."""
+    import numpy as np
+    from dqmc_amd import adjoint
+    nf = 64
+    fields = np.random.default_rng(2024).integers(0, 2, size=(nf, 50, 100)).astype(np.int32)
+    prob = adjoint.AdjointProblem(10, 10, 50, 0.1, fields, n_matmul=10)
+    K = prob.hopping(1.0, -0.2)
+    lam, _ = adjoint.lambda_of_U(4.0, 0.1)
+    adjoint.logdet_and_grads(prob, K, -0.3, lam)  # warm (engine build, kernels)
+    reps = max(1, args.steps)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        f, gK, gmu, glam, d = adjoint.logdet_and_grads(prob, K, -0.3, lam)
+        gt = float(np.sum(gK * prob.t_pattern))
+        gtp = float(np.sum(gK * prob.tp_pattern))
+    dt = time.perf_counter() - t0
+    return _line("DQMC adjoint evaluations/s, 10x10 N=100 L=50 (forward + gradients per HS field)",
+                 nf * reps / dt, "fields/s", reps, 1, dt,
+                 "10x10 square Hubbard N=100 L=50 t'=-0.2 U=4 mu=-0.3 beta=5, 64 HS fields "
+                 "(seed 2024), checkpoints every 10 slices; log|det M| and dlogdet/d{t,t',mu,"
+                 "lambda,K}", {"grad_t_sum": round(gt, 6), "grad_tp_sum": round(gtp, 6)}, nf)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -85,6 +254,12 @@ def main():
                          "calibrates the FETCH_SIZE / WRITE_SIZE counters of a profiler pass)")
     ap.add_argument("--gemm-tile", type=int, default=0,
                     help="GEMM C tile: 0 = autotune, 1 = 64x64, 2 = 128x128, 3 = 128x64")
+    ap.add_argument("--config", type=int, default=2, choices=(1, 2, 3, 4, 5),
+                    help="workload: 2 = the headline N=256 L=80 batch (default); 1 = 4x4 "
+                         "single chain; 3 = complex Peierls N=144; 4 = 128 uneven chains; "
+                         "5 = adjoint gradients")
+    ap.add_argument("--threads", type=int, default=8,
+                    help="host threads driving the per-shape engines of --config 4")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (chains shard trivially across GPUs, "
@@ -92,6 +267,11 @@ def main():
         sys.exit(2)
     import numpy as np
     from dqmc_amd import engine
+
+    if args.config != 2:
+        fn = {1: bench_config1, 3: bench_config3, 4: bench_config4, 5: bench_config5}[args.config]
+        print(json.dumps(fn(args)))
+        return
 
     if args.copy_probe > 0:
         import torch

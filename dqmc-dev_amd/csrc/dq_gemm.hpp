@@ -37,6 +37,11 @@ struct GemmArgs {
     T alpha, beta;
     int beta_zero;            // 1: C is write-only
     int adiv, bdiv;           // batch z uses A[z / adiv], B[z / bdiv] (shared operands)
+    // identity factor: for batch z with idl >= idLc[z / 2] the operand on side
+    // idside (1 = A, 2 = B) is the identity, C = alpha X + beta C with X the
+    // other operand, scales ignored (an identity slice of a padded chain);
+    // idLc null: none.  Square products with op N only.
+    const int* idLc; int idl; int idside;
 };
 
 // Tile shapes (rows x columns of C per 256-thread workgroup).  The engine's
@@ -98,6 +103,19 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgs<T> g, int tiles_m, int ti
     T* C = g.C + (long)b * g.sC;
     const double* u = g.u ? g.u + (long)b * g.su : nullptr;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    if (g.idLc && g.idl >= g.idLc[b >> 1]) {  // uniform over the workgroup
+        const T* X = (g.idside == 1) ? B : A;
+        const int ldx = (g.idside == 1) ? g.ldb : g.lda;
+        for (int e = tid; e < BM * BN; e += 256) {
+            const int row = m0 + e % BM, col = n0 + e / BM;
+            if (row < g.M && col < g.N) {
+                T v = g.alpha * X[row + (long)ldx * col];
+                if constexpr (BETA) v += g.beta * C[row + (long)g.ldc * col];
+                C[row + (long)g.ldc * col] = v;
+            }
+        }
+        return;
+    }
     const int l16 = lane & 15, l4 = lane >> 4;
     const int wm = (wave >> 1) * 16 * TM, wn = (wave & 1) * 16 * TN;
 

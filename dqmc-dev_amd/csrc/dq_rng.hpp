@@ -52,11 +52,14 @@ __device__ inline void xs_group<15>(XsState& st, uint64_t* out, int lane)
 // state advanced.  One wave per chain; all lanes run the (uniform) generator,
 // lane 0 stores.
 __global__ __launch_bounds__(64) void k_rng_raw(int N, int L, const int* draws, uint64_t* rng,
-                                                uint64_t* raw, long sraw, const int* active)
+                                                uint64_t* raw, long sraw, const int* active,
+                                                const int* Lc, const int* Nc)
 {
     const int c = blockIdx.x;
     if (!active[c]) return;
-    const int T = L * (N + draws[c]);
+    // a chain padded to N, L draws for its own sites and slices only
+    const int Lr = Lc ? Lc[c] : L, Nr = Nc ? Nc[c] : N;
+    const int T = Lr * (Nr + draws[c]);
     const int lane = threadIdx.x;
     uint64_t* s = rng + 17L * c;
     uint64_t* out = raw + c * sraw;
@@ -93,10 +96,14 @@ __global__ __launch_bounds__(64) void k_rng_raw(int N, int L, const int* draws, 
 // One 256-thread block per (chain); slices processed in sequence.
 __global__ __launch_bounds__(256) void k_rng_orders(int N, int L, const uint64_t* raw, long sraw,
                                                     const double* del, const int* draws,
-                                                    int* order, double* unif, const int* active)
+                                                    int* order, double* unif, const int* active,
+                                                    const int* Lc, const int* Nc)
 {
     const int c = blockIdx.x;
     if (!active[c]) return;
+    // padded chains: orders over the chain's own Nr sites for its own Lr
+    // slices; the padding sites follow in place and are never accepted
+    const int Lr = Lc ? Lc[c] : L, Nr = Nc ? Nc[c] : N;
     __shared__ int jj[RNG_MAXN];
     __shared__ int ord[RNG_MAXN];
     const int tid = threadIdx.x;
@@ -104,12 +111,12 @@ __global__ __launch_bounds__(256) void k_rng_orders(int N, int L, const uint64_t
     const double* dl = del + 2L * N * c;
     const int nd = draws[c];
     long pos = 0;
-    for (int l = 0; l < L; l++) {
-        for (int i = tid; i < N; i += blockDim.x)
+    for (int l = 0; l < Lr; l++) {
+        for (int i = tid; i < Nr; i += blockDim.x)
             jj[i] = (int)((x[pos + i] >> 3) % (uint64_t)(i + 1));
         __syncthreads();
         if (tid == 0) {
-            for (int i = 0; i < N; i++) {
+            for (int i = 0; i < Nr; i++) {
                 const int j = jj[i];
                 if (i != j) ord[i] = ord[j];
                 ord[j] = i;
@@ -118,15 +125,17 @@ __global__ __launch_bounds__(256) void k_rng_orders(int N, int L, const uint64_t
         __syncthreads();
         int* o = order + ((long)c * L + l) * N;
         double* u = unif + ((long)c * L + l) * N;
-        for (int i = tid; i < N; i += blockDim.x) o[i] = ord[i];
-        if (nd == N) {
-            for (int i = tid; i < N; i += blockDim.x) {
-                const uint64_t bits = (0x3FFULL << 52) | (x[pos + N + i] >> 12);
+        for (int i = tid; i < N; i += blockDim.x) o[i] = (i < Nr) ? ord[i] : i;
+        for (int i = Nr + tid; i < N; i += blockDim.x)
+            u[i] = __longlong_as_double(0x7FF0000000000000LL);  // +inf: never accepted
+        if (nd == Nr) {
+            for (int i = tid; i < Nr; i += blockDim.x) {
+                const uint64_t bits = (0x3FFULL << 52) | (x[pos + Nr + i] >> 12);
                 u[i] = __longlong_as_double((long long)bits) - 1.0;
             }
         } else if (tid == 0) {
-            long q = pos + N;
-            for (int ii = 0; ii < N; ii++) {
+            long q = pos + Nr;
+            for (int ii = 0; ii < Nr; ii++) {
                 const int i = ord[ii];
                 if (dl[i] == 0.0 && dl[i + N] == 0.0) {
                     u[ii] = __longlong_as_double(0x7FF0000000000000LL);  // +inf: never accepted
@@ -136,7 +145,7 @@ __global__ __launch_bounds__(256) void k_rng_orders(int N, int L, const uint64_t
                 }
             }
         }
-        pos += N + nd;
+        pos += Nr + nd;
         __syncthreads();
     }
 }

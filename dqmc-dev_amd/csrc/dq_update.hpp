@@ -46,6 +46,7 @@ struct UpdArgs {
     double* escale; double* iscale; // [b][L*N]
     unsigned long long* n_acc;      // [c]
     const int* active;              // [c]
+    const int* Lc;                  // [c] own slice counts of padded chains (or null)
 };
 
 template <typename T>
@@ -71,6 +72,7 @@ __global__ __launch_bounds__(UPD_THREADS) void k_update(UpdArgs<T> a)
     __shared__ T s_fu, s_fd;
     const int c = blockIdx.x;
     if (!a.active[c]) return;
+    if (a.Lc && a.l >= a.Lc[c]) return;  // identity slice of a padded chain
     const int N = a.N, tid = threadIdx.x;
     T* Gu = a.G + (2 * c) * a.sG;
     T* Gd = a.G + (2 * c + 1) * a.sG;
@@ -197,10 +199,11 @@ __global__ void k_scales(int N, int L, int nc, const int* hs, const double* exp_
 }
 
 // X[b] = A[b / adiv] * diag(s[b]) (column scaling); used for the first B_l of
-// a product chain.
+// a product chain.  The identity instead for an identity slice (l >=
+// Lc[b / 2], chains padded to a common L; Lc null: none).
 template <typename T>
 __global__ void k_colscale(int N, const T* A, long sA, int adiv, const double* s, long ss, T* X,
-                           long sX, int batch)
+                           long sX, int batch, const int* Lc = nullptr, int l = 0)
 {
     const long NN = (long)N * N;
     const long total = NN * batch;
@@ -209,7 +212,10 @@ __global__ void k_colscale(int N, const T* A, long sA, int adiv, const double* s
         const int b = e / NN;
         const long r = e % NN;
         const int j = r / N;
-        X[b * sX + r] = A[(b / adiv) * sA + r] * s[b * ss + j];
+        if (Lc && l >= Lc[b >> 1])
+            X[b * sX + r] = from_re<T>((r % N) == j ? 1.0 : 0.0);
+        else
+            X[b * sX + r] = A[(b / adiv) * sA + r] * s[b * ss + j];
     }
 }
 

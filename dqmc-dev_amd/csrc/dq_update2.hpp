@@ -32,6 +32,7 @@ struct Upd2Args {
     double* escale; double* iscale; // [b][L*N]
     unsigned long long* n_acc;      // [c]
     const int* active;              // [c]
+    const int* Lc;                  // [c] own slice counts of padded chains (or null)
 };
 
 // G += A(:, 0:KD) B(:, 0:KD)^T for the rows this thread owns, d = diag(G).
@@ -221,6 +222,7 @@ __global__ __launch_bounds__(UPD2_THREADS) void k_update_reg(Upd2Args<T> a)
     __shared__ T s_fu, s_fd;
     const int c = blockIdx.x;
     if (!a.active[c]) return;
+    if (a.Lc && a.l >= a.Lc[c]) return;  // identity slice of a padded chain
     const int N = a.N, tid = threadIdx.x;
     T* Gu = a.G + (2 * c) * a.sG;
     T* Gd = a.G + (2 * c + 1) * a.sG;

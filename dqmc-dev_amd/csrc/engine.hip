@@ -192,6 +192,11 @@ struct dqmc_engine {
     double* adjL = 0;
     int* invij = 0;      // [c][j][num_ij] row of column j in bin r (when ij_percol)
     bool ij_percol = false;
+    // chains of uneven size (padded to N, L): own slice / site counts
+    bool ragged = false;
+    std::vector<int> Lc, Nc;
+    int *d_Lc = 0, *d_Nc = 0;
+    int* flag_sl = 0;    // [c] per-slice equal-time measurement flags
     // blocked QR workspace: explicit V, compact-WY T factors, GEMM scratch, dets
     void *Vb = 0, *Tb = 0, *Wq1 = 0, *Wq2 = 0, *detq = 0, *detm = 0;
     long sVb = 0, sTb = 0, sWq = 0;
@@ -237,9 +242,14 @@ template <typename T>
 static int gemm(dqmc_engine* e, int opa, int opb, int M, int N, int K, T alpha, const T* A, int lda,
                 long sA, const T* B, int ldb, long sB, T beta, T* C, int ldc, long sC, int batch,
                 const double* r = nullptr, long sr = 0, const double* u = nullptr, long su = 0,
-                const double* w = nullptr, long sw = 0)
+                const double* w = nullptr, long sw = 0, int id_l = -1, int id_side = 0)
 {
     GemmArgs<T> g;
+    // B_l of an identity slice (l >= the chain's own L): the product by it is
+    // a copy of the other operand (batch entry b belongs to chain b / 2)
+    g.idLc = (e->ragged && id_l >= 0) ? e->d_Lc : nullptr;
+    g.idl = id_l;
+    g.idside = id_side;
     g.M = M; g.N = N; g.K = K;
     g.A = A; g.lda = lda; g.sA = sA;
     g.B = B; g.ldb = ldb; g.sB = sB;
@@ -300,12 +310,13 @@ static int build_C(dqmc_engine* e, int f0, int nf)
         int cur = ((n - 1) % 2 == 0) ? 0 : 1;  // the last product lands in C_f
         k_colscale<T><<<nblk(NN * B), 256, 0, e->st>>>(N, dp<T>(e->expK), NN, 1,
                                                       e->escale + (long)N * l0, LN, bufs[cur],
-                                                      (long)F * NN, B);
+                                                      (long)F * NN, B, e->ragged ? e->d_Lc : nullptr,
+                                                      l0);
         LCHK();
         for (int s = 1; s < n; s++) {
             RC(gemm<T>(e, OP_N, OP_N, N, N, N, one, dp<T>(e->expK), N, NN, bufs[cur], N,
                        (long)F * NN, zero, bufs[1 - cur], N, (long)F * NN, B, nullptr, 0,
-                       e->escale + (long)N * (l0 + s), LN));
+                       e->escale + (long)N * (l0 + s), LN, nullptr, 0, l0 + s, 1));
             cur = 1 - cur;
         }
     }
@@ -616,6 +627,13 @@ __global__ void k_sweep_flags(int nc, int period_uneqlt, const int* sweep, const
     flag_ue[c] = act && s >= warm[c] && period_uneqlt > 0 && (s % period_uneqlt) == 0;
 }
 
+// equal-time measurement flags of slice l for chains padded to a common L
+__global__ void k_flag_slice(int nc, const int* flag_eq, const int* Lc, int l, int* out)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < nc) out[c] = flag_eq[c] && l < Lc[c];
+}
+
 __global__ void k_sweep_advance(int nc, int* sweep, const int* active)
 {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
@@ -781,6 +799,7 @@ static int launch_update(dqmc_engine* e, int l)
         a.order = e->order; a.unif = e->unif;
         a.phase = dp<T>(e->phase); a.escale = e->escale; a.iscale = e->iscale;
         a.n_acc = e->nacc; a.active = e->active;
+        a.Lc = e->ragged ? e->d_Lc : nullptr;
         constexpr bool Z = is_cplx<T>::value;
         if (N <= 256) k_update_reg<T, 1, Z ? 8 : 16><<<nc, UPD2_THREADS, 0, e->st>>>(a);
         else k_update_reg<T, 2, Z ? 4 : 8><<<nc, UPD2_THREADS, 0, e->st>>>(a);
@@ -795,6 +814,7 @@ static int launch_update(dqmc_engine* e, int l)
     ua.order = e->order; ua.unif = e->unif;
     ua.phase = dp<T>(e->phase); ua.escale = e->escale; ua.iscale = e->iscale;
     ua.n_acc = e->nacc; ua.active = e->active;
+    ua.Lc = e->ragged ? e->d_Lc : nullptr;
     k_update<T><<<nc, UPD_THREADS, 0, e->st>>>(ua);
     LCHK();
     return 0;
@@ -828,10 +848,11 @@ static int run_t(dqmc_engine* e, int n_sweeps, const volatile int* stop)
                                                             e->flag_eq, e->flag_ue);
         LCHK();
         // the sweep's random stream: site orders and uniforms (dq_rng.hpp)
-        k_rng_raw<<<nc, 64, 0, e->st>>>(N, L, e->draws, e->rng, e->raw, 2L * L * N, e->active);
+        k_rng_raw<<<nc, 64, 0, e->st>>>(N, L, e->draws, e->rng, e->raw, 2L * L * N, e->active,
+                                        e->d_Lc, e->d_Nc);
         LCHK();
         k_rng_orders<<<nc, 256, 0, e->st>>>(N, L, e->raw, 2L * L * N, e->del, e->draws, e->order,
-                                            e->unif, e->active);
+                                            e->unif, e->active, e->d_Lc, e->d_Nc);
         LCHK();
         for (int l = 0; l < L; l++) {
             RC(launch_update<T>(e, l));
@@ -846,9 +867,9 @@ static int run_t(dqmc_engine* e, int n_sweeps, const volatile int* stop)
             } else {
                 // G <- B_l G B_l^-1 = expK diag(e_l) [G diag(e'_l) invexpK]  (src/dqmc.c:294-295)
                 RC(gemm<T>(e, OP_N, OP_N, N, N, N, one, G, N, NN, dp<T>(e->iexpK), N, NN, zero, T1,
-                           N, NN, B, nullptr, 0, e->iscale + (long)N * l, LN));
+                           N, NN, B, nullptr, 0, e->iscale + (long)N * l, LN, nullptr, 0, l, 2));
                 RC(gemm<T>(e, OP_N, OP_N, N, N, N, one, dp<T>(e->expK), N, NN, T1, N, NN, zero, G,
-                           N, NN, B, nullptr, 0, e->escale + (long)N * l, LN));
+                           N, NN, B, nullptr, 0, e->escale + (long)N * l, LN, nullptr, 0, l, 1));
             }
             if (any_eq && (l + 1) % e->period_eqlt == 0) {
                 // half-wrap e^{dtK/2} G e^{-dtK/2} (src/dqmc.c:364-380), into Y2
@@ -859,6 +880,12 @@ static int run_t(dqmc_engine* e, int n_sweeps, const volatile int* stop)
                 MeasArgs<T> ma = meas_args<T>(e, 0);
                 ma.G = dp<T>(e->Y2);
                 ma.sG = NN;
+                if (e->ragged) {  // chains past their own L do not measure
+                    k_flag_slice<<<(nc + 255) / 256, 256, 0, e->st>>>(nc, e->flag_eq, e->d_Lc, l,
+                                                                       e->flag_sl);
+                    LCHK();
+                    ma.flag = e->flag_sl;
+                }
                 const int nb = (std::max(e->num_i, e->num_ij) + 255) / 256;
                 k_meas_eq<T><<<dim3(nb, nc), 256, 0, e->st>>>(ma);
                 LCHK();
@@ -1132,6 +1159,13 @@ static int engine_setup(dqmc_engine* e, const dqmc_params* const* P)
         nd[c] = n;
     }
     HIPCHK(hipMemcpy(e->draws, nd.data(), nc * 4, hipMemcpyHostToDevice));
+    if (e->ragged) {
+        DALLOC(e->d_Lc, nc * 4);
+        DALLOC(e->d_Nc, nc * 4);
+        DALLOC(e->flag_sl, nc * 4);
+        HIPCHK(hipMemcpy(e->d_Lc, e->Lc.data(), nc * 4, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(e->d_Nc, e->Nc.data(), nc * 4, hipMemcpyHostToDevice));
+    }
     HIPCHK(hipMemcpy(e->d_nsweep, e->n_sweep.data(), nc * 4, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(e->d_warm, e->n_sweep_warm.data(), nc * 4, hipMemcpyHostToDevice));
     HIPCHK(hipStreamSynchronize(e->st));
@@ -1180,6 +1214,7 @@ extern "C" int dqmc_engine_adjoint(dqmc_engine* e, double* logdet, double* W, do
                                    double* sG)
 {
     if (!e) return set_err(DQMC_ERR_ARG, "null engine");
+    if (e->ragged) return set_err(DQMC_ERR_UNSUPPORTED, "the adjoint needs chains of equal size");
     HIPCHK(hipSetDevice(e->device));
     return e->cplx ? adjoint_t<zd>(e, logdet, W, trG, sG) : adjoint_t<double>(e, logdet, W, trG, sG);
 }
@@ -1228,6 +1263,27 @@ extern "C" int dqmc_engine_create(dqmc_engine** out, const dqmc_params* const* P
     e->kd = (tune && tune->delay_rank > 0) ? tune->delay_rank : (p0->n_delay > 0 ? p0->n_delay : 16);
     if (e->kd > e->N) e->kd = e->N;
     e->time_gemm = tune && (tune->flags & 1);
+    if (tune && (tune->chain_L || tune->chain_N)) {
+        // chains of uneven size, padded by the caller to N and L
+        e->ragged = true;
+        e->Lc.resize(n_chains);
+        e->Nc.resize(n_chains);
+        for (int c = 0; c < n_chains; c++) {
+            e->Lc[c] = tune->chain_L ? tune->chain_L[c] : e->L;
+            e->Nc[c] = tune->chain_N ? tune->chain_N[c] : e->N;
+            if (e->Lc[c] < 1 || e->Lc[c] > e->L || e->Nc[c] < 1 || e->Nc[c] > e->N) {
+                const int lc = e->Lc[c], ncs = e->Nc[c];
+                delete e;
+                return set_err(DQMC_ERR_ARG, "chain %d: own size L=%d N=%d outside 1..%d, 1..%d", c,
+                               lc, ncs, p0->L, p0->N);
+            }
+        }
+        if (e->period_uneqlt > 0) {
+            delete e;
+            return set_err(DQMC_ERR_UNSUPPORTED,
+                           "unequal-time measurements need chains of equal size (chain_L/chain_N)");
+        }
+    }
     e->sweep.assign(n_chains, 0);
     e->n_sweep.assign(n_chains, 0);
     e->n_sweep_warm.assign(n_chains, 0);

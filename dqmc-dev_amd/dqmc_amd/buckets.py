@@ -1,0 +1,156 @@
+"""Chains of uneven size in one engine: padding to a size bucket.
+
+No counterpart in the reference, which runs every chain as its own process
+(``src/main_stack.c``); one engine per distinct (N, L) would leave the GPU
+idle behind single-chain launches.  Here the chains of a bucket share one
+engine of the bucket's N and L:
+
+* sites ``>= N_c`` of chain c are inert: their blocks of every exponential are
+  the identity, ``exp_lambda = 1`` and ``del = 0`` (never flipped, no random
+  draw), so G keeps the chain's own block and a constant ``1/2`` block;
+* slices ``>= L_c`` are identity slices: no update, no wrap, no measurement
+  (the engine skips them, ``dqmc_tuning.chain_L``); the chain's groups of
+  ``n_matmul`` slices follow the engine's, so its last group may be partial;
+* the padding sites map to one extra measurement bin (and bins up to the
+  largest chain's count), all dropped again by :meth:`PaddedBatch.meas`.
+
+A chain whose L is a multiple of the engine's ``n_matmul`` and whose
+``period_eqlt`` equals it runs exactly as it would alone (same random
+stream, recalculation points and measurement slices).  Supported: the
+default equal-time measurements; the optional groups and unequal-time
+measurements need equal sizes.
+"""
+import numpy as np
+
+from . import engine as _eng
+
+_NUM_I = ("density", "density_u", "density_d", "double_occ")
+_NUM_IJ = ("g00", "g00_u", "g00_d", "nn", "xx", "zz", "pair_sw")
+_EXPS = ("exp_Ku", "exp_Kd", "inv_exp_Ku", "inv_exp_Kd", "exp_halfKu", "exp_halfKd",
+         "inv_exp_halfKu", "inv_exp_halfKd")
+_TOGGLES = ("meas_bond_corr", "meas_thermal", "meas_2bond_corr", "meas_energy_corr",
+            "meas_nematic_corr", "meas_chiral", "meas_local_JQ", "meas_pair_bb_only",
+            "meas_gen_suscept")
+_OPTIONAL_COUNTS = ("num_b", "num_bb", "num_bs", "num_b2", "num_bb2", "num_b2b", "num_b2b2",
+                    "num_plaq", "num_plaq_accum", "num_b_accum", "num_b2_accum")
+
+
+def _int(x):
+    return int(np.asarray(x).reshape(()))
+
+
+def _block(a, n_pad, fill_diag):
+    a = np.asarray(a)
+    out = np.zeros((n_pad, n_pad), dtype=a.dtype)
+    n = a.shape[0]
+    out[:n, :n] = a
+    if fill_diag is not None:
+        out[np.arange(n, n_pad), np.arange(n, n_pad)] = fill_diag
+    return out
+
+
+def pad_sim(sim, N_pad, L_pad, n_matmul, num_i_pad, num_ij_pad):
+    """Padded copy of a simulation dict (params, state, meas_eqlt)."""
+    p = sim["params"]
+    N, L = _int(p["N"]), _int(p["L"])
+    if _int(p.get("period_uneqlt", 0)) > 0:
+        raise ValueError("unequal-time measurements need chains of equal size")
+    for t in _TOGGLES:
+        if _int(p.get(t, 0)):
+            raise ValueError("%s needs chains of equal size" % t)
+    if N > N_pad or L > L_pad or L_pad % n_matmul:
+        raise ValueError("bad padding N=%d->%d L=%d->%d n_matmul=%d" % (N, N_pad, L, L_pad, n_matmul))
+    q = dict(p)
+    for k in _EXPS:
+        q[k] = _block(p[k], N_pad, 1.0)
+    for k in ("peierlsu", "peierlsd"):
+        q[k] = _block(p[k], N_pad, None)
+        q[k][N:, :] = 1.0
+        q[k][:, N:] = 1.0
+    el = np.ones((2, N_pad))
+    el[:, :N] = np.asarray(p["exp_lambda"]).reshape(2, N)
+    dl = np.zeros((2, N_pad))
+    dl[:, :N] = np.asarray(p["del"]).reshape(2, N)
+    q["exp_lambda"], q["del"] = el, dl
+    trash_i, trash_ij = num_i_pad - 1, num_ij_pad - 1
+    mi = np.full(N_pad, trash_i, dtype=np.int32)
+    mi[:N] = np.asarray(p["map_i"]).reshape(N)
+    mij = np.full((N_pad, N_pad), trash_ij, dtype=np.int32)
+    mij[:N, :N] = np.asarray(p["map_ij"]).reshape(N, N)
+    q["map_i"], q["map_ij"] = mi, mij
+    q.update(N=np.int32(N_pad), L=np.int32(L_pad), F=np.int32(L_pad // n_matmul),
+             n_matmul=np.int32(n_matmul), period_eqlt=np.int32(n_matmul),
+             num_i=np.int32(num_i_pad), num_ij=np.int32(num_ij_pad))
+    for k in _OPTIONAL_COUNTS:
+        q[k] = np.int32(0)
+    st = dict(sim["state"])
+    hs = np.zeros((L_pad, N_pad), dtype=np.int32)
+    hs[:L, :N] = np.asarray(sim["state"]["hs"]).reshape(L, N)
+    st["hs"] = hs
+    meq = {}
+    for k, v in sim["meas_eqlt"].items():
+        v = np.asarray(v)
+        if k in _NUM_I or k in _NUM_IJ:
+            out = np.zeros(num_i_pad if k in _NUM_I else num_ij_pad, dtype=v.dtype)
+            out[:v.shape[0]] = v
+            meq[k] = out
+        else:
+            meq[k] = v.copy()
+    return {"params": q, "state": st, "meas_eqlt": meq, "meas_uneqlt": {},
+            "metadata": sim.get("metadata", {})}
+
+
+class PaddedBatch:
+    """One engine for chains of uneven N and L (see the module docstring).
+
+    sims: simulation dicts of one dtype; n_matmul: the engine's group size
+    (the last group of a chain whose L it does not divide is partial)."""
+
+    def __init__(self, sims, n_matmul, device=0, gemm_tile=0):
+        self.sims = sims
+        self.Nc = [_int(s["params"]["N"]) for s in sims]
+        self.Lc = [_int(s["params"]["L"]) for s in sims]
+        self.N = max(self.Nc)
+        self.L = max(-(-l // n_matmul) * n_matmul for l in self.Lc)
+        self.n_matmul = n_matmul
+        self.num_i = [_int(s["params"]["num_i"]) for s in sims]
+        self.num_ij = [_int(s["params"]["num_ij"]) for s in sims]
+        ni, nij = max(self.num_i) + 1, max(self.num_ij) + 1
+        self.padded = [pad_sim(s, self.N, self.L, n_matmul, ni, nij) for s in sims]
+        self.engine = _eng.Engine(self.padded, device=device, gemm_tile=gemm_tile,
+                                  chain_L=self.Lc, chain_N=self.Nc)
+
+    @property
+    def nc(self):
+        return len(self.sims)
+
+    def run(self, n_sweeps, stop=None):
+        return self.engine.run(n_sweeps, stop)
+
+    def close(self):
+        self.engine.close()
+
+    def state(self, c):
+        """Chain c's state in its own shape."""
+        st = self.engine.get_state(c)
+        st["hs"] = st["hs"][:self.Lc[c], :self.Nc[c]].copy()
+        return st
+
+    def meas(self, c):
+        """Chain c's equal-time accumulators in its own bins."""
+        eq, _ = self.engine.get_meas(c)
+        out = {}
+        for k, v in eq.items():
+            if k in _NUM_I:
+                out[k] = v[:self.num_i[c]].copy()
+            elif k in _NUM_IJ:
+                out[k] = v[:self.num_ij[c]].copy()
+            else:
+                out[k] = v
+        return out
+
+    def green(self, c, spin):
+        """Chain c's current equal-time G (its own block) and phase."""
+        G, ph = self.engine.green(c, spin)
+        n = self.Nc[c]
+        return G[:n, :n].copy(), ph

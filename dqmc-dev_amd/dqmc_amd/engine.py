@@ -18,7 +18,8 @@ _lib = None
 
 
 class DqmcTuning(C.Structure):
-    _fields_ = [("delay_rank", C.c_int), ("flags", C.c_int), ("gemm_tile", C.c_int)]
+    _fields_ = [("delay_rank", C.c_int), ("flags", C.c_int), ("gemm_tile", C.c_int),
+                ("chain_L", C.POINTER(C.c_int)), ("chain_N", C.POINTER(C.c_int))]
 
 
 def lib():
@@ -81,7 +82,8 @@ class Engine:
     "meas_uneqlt"}) -- all with the same N, L, n_matmul, dtype and
     measurement set.  State and accumulators are uploaded at construction."""
 
-    def __init__(self, sims, device=0, delay_rank=0, time_gemm=False, gemm_tile=0):
+    def __init__(self, sims, device=0, delay_rank=0, time_gemm=False, gemm_tile=0, chain_L=None,
+                 chain_N=None):
         self.L = lib()
         self.sims = sims
         self.cplx = capi.is_complex(sims[0])
@@ -95,7 +97,12 @@ class Engine:
             structs.append(C.pointer(ps))
         arr = (C.POINTER(capi.DqmcParams) * len(sims))(*structs)
         self._arr = arr
-        tune = DqmcTuning(delay_rank, 1 if time_gemm else 0, gemm_tile)
+        # own sizes of chains padded to the common N, L (dqmc_amd.buckets)
+        self._cl = None if chain_L is None else (C.c_int * len(sims))(*chain_L)
+        self._cn = None if chain_N is None else (C.c_int * len(sims))(*chain_N)
+        tune = DqmcTuning(delay_rank, 1 if time_gemm else 0, gemm_tile,
+                          C.cast(self._cl, C.POINTER(C.c_int)) if self._cl is not None else None,
+                          C.cast(self._cn, C.POINTER(C.c_int)) if self._cn is not None else None)
         h = C.c_void_p()
         _check(self.L.dqmc_engine_create(C.byref(h), arr, len(sims), device, C.byref(tune)),
                "dqmc_engine_create")

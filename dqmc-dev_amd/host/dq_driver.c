@@ -124,7 +124,7 @@ static void *run_group(void *arg)
     const dqmc_params **P = calloc(n, sizeof *P);
     for (int c = 0; c < n; c++) P[c] = &g->j[c]->sim.p;
     dqmc_engine *e = NULL;
-    dqmc_tuning tune = {0, 0, 0};
+    dqmc_tuning tune = {0, 0, 0, NULL, NULL};
     int rc = dqmc_engine_create(&e, P, n, 0, &tune);
     free(P);
     if (rc) {

@@ -106,7 +106,8 @@ long dqmc_acc_len(const dqmc_params *p, int ue, int k);
 /* ---------------------------------------------------------------------- */
 /* Engine: a batch of independent Markov chains of one shape (N, L, F,
  * n_matmul, dtype and measurement set equal; all physical parameters may
- * differ per chain).  Chains with other shapes go into other engines.        */
+ * differ per chain).  Chains of uneven size share an engine by padding (see
+ * chain_L / chain_N below); otherwise other shapes go into other engines.    */
 typedef struct dqmc_engine dqmc_engine;
 
 /* Tuning knobs picked by the autotuner; zero fields mean "engine default". */
@@ -115,6 +116,18 @@ typedef struct dqmc_tuning {
     int flags;      /* bit 0: time the N x N x N GEMMs with HIP events */
     int gemm_tile;  /* C tile of the MFMA GEMM: 0 = time the shapes at creation
                        and keep the fastest, 1 = 64x64, 2 = 128x128, 3 = 128x64 */
+    /* Chains of uneven size in one engine (both NULL: every chain has the
+     * params' N and L).  chain_L[c] <= L and chain_N[c] <= N are chain c's own
+     * slice and site counts; its params are padded to the common N and L by
+     * the caller: sites >= chain_N[c] carry the identity (exp_K blocks = I,
+     * exp_lambda = 1, del = 0) and map to measurement bins the caller drops,
+     * and slices >= chain_L[c] are identity slices (no update, no wrap, no
+     * measurement; the chain's last group of n_matmul slices may be partial).
+     * Its random stream advances exactly as for a chain of its own size.
+     * Unequal-time measurements and the adjoint need equal sizes.
+     * No counterpart in the reference, which runs one chain per process. */
+    const int *chain_L;
+    const int *chain_N;
 } dqmc_tuning;
 
 /* Create an engine for n_chains chains on HIP device `device`.  params[c]

@@ -1,0 +1,83 @@
+"""GPU: chains of uneven size padded into one engine (dqmc_amd.buckets) run as
+they run alone.
+
+A chain whose L is a multiple of the engine's n_matmul (with period_eqlt
+equal to it) has the same random stream, recalculation points and
+measurement slices as in an engine of its own: its HS field and RNG state
+must match bit for bit, G and the equal-time accumulators to 1e-10 relative.
+A chain whose last group is partial (L = 23 in groups of 10) is compared with
+its own engine at n_matmul = 1: the same Markov chain, stabilised at other
+slices, so the fields agree and G agrees to 1e-9.
+"""
+import numpy as np
+import pytest
+
+from dqmc_amd import buckets, engine, gen
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return np.abs(np.asarray(a) - np.asarray(b)).max() / max(np.abs(np.asarray(b)).max(), 1e-300)
+
+
+def _sim(side, L, nm, seed, U=4.0, mu=-0.5):
+    return gen.create_batch(Nfiles=1, seed=seed, Nx=side, Ny=side, L=L, n_matmul=nm,
+                            period_eqlt=nm, U=U, mu=mu, tp=0.0, n_sweep_warm=1,
+                            n_sweep_meas=100)[0]
+
+
+def _alone(sim, sweeps):
+    e = engine.Engine([sim])
+    e.run(sweeps)
+    out = (e.get_state(0), e.get_meas(0)[0], e.green(0, 0), e.green(0, 1), e.accepts())
+    e.close()
+    return out
+
+
+def test_padded_chains_match_their_own_engines():
+    shapes = [(4, 20, 10, 11, 4.0), (6, 30, 10, 12, 8.0), (5, 10, 10, 13, 2.0), (6, 20, 10, 14, 4.0)]
+    sims = [_sim(s, L, nm, seed, U=U) for s, L, nm, seed, U in shapes]
+    sweeps = 4
+    pb = buckets.PaddedBatch(sims, 10)
+    assert (pb.N, pb.L) == (36, 30)
+    pb.run(sweeps)
+    for c, sim in enumerate(sims):
+        st, meq, (gu, pu), (gd, pd), _ = _alone(sim, sweeps)
+        s2 = pb.state(c)
+        print("chain", c, "shape", shapes[c][:2], flush=True)
+        assert int(s2["sweep"]) == int(st["sweep"]) == sweeps
+        assert np.array_equal(s2["rng"], st["rng"])
+        assert np.array_equal(s2["hs"], st["hs"])
+        g2u, p2u = pb.green(c, 0)
+        g2d, p2d = pb.green(c, 1)
+        assert rel(g2u, gu) < 1e-10 and rel(g2d, gd) < 1e-10
+        assert abs(p2u - pu) < 1e-10
+        m2 = pb.meas(c)
+        assert int(m2["n_sample"]) == int(meq["n_sample"]) > 0
+        for k in ("density", "double_occ", "g00", "nn", "xx", "zz", "pair_sw"):
+            assert m2[k].shape == meq[k].shape
+            assert rel(m2[k], meq[k]) < 1e-10, k
+    pb.close()
+
+
+def test_partial_last_group():
+    a = _sim(5, 23, 1, 21)
+    b = _sim(4, 30, 10, 22)
+    sweeps = 3
+    pb = buckets.PaddedBatch([a, b], 10)
+    assert pb.L == 30
+    pb.run(sweeps)
+    st, _, (gu, _), (gd, _), _ = _alone(a, sweeps)
+    s2 = pb.state(0)
+    assert np.array_equal(s2["rng"], st["rng"])
+    assert np.array_equal(s2["hs"], st["hs"])
+    assert rel(pb.green(0, 0)[0], gu) < 1e-9 and rel(pb.green(0, 1)[0], gd) < 1e-9
+    pb.close()
+
+
+def test_uneven_sizes_refuse_unequal_time():
+    a = gen.create_batch(Nfiles=1, seed=3, Nx=4, Ny=4, L=20, n_matmul=10, period_eqlt=10,
+                         period_uneqlt=1)[0]
+    with pytest.raises(ValueError):
+        buckets.PaddedBatch([a], 10)
