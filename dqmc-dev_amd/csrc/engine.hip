@@ -924,15 +924,19 @@ static int same_shape(const dqmc_params* a, const dqmc_params* b)
            a->num_bb == b->num_bb && a->num_bs == b->num_bs;
 }
 
+// Bin lists of a site / pair map.  Entries outside [0, nbins) belong to no
+// bin (the padding sites of chains padded to a common N map to -1).
 static void csr_of_map(const int32_t* map, int n_in, int nbins, std::vector<int>& rp,
                        std::vector<int>& ix)
 {
     rp.assign(nbins + 1, 0);
-    for (int q = 0; q < n_in; q++) rp[map[q] + 1]++;
+    for (int q = 0; q < n_in; q++)
+        if (map[q] >= 0 && map[q] < nbins) rp[map[q] + 1]++;
     for (int r = 0; r < nbins; r++) rp[r + 1] += rp[r];
     std::vector<int> fill(rp.begin(), rp.end() - 1);
     ix.assign(n_in, 0);
-    for (int q = 0; q < n_in; q++) ix[fill[map[q]]++] = q;  // increasing q within a bin
+    for (int q = 0; q < n_in; q++)  // increasing q within a bin
+        if (map[q] >= 0 && map[q] < nbins) ix[fill[map[q]]++] = q;
 }
 
 static int engine_setup(dqmc_engine* e, const dqmc_params* const* P)

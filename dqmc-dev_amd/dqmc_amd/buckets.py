@@ -11,8 +11,9 @@ engine of the bucket's N and L:
 * slices ``>= L_c`` are identity slices: no update, no wrap, no measurement
   (the engine skips them, ``dqmc_tuning.chain_L``); the chain's groups of
   ``n_matmul`` slices follow the engine's, so its last group may be partial;
-* the padding sites map to one extra measurement bin (and bins up to the
-  largest chain's count), all dropped again by :meth:`PaddedBatch.meas`.
+* the padding sites and pairs map to no measurement bin (-1), and the bins
+  beyond a chain's own count (up to the largest chain's) stay empty; both
+  are dropped again by :meth:`PaddedBatch.meas`.
 
 A chain whose L is a multiple of the engine's ``n_matmul`` and whose
 ``period_eqlt`` equals it runs exactly as it would alone (same random
@@ -72,10 +73,9 @@ def pad_sim(sim, N_pad, L_pad, n_matmul, num_i_pad, num_ij_pad):
     dl = np.zeros((2, N_pad))
     dl[:, :N] = np.asarray(p["del"]).reshape(2, N)
     q["exp_lambda"], q["del"] = el, dl
-    trash_i, trash_ij = num_i_pad - 1, num_ij_pad - 1
-    mi = np.full(N_pad, trash_i, dtype=np.int32)
+    mi = np.full(N_pad, -1, dtype=np.int32)
     mi[:N] = np.asarray(p["map_i"]).reshape(N)
-    mij = np.full((N_pad, N_pad), trash_ij, dtype=np.int32)
+    mij = np.full((N_pad, N_pad), -1, dtype=np.int32)
     mij[:N, :N] = np.asarray(p["map_ij"]).reshape(N, N)
     q["map_i"], q["map_ij"] = mi, mij
     q.update(N=np.int32(N_pad), L=np.int32(L_pad), F=np.int32(L_pad // n_matmul),
@@ -115,7 +115,7 @@ class PaddedBatch:
         self.n_matmul = n_matmul
         self.num_i = [_int(s["params"]["num_i"]) for s in sims]
         self.num_ij = [_int(s["params"]["num_ij"]) for s in sims]
-        ni, nij = max(self.num_i) + 1, max(self.num_ij) + 1
+        ni, nij = max(self.num_i), max(self.num_ij)
         self.padded = [pad_sim(s, self.N, self.L, n_matmul, ni, nij) for s in sims]
         self.engine = _eng.Engine(self.padded, device=device, gemm_tile=gemm_tile,
                                   chain_L=self.Lc, chain_N=self.Nc)

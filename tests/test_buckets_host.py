@@ -1,6 +1,6 @@
 """Host side of the size buckets (dqmc_amd.buckets.pad_sim): the padded
-parameters describe the same chain plus inert sites, and the padding bins are
-the ones the engine never mixes with the chain's own."""
+parameters describe the same chain plus inert sites, and padding sites and
+pairs belong to no measurement bin."""
 import numpy as np
 import pytest
 
@@ -25,12 +25,12 @@ def test_padding_keeps_the_chain_block_and_adds_identity():
     # inert sites: no coupling to the field, never flipped
     assert np.all(pp["exp_lambda"][:, 16:] == 1.0) and np.all(pp["del"][:, 16:] == 0.0)
     assert np.array_equal(pp["exp_lambda"][:, :16], np.asarray(p["exp_lambda"]).reshape(2, 16))
-    # the chain's bins are unchanged, padding sites and pairs go to the last bin
+    # the chain's bins are unchanged, padding sites and pairs belong to none
     assert np.array_equal(pp["map_i"][:16], np.asarray(p["map_i"]).reshape(16))
-    assert np.all(pp["map_i"][16:] == 1)
+    assert np.all(pp["map_i"][16:] == -1)
     mij = pp["map_ij"]
     assert np.array_equal(mij[:16, :16], np.asarray(p["map_ij"]).reshape(16, 16))
-    assert np.all(mij[16:, :] == 16) and np.all(mij[:, 16:] == 16)
+    assert np.all(mij[16:, :] == -1) and np.all(mij[:, 16:] == -1)
     assert q["state"]["hs"].shape == (30, 25)
     assert np.array_equal(q["state"]["hs"][:20, :16], s["state"]["hs"])
     assert q["meas_eqlt"]["nn"].shape == (17,) and q["meas_eqlt"]["density"].shape == (2,)
