@@ -79,6 +79,119 @@ __device__ inline zd wave_sum(zd v)
     v.y = wave_sum(v.y);
     return v;
 }
+__device__ inline double readlane_d(double v, int l)
+{
+    const long long u = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)u, l);
+    const int hi = __builtin_amdgcn_readlane((int)(u >> 32), l);
+    return __longlong_as_double((long long)(unsigned)lo | ((long long)hi << 32));
+}
+
+// --- wave64 reductions in registers (no LDS round trips) -------------------
+// gfx950 lane exchanges:  permlane32_swap(x, y) -> ([x lanes 0-31, y lanes
+// 0-31], [x lanes 32-63, y lanes 32-63]);  permlane16_swap(x, y) ->
+// ([x row 0, y row 0, x row 2, y row 2], [x row 1, y row 1, x row 3, y row 3]);
+// DPP row_ror:n reads lane (i - n) mod 16 of the same 16-lane row.
+__device__ inline void swap32_d(double& x, double& y)
+{
+    const long long ux = __double_as_longlong(x), uy = __double_as_longlong(y);
+    const auto lo = __builtin_amdgcn_permlane32_swap((unsigned)ux, (unsigned)uy, false, false);
+    const auto hi = __builtin_amdgcn_permlane32_swap((unsigned)(ux >> 32), (unsigned)(uy >> 32),
+                                                     false, false);
+    x = __longlong_as_double(((long long)hi[0] << 32) | lo[0]);
+    y = __longlong_as_double(((long long)hi[1] << 32) | lo[1]);
+}
+__device__ inline void swap16_d(double& x, double& y)
+{
+    const long long ux = __double_as_longlong(x), uy = __double_as_longlong(y);
+    const auto lo = __builtin_amdgcn_permlane16_swap((unsigned)ux, (unsigned)uy, false, false);
+    const auto hi = __builtin_amdgcn_permlane16_swap((unsigned)(ux >> 32), (unsigned)(uy >> 32),
+                                                     false, false);
+    x = __longlong_as_double(((long long)hi[0] << 32) | lo[0]);
+    y = __longlong_as_double(((long long)hi[1] << 32) | lo[1]);
+}
+template <int CTRL>
+__device__ inline double dpp_d(double v)
+{
+    const long long u = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)u, CTRL, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(u >> 32), CTRL, 0xf, 0xf, false);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+#define DPP_ROR(n) (0x120 + (n))
+#define DPP_MIRROR 0x140
+#define DPP_HALF_MIRROR 0x141
+#define DPP_QUAD_REV 0x1B   // quad_perm [3,2,1,0]
+#define DPP_QUAD_XOR1 0xB1  // quad_perm [1,0,3,2]
+
+// Sums K in {1, 2, 4, 8} per-lane values over the wave at once, transposed:
+// each exchange halves the values a lane carries (lanes 32-63 take the upper
+// half, then odd rows, then the upper 8 lanes of a row), the rest is summed
+// within the lane groups.  Value i ends in lanes (64 / K) i ... ; on return
+// every a[i] holds its full sum, uniform over the wave.
+template <int K>
+__device__ inline void wave_sum_many(double (&a)[K], int lane)
+{
+    static_assert(K == 1 || K == 2 || K == 4 || K == 8, "K in {1, 2, 4, 8}");
+    constexpr int KA = K > 1 ? K / 2 : 1;
+    constexpr int KB = KA > 1 ? KA / 2 : 1;
+    double c1[KA], c2[KB];
+#pragma unroll
+    for (int i = 0; i < KA; i++) {
+        double x = a[i], y = a[K > 1 ? i + KA : i];
+        swap32_d(x, y);
+        c1[i] = x + y;
+    }
+#pragma unroll
+    for (int i = 0; i < KB; i++) {
+        double x = c1[i], y = c1[KA > 1 ? i + KB : i];
+        swap16_d(x, y);
+        c2[i] = x + y;
+    }
+    double cur;
+    if constexpr (K == 8) {
+        const bool up = (lane & 8) != 0;
+        const double keep = up ? c2[1] : c2[0], send = up ? c2[0] : c2[1];
+        cur = keep + dpp_d<DPP_MIRROR>(send);
+        cur += dpp_d<DPP_HALF_MIRROR>(cur);
+        cur += dpp_d<DPP_QUAD_REV>(cur);
+        cur += dpp_d<DPP_QUAD_XOR1>(cur);
+    } else {
+        cur = c2[0];
+        cur += dpp_d<DPP_ROR(8)>(cur);
+        cur += dpp_d<DPP_ROR(4)>(cur);
+        cur += dpp_d<DPP_ROR(2)>(cur);
+        cur += dpp_d<DPP_ROR(1)>(cur);
+    }
+#pragma unroll
+    for (int i = 0; i < K; i++) a[i] = readlane_d(cur, (64 / K) * i);
+}
+__device__ inline double wave_sum_r(double v, int lane)
+{
+    double a[1] = {v};
+    wave_sum_many<1>(a, lane);
+    return a[0];
+}
+__device__ inline zd wave_sum_r(zd v, int lane)
+{
+    double a[2] = {v.x, v.y};
+    wave_sum_many<2>(a, lane);
+    return mkz(a[0], a[1]);
+}
+
+// component-wise select (a conditional over whole complex values would be
+// lowered through memory)
+__device__ inline double selv(bool c, double a, double b) { return c ? a : b; }
+__device__ inline zd selv(bool c, zd a, zd b) { return mkz(c ? a.x : b.x, c ? a.y : b.y); }
+
+// doubles per value, and moving values through double arrays
+template <typename T> struct val_parts { static const int n = 1; };
+template <> struct val_parts<zd> { static const int n = 2; };
+__device__ inline void put_part(double* a, double v) { a[0] = v; }
+__device__ inline void put_part(double* a, zd v) { a[0] = v.x; a[1] = v.y; }
+__device__ inline void get_part(const double* a, double& v) { v = a[0]; }
+__device__ inline void get_part(const double* a, zd& v) { v = mkz(a[0], a[1]); }
+
 __device__ inline double shfl(double v, int src) { return __shfl(v, src, 64); }
 __device__ inline int shfl(int v, int src) { return __shfl(v, src, 64); }
 __device__ inline zd shfl(zd v, int src) { return mkz(__shfl(v.x, src, 64), __shfl(v.y, src, 64)); }

@@ -66,7 +66,7 @@ __global__ __launch_bounds__(QRB_THREADS) void k_qr_panel(int m, int k0, int nbp
                     if (r == j) alpha = x[q][cs];
                     if (r > j && r < mp) xn2 += abs2(x[q][cs]);
                 }
-                xn2 = wave_sum(xn2);
+                xn2 = wave_sum_r(xn2, lane);
                 alpha = shfl(alpha, j & 63);
                 T beta, t, scal;
                 larfg(alpha, xn2, &beta, &t, &scal);
@@ -87,14 +87,17 @@ __global__ __launch_bounds__(QRB_THREADS) void k_qr_panel(int m, int k0, int nbp
             }
             __syncthreads();
             // (b) apply H_j^H to the panel columns right of j; dots with the
-            //     reflectors left of j for the T recurrence
+            //     reflectors left of j for the T recurrence.  The NCW dots of
+            //     a wave are summed over the wave in one register reduction.
             const T tj = s_tau;
             const T ct = cj(tj);
+            constexpr int NP = val_parts<T>::n;
+            double dr[NCW * NP];
 #pragma unroll
             for (int c2 = 0; c2 < NCW; c2++) {
                 const int jc = wave + 16 * c2;
+                T d = from_re<T>(0.0);
                 if (jc < nbp && jc != j) {
-                    T d = from_re<T>(0.0);
 #pragma unroll
                     for (int q = 0; q < QM; q++) {
                         const int r = lane + 64 * q;
@@ -103,7 +106,16 @@ __global__ __launch_bounds__(QRB_THREADS) void k_qr_panel(int m, int k0, int nbp
                             else d += cj(x[q][c2]) * v[r];
                         }
                     }
-                    d = wave_sum(d);
+                }
+                put_part(&dr[c2 * NP], d);
+            }
+            wave_sum_many<NCW * NP>(dr, lane);
+#pragma unroll
+            for (int c2 = 0; c2 < NCW; c2++) {
+                const int jc = wave + 16 * c2;
+                if (jc < nbp && jc != j) {
+                    T d;
+                    get_part(&dr[c2 * NP], d);
                     if (jc > j) {
                         const T f = ct * d;
 #pragma unroll
@@ -117,8 +129,10 @@ __global__ __launch_bounds__(QRB_THREADS) void k_qr_panel(int m, int k0, int nbp
                 }
             }
             __syncthreads();
-            // (c) T(0:j, j) = -tau_j T(0:j, 0:j) z(0:j)   (wave 0; j <= 63)
-            if (wave == 0 && lane < j) {
+            // (c) T(0:j, j) = -tau_j T(0:j, 0:j) z(0:j)   (j <= 63), by the wave
+            //     that built reflector j: the owner of reflector j + 1 is
+            //     another wave, so this overlaps the next step's (a)
+            if (wave == w0 && lane < j) {
                 T s = from_re<T>(0.0);
                 for (int a = lane; a < j; a++) s += Ts[lane + NB * a] * z[a];
                 Ts[lane + NB * j] = -(tj * s);
