@@ -31,6 +31,7 @@
 #include "dq_ue.hpp"
 #include "dq_update.hpp"
 #include "dq_update2.hpp"
+#include "dq_update3.hpp"
 #include "dqmc_hip.h"
 
 static thread_local std::string g_err;
@@ -801,7 +802,12 @@ static int launch_update(dqmc_engine* e, int l)
         a.n_acc = e->nacc; a.active = e->active;
         a.Lc = e->ragged ? e->d_Lc : nullptr;
         constexpr bool Z = is_cplx<T>::value;
-        if (N <= 256) k_update_reg<T, 1, Z ? 8 : 16><<<nc, UPD2_THREADS, 0, e->st>>>(a);
+        // small N: the register-only kernel (rank 16 / 8 complex; its flushes
+        // are cheap there); up to N = 256 rank 32 / 16 with B in LDS and the
+        // flush on the matrix cores; above, the register-only kernel with rank
+        // 8 / 4
+        if (N <= 64) k_update_reg<T, 1, Z ? 8 : 16><<<nc, UPD2_THREADS, 0, e->st>>>(a);
+        else if (N <= UPD3_THREADS) k_update_lds<T, Z ? 16 : 32><<<nc, UPD3_THREADS, 0, e->st>>>(a);
         else k_update_reg<T, 2, Z ? 4 : 8><<<nc, UPD2_THREADS, 0, e->st>>>(a);
         LCHK();
         return 0;
