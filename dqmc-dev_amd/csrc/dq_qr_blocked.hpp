@@ -133,8 +133,13 @@ __global__ __launch_bounds__(QRB_THREADS) void k_qr_panel(int m, int k0, int nbp
             //     that built reflector j: the owner of reflector j + 1 is
             //     another wave, so this overlaps the next step's (a)
             if (wave == w0 && lane < j) {
+                // T is upper triangular (zeros below the diagonal), so the sum
+                // may start at 0; the bound j is uniform, and the unrolled
+                // loop issues its independent LDS reads together
                 T s = from_re<T>(0.0);
-                for (int a = lane; a < j; a++) s += Ts[lane + NB * a] * z[a];
+#pragma unroll
+                for (int a = 0; a < NB; a++)
+                    if (a < j) s += Ts[lane + NB * a] * z[a];
                 Ts[lane + NB * j] = -(tj * s);
             }
         }
