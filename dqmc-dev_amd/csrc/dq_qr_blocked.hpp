@@ -177,6 +177,7 @@ __global__ __launch_bounds__(256) void k_trsm_diag64(int nk, int ncols, const T*
                                                      int ldu, T* X_, long sX, int ldx)
 {
     __shared__ T Us[64 * 64];
+    __shared__ T Ui[64];  // 1 / U_kk, formed once instead of once per column
     const int b = blockIdx.y;
     const T* U = U_ + b * sU;
     T* X = X_ + b * sX;
@@ -185,6 +186,7 @@ __global__ __launch_bounds__(256) void k_trsm_diag64(int nk, int ncols, const T*
         const int i = e % nk, k = e / nk;
         Us[i + 64 * k] = (i <= k) ? U[i + (long)ldu * k] : from_re<T>(0.0);
     }
+    if (tid < nk) Ui[tid] = rdiv(from_re<T>(1.0), U[tid + (long)ldu * tid]);
     __syncthreads();
     const int c0 = (blockIdx.x * (blockDim.x >> 6) + wave) * LU_NC;
     if (c0 >= ncols) return;
@@ -193,12 +195,12 @@ __global__ __launch_bounds__(256) void k_trsm_diag64(int nk, int ncols, const T*
     for (int q = 0; q < LU_NC; q++)
         x[q] = (lane < nk && c0 + q < ncols) ? X[lane + (long)ldx * (c0 + q)] : from_re<T>(0.0);
     for (int k = nk - 1; k >= 0; k--) {
-        const T ukk = Us[k + 64 * k];
+        const T rkk = Ui[k];
         const T uik = Us[lane + 64 * k];
 #pragma unroll
         for (int q = 0; q < LU_NC; q++) {
             T xk = shfl(x[q], k);
-            xk = rdiv(xk, ukk);
+            xk = xk * rkk;
             if (lane == k) x[q] = xk;
             if (lane < k) x[q] -= uik * xk;
         }

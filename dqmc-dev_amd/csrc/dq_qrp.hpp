@@ -296,29 +296,30 @@ __global__ __launch_bounds__(QRP_THREADS) void k_qrp_res(int m, int n, T* A_, in
 // matrix (xlarft): on entry Tk[p] holds V_p^H V_p (NB x NB, ld NB), on exit
 // the upper-triangular T with H_{k0} ... H_{k0+nb-1} = I - V_p T V_p^H,
 //   T(j, j) = tau_j,  T(0:j, j) = -tau_j T(0:j, 0:j) (V_p^H v_j)(0:j).
-// One wave per (panel, matrix); NB <= 64.
+// One wave per (panel, matrix), NB <= 64; the panel is worked on in LDS.
 template <typename T>
 __global__ __launch_bounds__(64) void k_larft_gram(int NB, int kmax, const T* tau_, long stau,
                                                    T* Tb_, long sT)
 {
+    __shared__ T S[64 * 64];
     const int p = blockIdx.x, b = blockIdx.y, lane = threadIdx.x;
     const int k0 = p * NB;
     const int nbp = (kmax - k0 < NB) ? kmax - k0 : NB;
     const T* tau = tau_ + b * stau + k0;
     T* Tk = Tb_ + b * sT + (long)p * NB * NB;
+    for (int c = 0; c < nbp; c++) S[lane + 64 * c] = (lane < nbp) ? Tk[lane + NB * c] : from_re<T>(0.0);
+    __syncthreads();
     for (int j = 0; j < nbp; j++) {
-        const T z = (lane < j) ? Tk[lane + NB * j] : from_re<T>(0.0);
         const T tj = tau[j];
+        // column j still holds the Gram entries z(0:j); columns < j hold T
         T s = from_re<T>(0.0);
-        for (int c = 0; c < j; c++) {
-            const T zc = shfl(z, c);
-            if (lane <= c) s += Tk[lane + NB * c] * zc;
-        }
-        if (lane < nbp) {
-            T out = from_re<T>(0.0);
-            if (lane < j) out = -(tj * s);
-            else if (lane == j) out = tj;
-            Tk[lane + NB * j] = out;
-        }
+#pragma unroll
+        for (int c = 0; c < 64; c++)
+            if (c < j) s += S[lane + 64 * c] * S[c + 64 * j];  // T zero below its diagonal
+        __syncthreads();  // every lane has read z before column j is overwritten
+        S[lane + 64 * j] = (lane < j) ? -(tj * s) : (lane == j ? tj : from_re<T>(0.0));
+        __syncthreads();
     }
+    for (int c = 0; c < nbp; c++)
+        if (lane < nbp) Tk[lane + NB * c] = S[lane + 64 * c];
 }
