@@ -5,7 +5,7 @@
 // Each flush G += A B^T reads and writes both spins' G (4 N^2 values per
 // chain); with all chains flushing, that streams at HBM speed, and the number
 // of flushes falls as 1/KD.  One 256-thread workgroup per chain, both spins,
-// thread j owns row j:
+// thread t owns rows j = t + 256 r (r < R; R = 2 up to N = 512, rank 16):
 //   * its row of A (A[j][0..KD)) lives in registers -- the flush needs exactly
 //     that row, and a new column enters at a run-time slot by selects;
 //   * B lives in LDS as B[m][j] (128 KiB for both spins): the flush and the
@@ -24,9 +24,8 @@
 
 // G[j][c] += sum_m A[j][m] B[m][c] for the thread's row j (j < N), all c;
 // returns the new G[j][j].  FB columns of the row stay in flight at once.
-template <typename T, int KD>
-__device__ inline T upd3_flush(int N, int j, T* G, const T (&A)[KD], const T (*B)[UPD3_THREADS],
-                               int kused)
+template <typename T, int KD, int W>
+__device__ inline T upd3_flush(int N, int j, T* G, const T (&A)[KD], const T (*B)[W], int kused)
 {
     constexpr int FB = 8;
     T gjj = from_re<T>(0.0);
@@ -39,7 +38,7 @@ __device__ inline T upd3_flush(int N, int j, T* G, const T (&A)[KD], const T (*B
         for (int m = 0; m < KD; m++) {
             if (m < kused) {  // uniform
 #pragma unroll
-                for (int t = 0; t < FB; t++) g[t] += A[m] * B[m][(c0 + t) & (UPD3_THREADS - 1)];
+                for (int t = 0; t < FB; t++) g[t] += A[m] * B[m][(c0 + t) & (W - 1)];
             }
         }
 #pragma unroll
@@ -54,7 +53,8 @@ __device__ inline T upd3_flush(int N, int j, T* G, const T (&A)[KD], const T (*B
 }
 
 // The same flush on the FP64 matrix cores (real builds): per wave the rows
-// [64w, 64w + 64) and all columns in chunks of 32, as the transposed product
+// row0 + [64w, 64w + 64) and all columns in chunks of 32, as the transposed
+// product
 //   G^T[c][r] += sum_m B[m][c] A[r][m]
 // in 16x16x4 blocks: the B fragment (lane l: B[m0 + l/16][c0 + l%16]) is one
 // LDS read per block column, the A fragment (lane l: A[r0 + l%16][m0 + l/16])
@@ -64,9 +64,9 @@ __device__ inline T upd3_flush(int N, int j, T* G, const T (&A)[KD], const T (*B
 // (Operand layouts as in dq_gemm.hpp.)  The VALU flush above needs one LDS
 // broadcast per FMA and is bound by the LDS return path; this one needs one
 // LDS read per 64 FMAs per lane.
-template <int KD>
-__device__ inline void upd3_flush_mfma(int N, int j, double* G, const double (&A)[KD],
-                                       const double (*B)[UPD3_THREADS], int kused, double* d)
+template <int KD, int W>
+__device__ inline void upd3_flush_mfma(int N, int j, int row0, double* G, const double (&A)[KD],
+                                       const double (*B)[W], int kused, double* d)
 {
     constexpr int KB = KD / 4;
     const int lane = j & 63, w = j >> 6;
@@ -85,7 +85,7 @@ __device__ inline void upd3_flush_mfma(int N, int j, double* G, const double (&A
             af[b][kb] = v;
         }
     }
-    const int rl = 64 * w + (lane & 15);
+    const int rl = row0 + 64 * w + (lane & 15);
     constexpr int MB = 2;  // 16-column blocks per chunk
     for (int c0 = 0; c0 < N; c0 += 16 * MB) {
         dbl4 acc[MB][4];
@@ -103,7 +103,7 @@ __device__ inline void upd3_flush_mfma(int N, int j, double* G, const double (&A
             if (4 * kb < kused) {  // uniform
 #pragma unroll
                 for (int mb = 0; mb < MB; mb++) {
-                    const int cc = (c0 + 16 * mb + (lane & 15)) & (UPD3_THREADS - 1);
+                    const int cc = (c0 + 16 * mb + (lane & 15)) & (W - 1);
                     const double bf = B[4 * kb + (lane >> 4)][cc];
 #pragma unroll
                     for (int b = 0; b < 4; b++) acc[mb][b] = mfma(bf, af[b][kb], acc[mb][b]);
@@ -125,51 +125,57 @@ __device__ inline void upd3_flush_mfma(int N, int j, double* G, const double (&A
     }
 }
 
-template <typename T, int KD>
+// R rows per thread (thread t owns rows t + 256 r), N <= 256 R.
+template <typename T, int R, int KD>
 __global__ __launch_bounds__(UPD3_THREADS) void k_update_lds(Upd2Args<T> a)
 {
-    constexpr int NT = UPD3_THREADS;
-    __shared__ T du[NT], dd[NT];
-    __shared__ int hsl[NT], ordl[NT];
-    __shared__ double ul[NT], dl0[NT], dl1[NT];
-    __shared__ T Bu[KD][NT], Bd[KD][NT];
+    constexpr int NT = UPD3_THREADS, W = NT * R;
+    __shared__ T du[W], dd[W];
+    __shared__ int hsl[W], ordl[W];
+    __shared__ double ul[W], dl0[W], dl1[W];
+    __shared__ T Bu[KD][W], Bd[KD][W];
     __shared__ T rAu[KD], rAd[KD];
     __shared__ int s_site;
     __shared__ T s_fu, s_fd;
     const int c = blockIdx.x;
     if (!a.active[c]) return;
     if (a.Lc && a.l >= a.Lc[c]) return;  // identity slice of a padded chain
-    const int N = a.N, j = threadIdx.x;
-    const bool own = j < N;
+    const int N = a.N, tid = threadIdx.x;
     T* Gu = a.G + (2 * c) * a.sG;
     T* Gd = a.G + (2 * c + 1) * a.sG;
     int* hs = a.hs + (long)c * a.L * N + (long)N * a.l;
     const long ol = ((long)c * a.L + a.l) * N;
     const T zero = from_re<T>(0.0), one = from_re<T>(1.0);
-    if (own) {
-        du[j] = Gu[j + (long)N * j];
-        dd[j] = Gd[j + (long)N * j];
-        hsl[j] = hs[j];
-        ordl[j] = a.order[ol + j];
-        ul[j] = a.unif[ol + j];
-        dl0[j] = a.del[2L * N * c + j];
-        dl1[j] = a.del[2L * N * c + N + j];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const int j = tid + NT * r;
+        if (j < N) {
+            du[j] = Gu[j + (long)N * j];
+            dd[j] = Gd[j + (long)N * j];
+            hsl[j] = hs[j];
+            ordl[j] = a.order[ol + j];
+            ul[j] = a.unif[ol + j];
+            dl0[j] = a.del[2L * N * c + j];
+            dl1[j] = a.del[2L * N * c + N + j];
+        }
     }
     T phase = zero;
     unsigned long long nacc = 0;
-    if (j == 0) phase = a.phase[c];
+    if (tid == 0) phase = a.phase[c];
     int ii = 0;  // lane 0's scan position
-    T Au[KD], Ad[KD];
+    T Au[R][KD], Ad[R][KD];
     for (;;) {
 #pragma unroll
-        for (int m = 0; m < KD; m++) {
-            Au[m] = zero; Ad[m] = zero;
-            Bu[m][j] = zero; Bd[m][j] = zero;
-        }
+        for (int r = 0; r < R; r++)
+#pragma unroll
+            for (int m = 0; m < KD; m++) {
+                Au[r][m] = zero; Ad[r][m] = zero;
+                Bu[m][tid + NT * r] = zero; Bd[m][tid + NT * r] = zero;
+            }
         __syncthreads();
         int k = 0;
         for (; k < KD; k++) {
-            if (j == 0) {
+            if (tid == 0) {
                 int site = -1;
                 for (; ii < N; ii++) {
                     const int i = ordl[ii];
@@ -198,54 +204,77 @@ __global__ __launch_bounds__(UPD3_THREADS) void k_update_lds(Upd2Args<T> a)
             const int i = s_site;
             if (i < 0) break;
             // the owner of row i publishes its filled slots of A
-            if (j == i) {
+            if (tid == (i & (NT - 1))) {
 #pragma unroll
-                for (int m = 0; m < KD; m++)
-                    if (m < k) { rAu[m] = Au[m]; rAd[m] = Ad[m]; }
+                for (int r = 0; r < R; r++)
+                    if (r == i / NT) {
+#pragma unroll
+                        for (int m = 0; m < KD; m++)
+                            if (m < k) { rAu[m] = Au[r][m]; rAd[m] = Ad[r][m]; }
+                    }
             }
-            T au = zero, bu = zero, ad = zero, bd = zero;
-            if (own) {
-                au = Gu[j + (long)N * i];
-                bu = Gu[i + (long)N * j];
-                ad = Gd[j + (long)N * i];
-                bd = Gd[i + (long)N * j];
+            T au[R], bu[R], ad[R], bd[R];
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                const int j = tid + NT * r;
+                au[r] = zero; bu[r] = zero; ad[r] = zero; bd[r] = zero;
+                if (j < N) {
+                    au[r] = Gu[j + (long)N * i];
+                    bu[r] = Gu[i + (long)N * j];
+                    ad[r] = Gd[j + (long)N * i];
+                    bd[r] = Gd[i + (long)N * j];
+                }
             }
             __syncthreads();
             // a_k = G[:, i] + A B[i, :]^T,  b_k = G[i, :]^T + B A[i, :]^T
 #pragma unroll
             for (int m = 0; m < KD; m++) {
                 if (m < k) {  // uniform
-                    au += Au[m] * Bu[m][i];
-                    bu += Bu[m][j] * rAu[m];
-                    ad += Ad[m] * Bd[m][i];
-                    bd += Bd[m][j] * rAd[m];
+                    const T bui = Bu[m][i], bdi = Bd[m][i], rau = rAu[m], rad = rAd[m];
+#pragma unroll
+                    for (int r = 0; r < R; r++) {
+                        const int j = tid + NT * r;
+                        au[r] += Au[r][m] * bui;
+                        bu[r] += Bu[m][j] * rau;
+                        ad[r] += Ad[r][m] * bdi;
+                        bd[r] += Bd[m][j] * rad;
+                    }
                 }
             }
-            if (j == i) { au -= one; ad -= one; }
-            au = au * s_fu;
-            ad = ad * s_fd;
-            if (own) {
-                du[j] += au * bu;
-                dd[j] += ad * bd;
-            } else {
-                au = zero; bu = zero; ad = zero; bd = zero;
-            }
+            const T fu = s_fu, fd = s_fd;
 #pragma unroll
-            for (int m = 0; m < KD; m++)
-                if (m == k) { Au[m] = au; Ad[m] = ad; }
-            Bu[k][j] = bu;
-            Bd[k][j] = bd;
+            for (int r = 0; r < R; r++) {
+                const int j = tid + NT * r;
+                if (j == i) { au[r] -= one; ad[r] -= one; }
+                au[r] = au[r] * fu;
+                ad[r] = ad[r] * fd;
+                if (j < N) {
+                    du[j] += au[r] * bu[r];
+                    dd[j] += ad[r] * bd[r];
+                } else {
+                    au[r] = zero; bu[r] = zero; ad[r] = zero; bd[r] = zero;
+                }
+#pragma unroll
+                for (int m = 0; m < KD; m++)
+                    if (m == k) { Au[r][m] = au[r]; Ad[r][m] = ad[r]; }
+                Bu[k][j] = bu[r];
+                Bd[k][j] = bd[r];
+            }
             __syncthreads();
         }
         if (k > 0) {
-            if constexpr (is_cplx<T>::value) {
-                if (own) {
-                    du[j] = upd3_flush<T, KD>(N, j, Gu, Au, Bu, k);
-                    dd[j] = upd3_flush<T, KD>(N, j, Gd, Ad, Bd, k);
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                const int j = tid + NT * r;
+                if constexpr (is_cplx<T>::value) {
+                    if (j < N) {
+                        du[j] = upd3_flush<T, KD, W>(N, j, Gu, Au[r], Bu, k);
+                        dd[j] = upd3_flush<T, KD, W>(N, j, Gd, Ad[r], Bd, k);
+                    }
+                } else {  // every lane takes part (the A fragments are gathered)
+                    upd3_flush_mfma<KD, W>(N, tid, NT * r, Gu, Au[r], Bu, k, du);
+                    upd3_flush_mfma<KD, W>(N, tid, NT * r, Gd, Ad[r], Bd, k, dd);
                 }
-            } else {  // every lane takes part (the A fragments are gathered)
-                upd3_flush_mfma<KD>(N, j, Gu, Au, Bu, k, du);
-                upd3_flush_mfma<KD>(N, j, Gd, Ad, Bd, k, dd);
             }
             __syncthreads();
         }
@@ -257,15 +286,19 @@ __global__ __launch_bounds__(UPD3_THREADS) void k_update_lds(Upd2Args<T> a)
     double* esd = a.escale + (2 * c + 1) * (long)a.L * N + (long)N * a.l;
     double* isu = a.iscale + (2 * c) * (long)a.L * N + (long)N * a.l;
     double* isd = a.iscale + (2 * c + 1) * (long)a.L * N + (long)N * a.l;
-    if (own) {
-        const int sv = hsl[j];
-        hs[j] = sv;
-        esu[j] = el[j + N * sv];
-        esd[j] = el[j + N * (1 - sv)];
-        isu[j] = el[j + N * (1 - sv)];
-        isd[j] = el[j + N * sv];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const int j = tid + NT * r;
+        if (j < N) {
+            const int sv = hsl[j];
+            hs[j] = sv;
+            esu[j] = el[j + N * sv];
+            esd[j] = el[j + N * (1 - sv)];
+            isu[j] = el[j + N * (1 - sv)];
+            isd[j] = el[j + N * sv];
+        }
     }
-    if (j == 0) {
+    if (tid == 0) {
         a.phase[c] = phase;
         a.n_acc[c] += nacc;
     }

@@ -128,6 +128,29 @@ def test_trajectory_and_measurements(nflux):
         eng.close()
 
 
+# one case per Metropolis kernel: N <= 64 registers; 64 < N <= 256 and
+# 256 < N <= 512 LDS-resident B with the matrix-core flush (complex: VALU
+# flush, and registers above 256)
+@pytest.mark.parametrize("nflux", [0, 1])
+@pytest.mark.parametrize("side", [10, 17])
+def test_trajectory_per_update_kernel(nflux, side):
+    sim = gen.create_batch(seed=31, Nx=side, Ny=side, nflux=nflux, U=6.0, mu=-0.8, tp=-0.2,
+                           L=10, n_matmul=5, period_eqlt=5, n_sweep_warm=1, n_sweep_meas=2)[0]
+    print("side", side, "nflux", nflux, flush=True)
+    eng = engine.Engine([sim])
+    try:
+        eng.run(3)
+        st = eng.get_state(0)
+        out = oracle.run(sim)
+        assert np.array_equal(st["hs"], out["state"]["hs"])
+        assert np.array_equal(st["rng"], out["state"]["rng"])
+        eq, _ = eng.get_meas(0)
+        for k in ("density", "double_occ", "g00", "nn", "zz"):
+            assert rel(eq[k], out["meas_eqlt"][k]) < 1e-9, k
+    finally:
+        eng.close()
+
+
 @pytest.mark.parametrize("nflux", [0, 1])
 @pytest.mark.parametrize("Lnm", [(20, 10), (40, 10), (40, 8), (80, 10)])
 def test_unequal_time(nflux, Lnm):
