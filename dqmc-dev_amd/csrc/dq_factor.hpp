@@ -73,7 +73,7 @@ __global__ __launch_bounds__(QR_THREADS) void k_qr(int m, int n, T* A_, int lda,
                 const int i = lane + 64 * r;
                 if (i < m) s += abs2(A[i + (long)lda * c]);
             }
-            s = wave_sum(s);
+            s = wave_sum_r(s, lane);
             if (lane == 0) { nrm[c] = s; piv[c] = c; }
         }
         __syncthreads();
@@ -119,7 +119,7 @@ __global__ __launch_bounds__(QR_THREADS) void k_qr(int m, int n, T* A_, int lda,
                 if (i > j && i < m) s += abs2(xv[r]);
                 if (i == j) alpha = xv[r];
             }
-            s = wave_sum(s);
+            s = wave_sum_r(s, lane);
             alpha = shfl(alpha, j & 63);
             T beta, t, scal;
             larfg(alpha, s, &beta, &t, &scal);
@@ -154,6 +154,8 @@ __global__ __launch_bounds__(QR_THREADS) void k_qr(int m, int n, T* A_, int lda,
                     x[t][r] = (c < n && i >= j && i < m) ? A[i + (long)lda * c] : from_re<T>(0.0);
                 }
             }
+            constexpr int NP = val_parts<T>::n;
+            double wr[CB * NP];
 #pragma unroll
             for (int t = 0; t < CB; t++) {
 #pragma unroll
@@ -161,8 +163,11 @@ __global__ __launch_bounds__(QR_THREADS) void k_qr(int m, int n, T* A_, int lda,
                     const int i = lane + 64 * r;
                     if (i >= j && i < m) w[t] += cj(v[i]) * x[t][r];
                 }
-                w[t] = wave_sum(w[t]);
+                put_part(&wr[t * NP], w[t]);
             }
+            wave_sum_many<CB * NP>(wr, lane);  // one in-register reduction for the batch
+#pragma unroll
+            for (int t = 0; t < CB; t++) get_part(&wr[t * NP], w[t]);
 #pragma unroll
             for (int t = 0; t < CB; t++) {
                 const int c = c0 + QR_WAVES * t;
@@ -179,7 +184,7 @@ __global__ __launch_bounds__(QR_THREADS) void k_qr(int m, int n, T* A_, int lda,
                     }
                 }
                 if (PIVOT) {
-                    sn = wave_sum(sn);
+                    sn = wave_sum_r(sn, lane);
                     if (lane == 0) nrm[c] = sn;
                 }
             }
@@ -224,7 +229,7 @@ __global__ __launch_bounds__(QR_THREADS) void k_qr(int m, int n, T* A_, int lda,
                             const int i = lane + 64 * r;
                             if (i >= j && i < m) w[u] += cj(v[i]) * x[u][r];
                         }
-                        w[u] = wave_sum(w[u]);
+                        w[u] = wave_sum_r(w[u], lane);
                     }
 #pragma unroll
                     for (int u = 0; u < CB; u++) {
@@ -242,6 +247,18 @@ __global__ __launch_bounds__(QR_THREADS) void k_qr(int m, int n, T* A_, int lda,
             __syncthreads();
         }
     }
+}
+
+// Explicit unit-lower V[b] (ld n, n x n) from reflectors stored below the
+// diagonal of A[b] (ld n), for the compact-WY formation of Q.
+template <typename T>
+__global__ void k_extract_v(int n, const T* A_, long sA, T* V_, long sV)
+{
+    const int b = blockIdx.y, c = blockIdx.x;
+    const T* A = A_ + b * sA;
+    T* V = V_ + b * sV;
+    for (int i = threadIdx.x; i < n; i += blockDim.x)
+        V[i + (long)n * c] = (i > c) ? A[i + (long)n * c] : from_re<T>(i == c ? 1.0 : 0.0);
 }
 
 // After the pivoted QR of the first product (src/greens.c:91-101):

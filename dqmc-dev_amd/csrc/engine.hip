@@ -454,6 +454,9 @@ static int qr_blocked(dqmc_engine* e, int m, int n, T* A, int lda, long sA, T* t
     return 0;
 }
 
+template <typename T>
+static int q_from_reflectors(dqmc_engine* e, int n, const T* tau, T* Q, long sQ, int batch);
+
 // Q column-pivoted QR of the first product with the matrix resident on one CU
 // (k_qrp_res), when it fits: real n <= 256, complex n <= 144.
 template <typename T>
@@ -486,8 +489,18 @@ static int qrp_resident(dqmc_engine* e, int n, T* A, long sA, T* tau, int* jpvt,
     }
 #undef QRP
     LCHK();
-    // explicit Q: compact-WY factors of 64-column panels from their Gram
-    // matrices, then the backward three-GEMM accumulation
+    return q_from_reflectors<T>(e, n, tau, Q, sQ, batch);
+}
+
+// Explicit Q = H_0 ... H_{n-1} from the unit-lower reflectors in the V
+// workspace (ld n) and tau: compact-WY factors of 64-column panels from their
+// Gram matrices (one GEMM + k_larft_gram), then the backward three-GEMM
+// accumulation.
+template <typename T>
+static int q_from_reflectors(dqmc_engine* e, int n, const T* tau, T* Q, long sQ, int batch)
+{
+    T* V = dp<T>(e->Vb);
+    const long sV = e->sVb;
     const int NB = 64;
     const int np = (n + NB - 1) / NB;
     T* Tb = dp<T>(e->Tb);
@@ -568,15 +581,21 @@ static int calc_eq_g(dqmc_engine* e, int l, T* Gout, T* phase_b)
     int m = (l + 1 + (F - 1) % n_mul) % F;
     RC(mul_seq_C<T>(e, l, m, one, X, N, NN));
     // (1) column-pivoted QR of the first product (xgeqp3)
-    if (qrp_resident_fits<T>(N))
+    if (qrp_resident_fits<T>(N)) {
         RC(qrp_resident<T>(e, N, X, NN, tau, e->jpvt, Q, NN, B));
-    else if (N <= 256)
-        k_qr<T, true, 4><<<B, QR_THREADS, 0, e->st>>>(N, N, X, N, NN, tau, N, e->jpvt, N, Q, N, NN);
-    else if (N <= 512)
-        k_qr<T, true, 8><<<B, QR_THREADS, 0, e->st>>>(N, N, X, N, NN, tau, N, e->jpvt, N, Q, N, NN);
-    else
-        k_qr<T, true, 16><<<B, QR_THREADS, 0, e->st>>>(N, N, X, N, NN, tau, N, e->jpvt, N, Q, N, NN);
-    LCHK();
+    } else {
+        // streaming pivoted QR; Q from the reflectors by compact-WY GEMMs
+        if (N <= 256)
+            k_qr<T, true, 4><<<B, QR_THREADS, 0, e->st>>>(N, N, X, N, NN, tau, N, e->jpvt, N, nullptr, N, NN);
+        else if (N <= 512)
+            k_qr<T, true, 8><<<B, QR_THREADS, 0, e->st>>>(N, N, X, N, NN, tau, N, e->jpvt, N, nullptr, N, NN);
+        else
+            k_qr<T, true, 16><<<B, QR_THREADS, 0, e->st>>>(N, N, X, N, NN, tau, N, e->jpvt, N, nullptr, N, NN);
+        LCHK();
+        k_extract_v<T><<<dim3(N, B), 256, 0, e->st>>>(N, X, NN, dp<T>(e->Vb), e->sVb);
+        LCHK();
+        RC(q_from_reflectors<T>(e, N, tau, Q, NN, B));
+    }
     k_reflector_det<T><<<B, 64, 0, e->st>>>(N, N, X, N, NN, tau, N, detq);
     LCHK();
     k_eqg_init_T<T><<<dim3(N, B), 256, 0, e->st>>>(N, X, NN, e->jpvt, N, e->dvec, N, Tm, NN);
