@@ -125,11 +125,11 @@ __device__ inline void upd3_flush_mfma(int N, int j, int row0, double* G, const 
     }
 }
 
-// R rows per thread (thread t owns rows t + 256 r), N <= 256 R.
-template <typename T, int R, int KD>
-__global__ __launch_bounds__(UPD3_THREADS) void k_update_lds(Upd2Args<T> a)
+// NT threads, R rows per thread (thread t owns rows t + NT r), N <= NT R.
+template <typename T, int R, int KD, int NT = UPD3_THREADS>
+__global__ __launch_bounds__(NT) void k_update_lds(Upd2Args<T> a)
 {
-    constexpr int NT = UPD3_THREADS, W = NT * R;
+    constexpr int W = NT * R;
     __shared__ T du[W], dd[W];
     __shared__ int hsl[W], ordl[W];
     __shared__ double ul[W], dl0[W], dl1[W];

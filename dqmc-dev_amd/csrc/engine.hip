@@ -823,11 +823,11 @@ static int launch_update(dqmc_engine* e, int l)
         constexpr bool Z = is_cplx<T>::value;
         // small N: the register-only kernel (rank 16 / 8 complex; its flushes
         // are cheap there); up to N = 256 rank 32 / 16 with B in LDS and the
-        // flush on the matrix cores; up to N = 512 the same with two rows per
-        // thread, rank 16 (real; complex keeps the register kernel, rank 4)
+        // flush on the matrix cores; up to N = 512 the same with 512 threads,
+        // rank 16 (real; complex keeps the register kernel, rank 4)
         if (N <= 64) k_update_reg<T, 1, Z ? 8 : 16><<<nc, UPD2_THREADS, 0, e->st>>>(a);
         else if (N <= UPD3_THREADS) k_update_lds<T, 1, Z ? 16 : 32><<<nc, UPD3_THREADS, 0, e->st>>>(a);
-        else if constexpr (!Z) k_update_lds<T, 2, 16><<<nc, UPD3_THREADS, 0, e->st>>>(a);
+        else if constexpr (!Z) k_update_lds<T, 1, 16, 512><<<nc, 512, 0, e->st>>>(a);
         else k_update_reg<T, 2, 4><<<nc, UPD2_THREADS, 0, e->st>>>(a);
         LCHK();
         return 0;
