@@ -11,8 +11,9 @@
  *   - a chain whose sweep is a multiple of checkpoint_every is saved before
  *     that sweep runs (src/dqmc.c:233-244);
  *   - after the last sweep the file is saved unless in benchmark mode.
- * Files of one shape share an engine (one chain each); each shape group runs
- * in its own host thread with its own engine and stream.
+ * Files of one shape share an engine (one chain each); files of uneven size
+ * that dq_bucket.h admits share one too, padded to the group's largest N and
+ * L.  Each group runs in its own host thread with its own engine and stream.
  */
 #define _GNU_SOURCE
 #include "dq_driver.h"
@@ -27,6 +28,7 @@
 #include <sys/stat.h>
 #include <time.h>
 
+#include "dq_bucket.h"
 #include "dq_h5.h"
 #include "dqmc_hip.h"
 
@@ -69,6 +71,7 @@ typedef struct job {
 typedef struct group {
     job **j;
     int n;
+    int bucket; /* size bucket (dq_bucket_of), 0 for a group of one shape */
 } group;
 
 static pthread_mutex_t g_log_mu = PTHREAD_MUTEX_INITIALIZER;
@@ -108,8 +111,16 @@ static int same_shape(const dqmc_params *a, const dqmc_params *b)
            a->num_b2b2 == b->num_b2b2 && a->num_b2b == b->num_b2b && a->num_bb2 == b->num_bb2;
 }
 
-static int fetch(dqmc_engine *e, int c, job *j)
+/* chain c's state and accumulators into its file's buffers; a padded chain
+ * goes through its padded buffers */
+static int fetch(dqmc_engine *e, int c, job *j, dq_pad *pad)
 {
+    if (pad) {
+        int rc = dqmc_engine_get_state(e, c, j->sim.rng, &j->sim.sweep, pad->hs);
+        if (!rc) rc = dqmc_engine_get_meas(e, c, &pad->meq, NULL);
+        if (!rc) dq_pad_out(pad, &j->sim);
+        return rc;
+    }
     int rc = dqmc_engine_get_state(e, c, j->sim.rng, &j->sim.sweep, j->sim.hs);
     if (!rc)
         rc = dqmc_engine_get_meas(e, c, &j->sim.meq,
@@ -117,43 +128,88 @@ static int fetch(dqmc_engine *e, int c, job *j)
     return rc;
 }
 
+static void fail_group(group *g, const char *what)
+{
+    for (int c = 0; c < g->n; c++) {
+        logf_(g->j[c], "%s: %s\n", what, dqmc_last_error());
+        g->j[c]->code = WRAP_FAIL;
+    }
+}
+
 static void *run_group(void *arg)
 {
     group *g = (group *)arg;
     const int n = g->n;
+    /* uneven sizes: pad every file to the group's largest N, L and bin counts */
+    int uneven = 0, N_pad = 0, L_pad = 0, ni_pad = 0, nij_pad = 0;
+    for (int c = 0; c < n; c++) {
+        const dqmc_params *p = &g->j[c]->sim.p;
+        uneven |= !same_shape(p, &g->j[0]->sim.p);
+        if (p->N > N_pad) N_pad = p->N;
+        if (p->L > L_pad) L_pad = p->L;
+        if (p->num_i > ni_pad) ni_pad = p->num_i;
+        if (p->num_ij > nij_pad) nij_pad = p->num_ij;
+    }
+    dq_pad *pads = uneven ? calloc(n, sizeof *pads) : NULL;
+    int *Lc = uneven ? calloc(n, sizeof *Lc) : NULL, *Nc = uneven ? calloc(n, sizeof *Nc) : NULL;
     const dqmc_params **P = calloc(n, sizeof *P);
-    for (int c = 0; c < n; c++) P[c] = &g->j[c]->sim.p;
     dqmc_engine *e = NULL;
     dqmc_tuning tune = {0, 0, 0, NULL, NULL};
-    int rc = dqmc_engine_create(&e, P, n, 0, &tune);
-    free(P);
-    if (rc) {
-        for (int c = 0; c < n; c++) {
-            logf_(g->j[c], "engine creation failed: %s\n", dqmc_last_error());
-            g->j[c]->code = WRAP_FAIL;
-        }
-        return NULL;
-    }
+    int rc = 0;
     for (int c = 0; c < n && !rc; c++) {
         job *j = g->j[c];
+        P[c] = &j->sim.p;
+        if (!uneven) continue;
+        rc = dq_pad_build(&j->sim, N_pad, L_pad, ni_pad, nij_pad, &pads[c]);
+        if (rc) {
+            logf_(j, "padding to N=%d, L=%d failed: %d\n", N_pad, L_pad, rc);
+            break;
+        }
+        P[c] = &pads[c].p;
+        Lc[c] = j->sim.p.L;
+        Nc[c] = j->sim.p.N;
+    }
+    if (uneven) {
+        tune.chain_L = Lc;
+        tune.chain_N = Nc;
+    }
+    if (!rc) {
+        rc = dqmc_engine_create(&e, P, n, 0, &tune);
+        if (rc) fail_group(g, "engine creation failed");
+    } else {
+        for (int c = 0; c < n; c++) g->j[c]->code = WRAP_FAIL;
+    }
+    free(P);
+    free(Lc);
+    free(Nc);
+    if (rc) goto done;
+    for (int c = 0; c < n && !rc; c++) {
+        job *j = g->j[c];
+        if (uneven) {
+            dq_pad_in(&j->sim, &pads[c]);
+            rc = dqmc_engine_set_state(e, c, j->sim.rng, j->sim.sweep, pads[c].hs);
+            if (!rc) rc = dqmc_engine_set_meas(e, c, &pads[c].meq, NULL);
+            continue;
+        }
         rc = dqmc_engine_set_state(e, c, j->sim.rng, j->sim.sweep, j->sim.hs);
         if (!rc)
             rc = dqmc_engine_set_meas(e, c, &j->sim.meq,
                                       j->sim.p.period_uneqlt > 0 ? &j->sim.mue : NULL);
     }
     if (rc) {
-        for (int c = 0; c < n; c++) {
-            logf_(g->j[c], "engine upload failed: %s\n", dqmc_last_error());
-            g->j[c]->code = WRAP_FAIL;
-        }
+        fail_group(g, "engine upload failed");
         dqmc_engine_destroy(e);
-        return NULL;
+        goto done;
     }
     char ts[64];
     now_str(ts, sizeof ts);
     for (int c = 0; c < n; c++)
         logf_(g->j[c], "[%s] starting dqmc (MI355X engine, %d chain%s in this batch)\n", ts, n,
               n == 1 ? "" : "s");
+    if (uneven)
+        for (int c = 0; c < n; c++)
+            logf_(g->j[c], "files of uneven size in this batch: padded from N=%d, L=%d to N=%d, L=%d\n",
+                  g->j[c]->sim.p.N, g->j[c]->sim.p.L, N_pad, L_pad);
 
     int interrupted = 0, failed = 0;
     for (;;) {
@@ -169,7 +225,7 @@ static void *run_group(void *arg)
             any = 1;
             if (ce > 0 && sw % ce == 0 && j->last_ckpt != sw) {
                 j->last_ckpt = sw;
-                if (fetch(e, c, j) != 0 || dq_sim_save(&j->sim) < 0) {
+                if (fetch(e, c, j, uneven ? &pads[c] : NULL) != 0 || dq_sim_save(&j->sim) < 0) {
                     logf_(j, "dqmc() checkpoint failed\n");
                     j->failed = 1;
                     j->code = WRAP_FAIL;
@@ -201,7 +257,7 @@ static void *run_group(void *arg)
             j->code = WRAP_FAIL;
             continue;
         }
-        if (fetch(e, c, j) != 0) {
+        if (fetch(e, c, j, uneven ? &pads[c] : NULL) != 0) {
             logf_(j, "engine download failed: %s\n", dqmc_last_error());
             j->code = WRAP_FAIL;
             continue;
@@ -222,6 +278,11 @@ static void *run_group(void *arg)
         logf_(j, "[%s] %d/%d sweeps completed\n", ts, j->sim.sweep, j->sim.p.n_sweep);
     }
     dqmc_engine_destroy(e);
+done:
+    if (pads) {
+        for (int c = 0; c < n; c++) dq_pad_free(&pads[c]);
+        free(pads);
+    }
     return NULL;
 }
 
@@ -282,21 +343,25 @@ void dqmc_wrapper_batch(int n, const char *const *files, const char *const *logs
 
     if (!dry) {
         dq_sig_init();
-        /* group the runnable files by shape */
+        /* group the runnable files by shape, or by size bucket where the
+         * files allow padding */
         group *gs = calloc(n > 0 ? n : 1, sizeof *gs);
         int ng = 0;
         for (int i = 0; i < n; i++) {
             job *j = &jobs[i];
             if (!j->loaded || j->code != 0 || j->sim.sweep >= j->sim.p.n_sweep) continue;
+            const dqmc_params *p = &j->sim.p;
+            const int bk = dq_bucket_compatible(p, p) ? dq_bucket_of(p) : 0;
             int gi = -1;
-            for (int q = 0; q < ng; q++)
-                if (gs[q].n < MAX_CHAINS_PER_ENGINE && same_shape(&gs[q].j[0]->sim.p, &j->sim.p)) {
-                    gi = q;
-                    break;
-                }
+            for (int q = 0; q < ng && gi < 0; q++) {
+                if (gs[q].n >= MAX_CHAINS_PER_ENGINE || (gs[q].bucket != 0) != (bk != 0)) continue;
+                const dqmc_params *p0 = &gs[q].j[0]->sim.p;
+                if (bk ? dq_bucket_compatible(p0, p) : same_shape(p0, p)) gi = q;
+            }
             if (gi < 0) {
                 gi = ng++;
                 gs[gi].j = calloc(n, sizeof(job *));
+                gs[gi].bucket = bk;
             }
             gs[gi].j[gs[gi].n++] = j;
         }

@@ -26,7 +26,7 @@ def _run(args, timeout=120):
 
 
 def _check_against_oracle(fn, sim, eq_keys=("density", "double_occ", "nn", "zz", "pair_sw"),
-                          ue_keys=()):
+                          ue_keys=(), eq_tol=1e-9):
     r = h5.read_sim(fn)
     out = oracle.run(sim)
     assert np.array_equal(r["state"]["hs"], out["state"]["hs"]), fn
@@ -35,7 +35,7 @@ def _check_against_oracle(fn, sim, eq_keys=("density", "double_occ", "nn", "zz",
     assert int(r["state"]["partial_write"]) == 0
     assert int(r["meas_eqlt"]["n_sample"]) == int(out["meas_eqlt"]["n_sample"])
     for k in eq_keys:
-        assert rel(r["meas_eqlt"][k], out["meas_eqlt"][k]) < 1e-9, (fn, k)
+        assert rel(r["meas_eqlt"][k], out["meas_eqlt"][k]) < eq_tol, (fn, k)
     for k in ue_keys:
         assert rel(r["meas_uneqlt"][k], out["meas_uneqlt"][k]) < 1e-8, (fn, k)
     return r
@@ -94,3 +94,37 @@ def test_time_limit_interrupts_and_resume_matches_oracle(tmp_path):
     r = _run([os.path.join(BIN, "dqmc_1"), "-l", fn + ".log", fn])
     assert r.returncode == 0, open(fn + ".log").read()
     _check_against_oracle(fn, resumed)
+
+
+def test_dqmc_stack_pads_uneven_sizes(tmp_path):
+    # files of uneven N and L with one n_matmul / period_eqlt share a padded
+    # engine per element type; each must end as the oracle run of its own file:
+    # the same field and RNG state bit for bit, checkpoints included.  The
+    # accumulators are held to 1e-8 relative: at U = 6 with ten slices between
+    # recomputations the wraps amplify rounding ~exp(2 lambda 9) ~ 1e6 times,
+    # and sums of products of G entries then differ by ~1e-9 between any two
+    # correct evaluation orders (the padded G is factored at another size)
+    real = [gen.create_batch(seed=50 + k, Nx=nx, Ny=ny, U=U, mu=-0.3, L=L, n_matmul=10,
+                             period_eqlt=10, n_sweep_warm=2, n_sweep_meas=5,
+                             checkpoint_every=3)[0]
+            for k, (nx, ny, L, U) in enumerate([(4, 4, 20, 4.0), (5, 5, 30, 6.0), (6, 4, 10, 3.0)])]
+    cplx = [gen.create_batch(seed=60 + k, Nx=nx, Ny=ny, U=4.0, mu=-0.2, L=L, n_matmul=10,
+                             period_eqlt=10, nflux=1, n_sweep_warm=2, n_sweep_meas=4)[0]
+            for k, (nx, ny, L) in enumerate([(4, 4, 20), (5, 4, 30)])]
+    files = []
+    for k, s in enumerate(real + cplx):
+        fn = str(tmp_path / ("u%d.h5" % k))
+        h5.write_sim(fn, s)
+        files.append(fn)
+    stack = str(tmp_path / "stack")
+    with open(stack, "w") as f:
+        f.write("\n".join(files) + "\n")
+    r = _run([os.path.join(BIN, "dqmc_stack"), "-B", "8", stack])
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert r.stdout.count("completed:") == len(files), r.stdout
+    for fn in files:
+        assert "padded from" in open(fn + ".log").read(), fn
+    for fn, s in zip(files, real + cplx):
+        print("file", fn, "N", int(s["params"]["N"]), "L", int(s["params"]["L"]), flush=True)
+        _check_against_oracle(fn, s, eq_keys=("density", "double_occ", "g00", "nn", "xx", "zz",
+                                              "pair_sw"), eq_tol=1e-8)
