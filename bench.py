@@ -312,6 +312,10 @@ def main():
         dts, accs, _, _ = timed(1, max(2, args.steps // 2), 1, False)
         single = max(2, args.steps // 2) / dts
     achieved = g_flops / (g_ms * 1e-3) / 1e12 if g_ms > 0 else 0.0
+    # sustained FP64 MFMA rate of a register-only loop (no memory traffic):
+    # the ceiling of the instruction forms on this device, beside the
+    # published peak
+    loop = {"16x16x4": round(engine.probe_mfma(0), 2), "4x4x4": round(engine.probe_mfma(1), 2)}
     line = {
         "metric": "DQMC sweeps/s at N=256 L=80 FP64 (batched chains, aggregate)",
         "value": round(value, 4),
@@ -340,7 +344,9 @@ def main():
                      "achieved": round(achieved, 3), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(achieved / PEAK_FP64_TFLOPS, 4),
                      "traffic": GEMM_HBM_BYTES_PER_MATRIX_N256 * 2 * nch,
-                     "avg_launch_ms": round(g_ms, 5), "launches": g_n},
+                     "avg_launch_ms": round(g_ms, 5), "launches": g_n,
+                     "mfma_loop_tflops": loop,
+                     "frac_of_loop_4x4x4": round(achieved / loop["4x4x4"], 4) if loop["4x4x4"] else None},
     }
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline()

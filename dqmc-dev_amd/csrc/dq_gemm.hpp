@@ -17,7 +17,16 @@
 // Operand lane layout of v_mfma_f64_16x16x4f64 (wave64):
 //   A[16x4]: lane l holds A[l%16][l/16];  B[4x16]: lane l holds B[l/16][l%16]
 //   D[16x16]: lane l, element r holds D[4*r + l/16][l%16]
-// (input and output layouts verified on gfx950 with a single-element probe)
+// and of v_mfma_f64_4x4x4f64 (four independent 4x4x4 blocks b, one double
+// per lane):
+//   A_b[i][k] at lane i + 4b + 16k;  B_b[k][j] at lane j + 4b + 16k;
+//   D_b[i][j] at lane j + 4b + 16i
+// (layouts verified on gfx950 with probes of known operands).  On gfx950 the
+// 4x4x4 form sustains ~74 TFLOP/s in a register-only loop against ~49 for the
+// 16x16x4 form, so the main tiles use it: the four blocks form a 2 x 2 grid of
+// 4 x 4 sub-tiles, one instruction an 8 x 8 x 4 product (each operand value
+// read by two lanes).  Its CBSZ/ABID broadcast controls do not act as a
+// block broadcast for this instruction and are not used.
 #pragma once
 #include "dq_common.hpp"
 
@@ -53,6 +62,11 @@ __device__ inline dbl4 mfma(double a, double b, dbl4 c)
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
 
+__device__ inline double mfma4(double a, double b, double c)
+{
+    return __builtin_amdgcn_mfma_f64_4x4x4f64(a, b, c, 0, 0, 0);
+}
+
 // LDS image of one operand tile, BO (rows of C or columns of C) x BK, in the
 // orientation of its global storage so that the global->LDS copy is linear:
 //   OFAST: the tile's o index is contiguous in global memory -> rows [k][o],
@@ -71,12 +85,13 @@ struct GTile {
 // The MFMA computes the transposed block (op(B)^T op(A)^T), so that a lane's
 // accumulator entries run down a column of C and 16 lanes store 128
 // contiguous bytes of the column-major result.
-template <typename T, int OPA, int OPB, int TM, int TN, int BK, bool BETA>
+template <typename T, int OPA, int OPB, int TM, int TN, int BK, bool BETA, int MF = 0>
 __global__ __launch_bounds__(256) void k_gemm(GemmArgs<T> g, int tiles_m, int tiles_n, int nwg)
 {
     constexpr bool Z = is_cplx<T>::value;
     constexpr int NP = Z ? 2 : 1;
     constexpr int BM = 32 * TM, BN = 32 * TN;
+    constexpr bool M4 = MF == 1;
     constexpr bool AF = (OPA == OP_N), BF = (OPB != OP_N);
     using LA = GTile<BM, BK, AF>;
     using LB = GTile<BN, BK, BF>;
@@ -126,6 +141,20 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgs<T> g, int tiles_m, int ti
         for (int i = 0; i < TM; i++)
 #pragma unroll
             for (int j = 0; j < TN; j++) acc[p][i][j] = (dbl4){0.0, 0.0, 0.0, 0.0};
+    // 4x4x4 form: the wave tile is (8 * 2TM) x (8 * 2TN) sub-tiles of 8 x 8;
+    // lane l works on row 4 * (b & 1) + (l & 3) of the C rows of a sub-tile and
+    // column 4 * (b >> 1) + (l & 3) of its C columns for the operands, with
+    // b = (l >> 2) & 3; it holds C[row 4 * (b & 1) + (l & 3)][col 4 * (b >> 1) + (l >> 4)].
+    constexpr int QM = 2 * TM, QN = 2 * TN;
+    double acc4[NP][QN][QM];
+#pragma unroll
+    for (int p = 0; p < NP; p++)
+#pragma unroll
+        for (int i = 0; i < QN; i++)
+#pragma unroll
+            for (int j = 0; j < QM; j++) acc4[p][i][j] = 0.0;
+    const int b4 = (lane >> 2) & 3;
+    const int r4 = 4 * (b4 & 1) + (lane & 3), c4 = 4 * (b4 >> 1) + (lane & 3);
 
     // Raw global values of the next K step.  Conjugation is applied when the
     // values go to LDS and the inner scale u when the B operand is read from
@@ -174,7 +203,55 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgs<T> g, int tiles_m, int ti
         }
         if (u && tid < BK) Us[buf][tid] = uk;
     };
+    auto compute4 = [&](int buf) {
+#pragma unroll
+        for (int ks = 0; ks < BK; ks += 4) {
+            const int kk = ks + l4;
+            double xr[QM], xi[QM], yr[QN], yi[QN];
+#pragma unroll
+            for (int q = 0; q < QM; q++) {
+                xr[q] = As[buf][0][LA::at(wm + 8 * q + r4, kk)];
+                if constexpr (Z) xi[q] = As[buf][NP - 1][LA::at(wm + 8 * q + r4, kk)];
+            }
+#pragma unroll
+            for (int p = 0; p < QN; p++) {
+                yr[p] = Bs[buf][0][LB::at(wn + 8 * p + c4, kk)];
+                if constexpr (Z) yi[p] = Bs[buf][NP - 1][LB::at(wn + 8 * p + c4, kk)];
+            }
+            if (u) {
+                const double us = Us[buf][kk];
+#pragma unroll
+                for (int p = 0; p < QN; p++) {
+                    yr[p] *= us;
+                    if constexpr (Z) yi[p] *= us;
+                }
+            }
+            // D_b[i][j] = sum_k y[i][k] x[k][j]: i runs over C columns, j over C rows
+#pragma unroll
+            for (int p = 0; p < QN; p++)
+#pragma unroll
+                for (int q = 0; q < QM; q++) acc4[0][p][q] = mfma4(yr[p], xr[q], acc4[0][p][q]);
+            if constexpr (Z) {
+#pragma unroll
+                for (int p = 0; p < QN; p++)
+#pragma unroll
+                    for (int q = 0; q < QM; q++) acc4[NP - 1][p][q] = mfma4(yi[p], xr[q], acc4[NP - 1][p][q]);
+#pragma unroll
+                for (int p = 0; p < QN; p++)
+#pragma unroll
+                    for (int q = 0; q < QM; q++) acc4[0][p][q] = mfma4(-yi[p], xi[q], acc4[0][p][q]);
+#pragma unroll
+                for (int p = 0; p < QN; p++)
+#pragma unroll
+                    for (int q = 0; q < QM; q++) acc4[NP - 1][p][q] = mfma4(yr[p], xi[q], acc4[NP - 1][p][q]);
+            }
+        }
+    };
     auto compute = [&](int buf) {
+        if constexpr (M4) {
+            compute4(buf);
+            return;
+        }
 #pragma unroll
         for (int ks = 0; ks < BK; ks += 4) {
             const int kk = ks + l4;
@@ -233,6 +310,27 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgs<T> g, int tiles_m, int ti
         __syncthreads();
     }
 
+    if constexpr (M4) {
+#pragma unroll
+        for (int p = 0; p < QN; p++)
+#pragma unroll
+            for (int q = 0; q < QM; q++) {
+                const int row = m0 + wm + 8 * q + r4;
+                const int col = n0 + wn + 8 * p + 4 * (b4 >> 1) + l4;
+                if (row < g.M && col < g.N) {
+                    T v;
+                    if constexpr (Z) v = mkz(acc4[0][p][q], acc4[NP - 1][p][q]);
+                    else v = acc4[0][p][q];
+                    v = g.alpha * v;
+                    if (rr) v = v * rr[row];
+                    if (ww) v = v * ww[col];
+                    const long o = row + (long)g.ldc * col;
+                    if constexpr (BETA) v += g.beta * C[o];
+                    C[o] = v;
+                }
+            }
+        return;
+    }
 #pragma unroll
     for (int i = 0; i < TM; i++)
 #pragma unroll
@@ -255,7 +353,7 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgs<T> g, int tiles_m, int ti
             }
 }
 
-template <typename T, int TM, int TN, int BK>
+template <typename T, int TM, int TN, int BK, int MF>
 static hipError_t gemm_launch_tile(int opa, int opb, const GemmArgs<T>& g, int batch, hipStream_t s)
 {
     constexpr int BM = 32 * TM, BN = 32 * TN;
@@ -265,8 +363,8 @@ static hipError_t gemm_launch_tile(int opa, int opb, const GemmArgs<T>& g, int b
     if (nwg > 0x7fffffffL) return hipErrorInvalidValue;
     const int n = (int)nwg;
 #define GL(a, b)                                                              \
-    (g.beta_zero ? k_gemm<T, a, b, TM, TN, BK, false><<<n, 256, 0, s>>>(g, tm, tn, n) \
-                 : k_gemm<T, a, b, TM, TN, BK, true><<<n, 256, 0, s>>>(g, tm, tn, n))
+    (g.beta_zero ? k_gemm<T, a, b, TM, TN, BK, false, MF><<<n, 256, 0, s>>>(g, tm, tn, n) \
+                 : k_gemm<T, a, b, TM, TN, BK, true, MF><<<n, 256, 0, s>>>(g, tm, tn, n))
     if constexpr (!is_cplx<T>::value) {
         // conjugation is the identity on real data
         const bool ta = opa != OP_N, tb = opb != OP_N;
@@ -294,7 +392,9 @@ static hipError_t gemm_launch_tile(int opa, int opb, const GemmArgs<T>& g, int b
 }
 
 // Launch with tile shape `tile` (GT_*); GT_AUTO and shapes a product is too
-// small for fall back to 64 x 64.  GT_128 (128 x 128) is accepted as a request
+// small for fall back to 64 x 64.  The 4x4x4 form is used (M4): on the
+// 256 x 256 x 256 batch-512 wraps it takes 382 us against 412 us for the
+// 16x16x4 form, whose path stays selectable through the template argument.  GT_128 (128 x 128) is accepted as a request
 // but runs as 128 x 64: its accumulators left one wave per SIMD, and it was
 // the slowest shape at every size measured.
 template <typename T>
@@ -303,6 +403,6 @@ static hipError_t gemm_launch(int opa, int opb, const GemmArgs<T>& g, int batch,
 {
     constexpr bool Z = is_cplx<T>::value;
     if ((tile == GT_128x64 || tile == GT_128) && g.M > 64)
-        return gemm_launch_tile<T, 4, 2, 8>(opa, opb, g, batch, s);
-    return gemm_launch_tile<T, 2, 2, Z ? 8 : 16>(opa, opb, g, batch, s);
+        return gemm_launch_tile<T, 4, 2, 8, 1>(opa, opb, g, batch, s);
+    return gemm_launch_tile<T, 2, 2, Z ? 8 : 16, 1>(opa, opb, g, batch, s);
 }

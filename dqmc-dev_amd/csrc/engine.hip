@@ -1580,3 +1580,75 @@ extern "C" int dqmc_test_gemm(int is_complex, int opa, int opb, int M, int N, in
                : test_gemm_t<double>(opa, opb, M, N, K, alpha, A, lda, B, ldb, beta, Cm, ldc, r, u, w,
                                      tile);
 }
+
+// Sustained FP64 matrix-core rate (no counterpart in the reference): every
+// wave runs `iters` rounds of 8 independent MFMA accumulation chains out of
+// registers (no memory traffic), 4 workgroups of 4 waves per CU.  form 0:
+// v_mfma_f64_16x16x4f64, form 1: v_mfma_f64_4x4x4f64.  The rate is the
+// ceiling a compute-bound FP64 kernel of that form can reach on this device
+// and is read beside the published peak.
+template <int FORM>
+__global__ __launch_bounds__(256) void k_mfma_loop(double* out, int iters, double a0)
+{
+    const double a = a0 + threadIdx.x * 1e-9, b = a0 - threadIdx.x * 1e-9;
+    double s = 0.0;
+    if constexpr (FORM == 0) {
+        dbl4 acc[8];
+#pragma unroll
+        for (int c = 0; c < 8; c++) acc[c] = (dbl4){0.0, 0.0, 0.0, 0.0};
+        for (int it = 0; it < iters; it++) {
+#pragma unroll
+            for (int c = 0; c < 8; c++) acc[c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[c], 0, 0, 0);
+        }
+#pragma unroll
+        for (int c = 0; c < 8; c++) s += acc[c][0] + acc[c][1] + acc[c][2] + acc[c][3];
+    } else {
+        double acc[8];
+#pragma unroll
+        for (int c = 0; c < 8; c++) acc[c] = 0.0;
+        for (int it = 0; it < iters; it++) {
+#pragma unroll
+            for (int c = 0; c < 8; c++) acc[c] = __builtin_amdgcn_mfma_f64_4x4x4f64(a, b, acc[c], 0, 0, 0);
+        }
+#pragma unroll
+        for (int c = 0; c < 8; c++) s += acc[c];
+    }
+    out[blockIdx.x * 256 + threadIdx.x] = s;
+}
+
+extern "C" int dqmc_probe_mfma(int form, double* tflops)
+{
+    if ((form != 0 && form != 1) || !tflops) return set_err(DQMC_ERR_ARG, "bad probe form %d", form);
+    int dev = 0, ncu = 0;
+    HIPCHK(hipGetDevice(&dev));
+    HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    const int blocks = 4 * ncu, iters = form == 0 ? 4000 : 16000;
+    double* out = nullptr;
+    HIPCHK(hipMalloc(&out, sizeof(double) * 256 * blocks));
+    hipStream_t st;
+    hipEvent_t e0, e1;
+    int rc = 0;
+    if (hipStreamCreate(&st) != hipSuccess) {
+        (void)hipFree(out);
+        return set_err(DQMC_ERR_HIP, "probe stream");
+    }
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
+    float ms = 0.f;
+    for (int r = 0; r < 3 && !rc; r++) {  // the last of three launches is timed
+        hipEventRecord(e0, st);
+        if (form == 0) k_mfma_loop<0><<<blocks, 256, 0, st>>>(out, iters, 1e-3);
+        else k_mfma_loop<1><<<blocks, 256, 0, st>>>(out, iters, 1e-3);
+        hipEventRecord(e1, st);
+        if (hipEventSynchronize(e1) != hipSuccess || hipGetLastError() != hipSuccess)
+            rc = set_err(DQMC_ERR_HIP, "probe launch");
+        hipEventElapsedTime(&ms, e0, e1);
+    }
+    const double flop_per_inst = form == 0 ? 2.0 * 16 * 16 * 4 : 4 * 2.0 * 4 * 4 * 4;
+    if (!rc) *tflops = (double)blocks * 4 * iters * 8 * flop_per_inst / (ms * 1e-3) * 1e-12;
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    (void)hipStreamDestroy(st);
+    (void)hipFree(out);
+    return rc;
+}

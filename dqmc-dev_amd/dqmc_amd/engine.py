@@ -22,6 +22,15 @@ class DqmcTuning(C.Structure):
                 ("chain_L", C.POINTER(C.c_int)), ("chain_N", C.POINTER(C.c_int))]
 
 
+def probe_mfma(form):
+    """Sustained FP64 matrix-core TFLOP/s of the current device (form 0:
+    16x16x4 MFMA, 1: 4x4x4 MFMA), from a register-only loop."""
+    out = C.c_double(0.0)
+    if lib().dqmc_probe_mfma(int(form), C.byref(out)) != 0:
+        raise RuntimeError(lib().dqmc_last_error().decode())
+    return out.value
+
+
 def lib():
     global _lib
     if _lib is None:
@@ -54,6 +63,8 @@ def lib():
         L.dqmc_acc_len.restype = C.c_long
         L.dqmc_engine_gemm_tile.argtypes = [C.c_void_p]
         L.dqmc_engine_gemm_tile.restype = C.c_int
+        L.dqmc_probe_mfma.argtypes = [C.c_int, P(C.c_double)]
+        L.dqmc_probe_mfma.restype = C.c_int
         for f in ("set_state", "get_state", "set_meas", "get_meas", "run", "green", "calc_eq_g",
                   "calc_ue_g", "gemm_stats"):
             getattr(L, "dqmc_engine_" + f).restype = C.c_int

@@ -207,6 +207,13 @@ int dqmc_test_gemm(int is_complex, int opa, int opb, int M, int N, int K, const 
                    double *C, int ldc, const double *r, const double *u, const double *w,
                    int tile);
 
+/* Measurement hook (no counterpart in the reference): the sustained FP64
+ * matrix-core rate of this device in TFLOP/s, from a register-only loop of
+ * independent MFMA chains on every CU (form 0: the 16x16x4 instruction,
+ * form 1: the 4x4x4 one, which the GEMM tiles use).  Roofline context for
+ * the published peak. */
+int dqmc_probe_mfma(int form, double *tflops);
+
 /* The GEMM tile shape the engine uses (dqmc_tuning.gemm_tile values 1..3). */
 int dqmc_engine_gemm_tile(dqmc_engine *e);
 

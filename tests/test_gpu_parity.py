@@ -258,3 +258,13 @@ def test_optional_measurement_groups(nflux, trans_sym):
         assert checked >= 40
     finally:
         eng.close()
+
+
+def test_mfma_probe_rates():
+    # the register-only MFMA loop runs and gives a rate below the device peak
+    from dqmc_amd import engine as eng
+    r16, r4 = eng.probe_mfma(0), eng.probe_mfma(1)
+    print("MFMA loop TFLOP/s: 16x16x4 %.1f, 4x4x4 %.1f" % (r16, r4), flush=True)
+    assert 1.0 < r16 < 80.0 and 1.0 < r4 < 80.0
+    with pytest.raises(RuntimeError):
+        eng.probe_mfma(2)
