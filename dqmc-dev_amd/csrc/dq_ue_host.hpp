@@ -33,6 +33,7 @@ static int ue_alloc(dqmc_engine* e)
     DALLOC(e->Qb, B * (E > 1 ? E - 1 : 1) * 4 * N * N * es);
     DALLOC(e->taub, B * (NE + N) * es);
     DALLOC(e->T2, B * (long)e->L * N * N * es);
+    DALLOC(e->iehalfT, B * N * N * es);
     return 0;
 }
 
@@ -306,12 +307,39 @@ static int ue_half_wrap(dqmc_engine* e, T* X)
     return 0;
 }
 
+// Only the diagonals of the half-wrapped G(t,t) for all b, l: one GEMM
+// Y = X e^{-dtK/2}-side factor (as in ue_half_wrap) and the diagonal of
+// iehalf Y by column dot products (k_diag_dot); X keeps its off-diagonal
+// entries un-wrapped.
+template <typename T>
+static int ue_half_wrap_diag(dqmc_engine* e, T* X)
+{
+    const int N = e->N, L = e->L, B = 2 * e->nc;
+    const long NN = (long)N * N;
+    const T one = from_re<T>(1.0), zero = from_re<T>(0.0);
+    T* T2 = dp<T>(e->T2);
+    e->gemm_bdiv = L;
+    RC(gemm<T>(e, OP_N, OP_N, N, N, N, one, X, N, NN, dp<T>(e->ehalf), N, NN, zero, T2, N, NN,
+               B * L));
+    k_diag_dot<T><<<B * L, 256, 0, e->st>>>(N, L, dp<T>(e->iehalfT), NN, T2, NN, X, NN);
+    LCHK();
+    return 0;
+}
+
+// the contractions read G(t,t) off the diagonal only for the optional groups
+static bool ue_needs_full_gtt(const dqmc_engine* e)
+{
+    return e->bond || e->thermal || e->pbbonly || e->twobond || e->energy || e->gensus ||
+           (!e->cplx && e->nematic);
+}
+
 template <typename T>
 static int ue_measure(dqmc_engine* e)
 {
     RC(ue_calc<T>(e));
     RC(ue_half_wrap<T>(e, dp<T>(e->G0t)));
-    RC(ue_half_wrap<T>(e, dp<T>(e->Gtt)));
+    if (ue_needs_full_gtt(e)) RC(ue_half_wrap<T>(e, dp<T>(e->Gtt)));
+    else RC(ue_half_wrap_diag<T>(e, dp<T>(e->Gtt)));
     RC(ue_half_wrap<T>(e, dp<T>(e->Gt0)));
     MeasArgs<T> ma = meas_args<T>(e, 1);
     ma.G0t = dp<T>(e->G0t);
