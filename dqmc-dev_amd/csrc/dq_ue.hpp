@@ -153,39 +153,3 @@ __global__ __launch_bounds__(256) void k_trsm_upper(int n, int ncols, const T* U
             if (i < n && c0 + q < ncols) X[i + (long)ldx * (c0 + q)] = x[q][r];
         }
 }
-
-// dst[b] = src[b]^T (n x n, ld n), `es` doubles per element
-__global__ void k_transpose_raw(int n, int es, const double* src, double* dst, int batch)
-{
-    const long nn = (long)n * n;
-    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < nn * batch;
-         e += (long)gridDim.x * blockDim.x) {
-        const long b = e / nn, r = e % nn;
-        const int i = r % n, k = r / n;  // src element (i, k) -> dst (k, i)
-        for (int q = 0; q < es; q++)
-            dst[(b * nn + k + (long)n * i) * es + q] = src[(b * nn + i + (long)n * k) * es + q];
-    }
-}
-
-// Diagonal of the half-wrap A Y for every (b, l), z = b L + l: X[z](i, i) <-
-// sum_k A[b](i, k) Y[z](k, i), with A^T given (AT[b](k, i) = A[b](i, k)), so
-// that both operands are read down columns.  One workgroup per matrix, one
-// wave per diagonal entry at a time; the off-diagonal entries of X are left
-// as they are.
-template <typename T>
-__global__ __launch_bounds__(256) void k_diag_dot(int N, int L, const T* AT, long sA, const T* Y,
-                                                  long sY, T* X, long sX)
-{
-    const long z = blockIdx.x;
-    const T* a = AT + (z / L) * sA;
-    const T* y = Y + z * sY;
-    T* x = X + z * sX;
-    const int lane = threadIdx.x & 63;
-    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    for (int i = w; i < N; i += 4) {
-        T s = from_re<T>(0.0);
-        for (int k = lane; k < N; k += 64) s += a[k + (long)N * i] * y[k + (long)N * i];
-        s = wave_sum_r(s, lane);
-        if (lane == 0) x[i + (long)N * i] = s;
-    }
-}

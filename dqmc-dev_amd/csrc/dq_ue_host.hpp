@@ -33,7 +33,7 @@ static int ue_alloc(dqmc_engine* e)
     DALLOC(e->Qb, B * (E > 1 ? E - 1 : 1) * 4 * N * N * es);
     DALLOC(e->taub, B * (NE + N) * es);
     DALLOC(e->T2, B * (long)e->L * N * N * es);
-    DALLOC(e->iehalfT, B * N * N * es);
+    DALLOC(e->Bthi, B * (long)e->L * N * N * es);
     return 0;
 }
 
@@ -66,9 +66,203 @@ static int trtri_upper(dqmc_engine* e, int n, T* X, long sX, int ldx, T* W, long
     return 0;
 }
 
-// G0t, Gtt, Gt0 <- calc_ue_g for all b (before the half-wrap)
+// The slice products of the expansion, in the plain frame (B_l = expK
+// diag(e_l), B_l^-1 = diag(e'_l) invexpK, scales fused into the GEMM) or in
+// the half-wrapped frame (B~_l = H B_l H^-1 and its inverse, H = e^{-dtK/2}
+// side factor `iehalf`, both precomputed for all l by ue_half_factors).
+// C = alpha X op + beta C (right), C = alpha op X + beta C (left), C = B_l^-1.
 template <typename T>
-static int ue_calc(dqmc_engine* e)
+struct UeSteps {
+    dqmc_engine* e;
+    bool half;
+    int N, L, B;
+    long NN, LNN, LN;
+    T one, zero;
+    int right(T* C, const T* X, int l, bool inv, T alpha, T beta) const
+    {
+        if (half)
+            return gemm<T>(e, OP_N, OP_N, N, N, N, alpha, X, N, LNN,
+                           dp<T>(inv ? e->Bthi : e->T2) + l * NN, N, LNN, beta, C, N, LNN, B);
+        if (inv)
+            return gemm<T>(e, OP_N, OP_N, N, N, N, alpha, X, N, LNN, dp<T>(e->iexpK), N, NN, beta, C,
+                           N, LNN, B, nullptr, 0, e->iscale + (long)N * l, LN);
+        return gemm<T>(e, OP_N, OP_N, N, N, N, alpha, X, N, LNN, dp<T>(e->expK), N, NN, beta, C, N,
+                       LNN, B, nullptr, 0, nullptr, 0, e->escale + (long)N * l, LN);
+    }
+    // X, C with the stride of T1 (one matrix per b) when `tmp`
+    int left(T* C, long sC, const T* X, long sX, int l, bool inv, T alpha, T beta) const
+    {
+        if (half)
+            return gemm<T>(e, OP_N, OP_N, N, N, N, alpha, dp<T>(inv ? e->Bthi : e->T2) + l * NN, N,
+                           LNN, X, N, sX, beta, C, N, sC, B);
+        if (inv)
+            return gemm<T>(e, OP_N, OP_N, N, N, N, alpha, dp<T>(e->iexpK), N, NN, X, N, sX, beta, C,
+                           N, sC, B, e->iscale + (long)N * l, LN);
+        return gemm<T>(e, OP_N, OP_N, N, N, N, alpha, dp<T>(e->expK), N, NN, X, N, sX, beta, C, N,
+                       sC, B, nullptr, 0, e->escale + (long)N * l, LN);
+    }
+    int right_tmp(T* C, const T* X, int l, bool inv) const  // C (stride NN) = X op
+    {
+        if (half)
+            return gemm<T>(e, OP_N, OP_N, N, N, N, one, X, N, LNN,
+                           dp<T>(inv ? e->Bthi : e->T2) + l * NN, N, LNN, zero, C, N, NN, B);
+        if (inv)
+            return gemm<T>(e, OP_N, OP_N, N, N, N, one, X, N, LNN, dp<T>(e->iexpK), N, NN, zero, C,
+                           N, NN, B, nullptr, 0, e->iscale + (long)N * l, LN);
+        return gemm<T>(e, OP_N, OP_N, N, N, N, one, X, N, LNN, dp<T>(e->expK), N, NN, zero, C, N,
+                       NN, B, nullptr, 0, nullptr, 0, e->escale + (long)N * l, LN);
+    }
+    int inverse(T* C, int l) const  // C (stride LNN) = B_l^-1
+    {
+        if (half)
+            k_copy_mn<T><<<nblk(NN * B), 256, 0, e->st>>>(N, N, dp<T>(e->Bthi) + l * NN, LNN, N,
+                                                           one, C, LNN, N, B);
+        else
+            k_rowscale<T><<<nblk(NN * B), 256, 0, e->st>>>(N, dp<T>(e->iexpK), NN, 1,
+                                                          e->iscale + (long)N * l, LN, C, LNN, B);
+        LCHK();
+        return 0;
+    }
+};
+
+// B~_l = H expK diag(e_l) H^-1 into T2 and B~_l^-1 = H diag(e'_l) invexpK H^-1
+// into Bthi for all (b, l), H = iehalf, H^-1 = ehalf: two batched GEMMs over
+// all slices after the per-b products H expK (Y2) and invexpK H^-1 (Z).
+template <typename T>
+static int ue_half_factors(dqmc_engine* e)
+{
+    const int N = e->N, L = e->L, B = 2 * e->nc;
+    const long NN = (long)N * N;
+    const T one = from_re<T>(1.0), zero = from_re<T>(0.0);
+    T* M1 = dp<T>(e->Y2);
+    T* M2 = dp<T>(e->Z);
+    RC(gemm<T>(e, OP_N, OP_N, N, N, N, one, dp<T>(e->iehalf), N, NN, dp<T>(e->expK), N, NN, zero,
+               M1, N, NN, B));
+    RC(gemm<T>(e, OP_N, OP_N, N, N, N, one, dp<T>(e->iexpK), N, NN, dp<T>(e->ehalf), N, NN, zero,
+               M2, N, NN, B));
+    e->gemm_adiv = L;
+    e->gemm_bdiv = L;
+    RC(gemm<T>(e, OP_N, OP_N, N, N, N, one, M1, N, NN, dp<T>(e->ehalf), N, NN, zero, dp<T>(e->T2),
+               N, NN, B * L, nullptr, 0, e->escale, N));
+    e->gemm_adiv = L;
+    e->gemm_bdiv = L;
+    RC(gemm<T>(e, OP_N, OP_N, N, N, N, one, dp<T>(e->iehalf), N, NN, M2, N, NN, zero,
+               dp<T>(e->Bthi), N, NN, B * L, nullptr, 0, e->iscale, N));
+    return 0;
+}
+
+// X[b] (one slice of an [b][L][NN] array) <- H X H^-1 through T1
+template <typename T>
+static int half_wrap_slice(dqmc_engine* e, T* X)
+{
+    const int N = e->N, B = 2 * e->nc;
+    const long NN = (long)N * N, LNN = (long)e->L * NN;
+    const T one = from_re<T>(1.0), zero = from_re<T>(0.0);
+    T* T1 = dp<T>(e->T1);
+    RC(gemm<T>(e, OP_N, OP_N, N, N, N, one, X, N, LNN, dp<T>(e->ehalf), N, NN, zero, T1, N, NN, B));
+    RC(gemm<T>(e, OP_N, OP_N, N, N, N, one, dp<T>(e->iehalf), N, NN, T1, N, NN, zero, X, N, LNN, B));
+    return 0;
+}
+
+// expand_g (src/greens.c:339-469) with nm = (L/F)*2 slices per block: the
+// block-boundary values from bsofi (Gred) are spread over all slices by
+// products with B_l and B_l^-1.  half: the boundary values are half-wrapped
+// first and the products are the half-wrapped ones, so that G0t, Gtt, Gt0 come
+// out half-wrapped (src/dqmc.c:415-451) without a pass over all slices; the
+// identity terms are unchanged by the half-wrap.
+template <typename T>
+static int ue_expand(dqmc_engine* e, T* Gred, long sU2, bool half)
+{
+    const int N = e->N, L = e->L, F = e->F, B = 2 * e->nc;
+    const int E = 1 + (F - 1) / 2, NE = N * E, NL = NE;
+    const long NN = (long)N * N, LNN = (long)L * NN, LN = (long)L * N;
+    const T one = from_re<T>(1.0), mone = from_re<T>(-1.0), zero = from_re<T>(0.0);
+    auto BLK = [&](int i, int j) { return Gred + (long)N * i + (long)NL * N * j; };
+    T* G0t = dp<T>(e->G0t);
+    T* Gtt = dp<T>(e->Gtt);
+    T* Gt0 = dp<T>(e->Gt0);
+    T* tmp = dp<T>(e->T1);
+    const UeSteps<T> st{e, half, N, L, B, NN, LNN, LN, one, zero};
+    if (half) RC(ue_half_factors<T>(e));
+    const int nm = (L / F) * 2;
+    const int n_left = (nm - 1) / 2, n_right = nm / 2, n_up = n_left, n_down = n_right;
+    const int rstop_last = ((E - 1) * nm + L) / 2;
+    const int lstop_first = (rstop_last + 1) % L;
+    const int dstop_last = rstop_last, ustop_first = lstop_first;
+    auto cpy = [&](const T* src, T* dst) -> int {
+        k_copy_mn<T><<<nblk(NN * B), 256, 0, e->st>>>(N, N, src, sU2, NE, one, dst, LNN, N, B);
+        LCHK();
+        if (half) RC(half_wrap_slice<T>(e, dst));
+        return 0;
+    };
+    for (int f = 0; f < E; f++) RC(cpy(BLK(0, f), G0t + (long)f * nm * NN));
+    for (int f = 0; f < E; f++) {
+        const int l = f * nm;
+        const int lstop = (f == 0) ? lstop_first : l - n_left;
+        const int rstop = (f == E - 1) ? rstop_last : l + n_right;
+        for (int m = l; m != lstop;) {  // G(0, next) = alpha G(0, m) B_next
+            const int next = (m - 1 + L) % L;
+            const T alpha = (m == 0) ? mone : one;
+            RC(st.right(G0t + next * NN, G0t + m * NN, next, false, alpha, zero));
+            m = next;
+        }
+        for (int m = l; m != rstop;) {  // G(0, next) = alpha G(0, m) B_m^-1 (+ beta B_m^-1)
+            const int next = (m + 1) % L;
+            const T alpha = (next == 0) ? mone : one;
+            const T beta = (m == 0) ? from_re<T>(-re(alpha)) : zero;
+            if (m == 0) RC(st.inverse(G0t + next * NN, m));
+            RC(st.right(G0t + next * NN, G0t + m * NN, m, true, alpha, beta));
+            m = next;
+        }
+    }
+    for (int q = 0; q < E; q++) RC(cpy(BLK(q, q), Gtt + (long)q * nm * NN));
+    for (int q = 0; q < E; q++) {
+        const int k = q * nm;
+        const int ustop = (q == 0) ? ustop_first : k - n_up;
+        const int dstop = (q == E - 1) ? dstop_last : k + n_down;
+        for (int m = k; m != ustop;) {  // G(next, next) = B_next^-1 G(m, m) B_next
+            const int next = (m - 1 + L) % L;
+            RC(st.right_tmp(tmp, Gtt + m * NN, next, false));
+            RC(st.left(Gtt + next * NN, LNN, tmp, NN, next, true, one, zero));
+            m = next;
+        }
+        for (int m = k; m != dstop;) {  // G(next, next) = B_m G(m, m) B_m^-1
+            const int next = (m + 1) % L;
+            RC(st.right_tmp(tmp, Gtt + m * NN, m, true));
+            RC(st.left(Gtt + next * NN, LNN, tmp, NN, m, false, one, zero));
+            m = next;
+        }
+    }
+    for (int q = 0; q < E; q++) RC(cpy(BLK(q, 0), Gt0 + (long)q * nm * NN));
+    for (int q = 0; q < E; q++) {
+        const int k = q * nm;
+        const int ustop = (q == 0) ? ustop_first : k - n_up;
+        const int dstop = (q == E - 1) ? dstop_last : k + n_down;
+        for (int m = k; m != ustop;) {  // G(next, 0) = alpha B_next^-1 G(m, 0) (+ beta B_next^-1)
+            const int next = (m - 1 + L) % L;
+            const T alpha = (m == 0) ? mone : one;
+            const T beta = (m == 0) ? from_re<T>(-re(alpha)) : zero;
+            if (m == 0) RC(st.inverse(Gt0 + next * NN, next));
+            RC(st.left(Gt0 + next * NN, LNN, Gt0 + m * NN, LNN, next, true, alpha, beta));
+            m = next;
+        }
+        for (int m = k; m != dstop;) {  // G(next, 0) = alpha B_m G(m, 0) (+ I at next = 0)
+            const int next = (m + 1) % L;
+            const T alpha = (next == 0) ? mone : one;
+            RC(st.left(Gt0 + next * NN, LNN, Gt0 + m * NN, LNN, m, false, alpha, zero));
+            if (next == 0) {
+                k_set_identity<T><<<nblk(NN * B), 256, 0, e->st>>>(N, Gt0 + next * NN, LNN, N, 1, B);
+                LCHK();
+            }
+            m = next;
+        }
+    }
+    return 0;
+}
+
+// G0t, Gtt, Gt0 <- calc_ue_g for all b; half: half-wrapped (as measured)
+template <typename T>
+static int ue_calc(dqmc_engine* e, bool half)
 {
     const int N = e->N, L = e->L, F = e->F, B = 2 * e->nc;
     const int E = 1 + (F - 1) / 2, NE = N * E, NL = NE, N2 = 2 * N;
@@ -190,157 +384,13 @@ static int ue_calc(dqmc_engine* e)
         }
     }
 
-    // expand_g (src/greens.c:339-469) with nm = (L/F)*2 slices per block
-    T* G0t = dp<T>(e->G0t);
-    T* Gtt = dp<T>(e->Gtt);
-    T* Gt0 = dp<T>(e->Gt0);
-    T* tmp = dp<T>(e->T1);
-    const T* expK = dp<T>(e->expK);
-    const T* iexpK = dp<T>(e->iexpK);
-    const double* es = e->escale;
-    const double* is = e->iscale;
-    const int nm = (L / F) * 2;
-    const int n_left = (nm - 1) / 2, n_right = nm / 2, n_up = n_left, n_down = n_right;
-    const int rstop_last = ((E - 1) * nm + L) / 2;
-    const int lstop_first = (rstop_last + 1) % L;
-    const int dstop_last = rstop_last, ustop_first = lstop_first;
-    auto cpy = [&](const T* src, T* dst) -> int {
-        k_copy_mn<T><<<nblk(NN * B), 256, 0, e->st>>>(N, N, src, sU2, NE, one, dst, LNN, N, B);
-        LCHK();
-        return 0;
-    };
-    for (int f = 0; f < E; f++) RC(cpy(BLK(0, f), G0t + (long)f * nm * NN));
-    for (int f = 0; f < E; f++) {
-        const int l = f * nm;
-        const int lstop = (f == 0) ? lstop_first : l - n_left;
-        const int rstop = (f == E - 1) ? rstop_last : l + n_right;
-        for (int m = l; m != lstop;) {
-            const int next = (m - 1 + L) % L;
-            const T alpha = (m == 0) ? mone : one;
-            RC(gemm<T>(e, OP_N, OP_N, N, N, N, alpha, G0t + m * NN, N, LNN, expK, N, NN, zero,
-                       G0t + next * NN, N, LNN, B, nullptr, 0, nullptr, 0, es + (long)N * next, LN));
-            m = next;
-        }
-        for (int m = l; m != rstop;) {
-            const int next = (m + 1) % L;
-            const T alpha = (next == 0) ? mone : one;
-            const T beta = (m == 0) ? from_re<T>(-re(alpha)) : zero;
-            if (m == 0) {
-                k_rowscale<T><<<nblk(NN * B), 256, 0, e->st>>>(N, iexpK, NN, 1, is + (long)N * m, LN,
-                                                              G0t + next * NN, LNN, B);
-                LCHK();
-            }
-            RC(gemm<T>(e, OP_N, OP_N, N, N, N, alpha, G0t + m * NN, N, LNN, iexpK, N, NN, beta,
-                       G0t + next * NN, N, LNN, B, nullptr, 0, is + (long)N * m, LN));
-            m = next;
-        }
-    }
-    for (int q = 0; q < E; q++) RC(cpy(BLK(q, q), Gtt + (long)q * nm * NN));
-    for (int q = 0; q < E; q++) {
-        const int k = q * nm;
-        const int ustop = (q == 0) ? ustop_first : k - n_up;
-        const int dstop = (q == E - 1) ? dstop_last : k + n_down;
-        for (int m = k; m != ustop;) {
-            const int next = (m - 1 + L) % L;
-            RC(gemm<T>(e, OP_N, OP_N, N, N, N, one, Gtt + m * NN, N, LNN, expK, N, NN, zero, tmp, N,
-                       NN, B, nullptr, 0, nullptr, 0, es + (long)N * next, LN));
-            RC(gemm<T>(e, OP_N, OP_N, N, N, N, one, iexpK, N, NN, tmp, N, NN, zero, Gtt + next * NN,
-                       N, LNN, B, is + (long)N * next, LN));
-            m = next;
-        }
-        for (int m = k; m != dstop;) {
-            const int next = (m + 1) % L;
-            RC(gemm<T>(e, OP_N, OP_N, N, N, N, one, Gtt + m * NN, N, LNN, iexpK, N, NN, zero, tmp,
-                       N, NN, B, nullptr, 0, is + (long)N * m, LN));
-            RC(gemm<T>(e, OP_N, OP_N, N, N, N, one, expK, N, NN, tmp, N, NN, zero, Gtt + next * NN,
-                       N, LNN, B, nullptr, 0, es + (long)N * m, LN));
-            m = next;
-        }
-    }
-    for (int q = 0; q < E; q++) RC(cpy(BLK(q, 0), Gt0 + (long)q * nm * NN));
-    for (int q = 0; q < E; q++) {
-        const int k = q * nm;
-        const int ustop = (q == 0) ? ustop_first : k - n_up;
-        const int dstop = (q == E - 1) ? dstop_last : k + n_down;
-        for (int m = k; m != ustop;) {
-            const int next = (m - 1 + L) % L;
-            const T alpha = (m == 0) ? mone : one;
-            const T beta = (m == 0) ? from_re<T>(-re(alpha)) : zero;
-            if (m == 0) {
-                k_rowscale<T><<<nblk(NN * B), 256, 0, e->st>>>(N, iexpK, NN, 1, is + (long)N * next,
-                                                              LN, Gt0 + next * NN, LNN, B);
-                LCHK();
-            }
-            RC(gemm<T>(e, OP_N, OP_N, N, N, N, alpha, iexpK, N, NN, Gt0 + m * NN, N, LNN, beta,
-                       Gt0 + next * NN, N, LNN, B, is + (long)N * next, LN));
-            m = next;
-        }
-        for (int m = k; m != dstop;) {
-            const int next = (m + 1) % L;
-            const T alpha = (next == 0) ? mone : one;
-            RC(gemm<T>(e, OP_N, OP_N, N, N, N, alpha, expK, N, NN, Gt0 + m * NN, N, LNN, zero,
-                       Gt0 + next * NN, N, LNN, B, nullptr, 0, es + (long)N * m, LN));
-            if (next == 0) {
-                k_set_identity<T><<<nblk(NN * B), 256, 0, e->st>>>(N, Gt0 + next * NN, LNN, N, 1, B);
-                LCHK();
-            }
-            m = next;
-        }
-    }
-    return 0;
-}
-
-// X[b][l] <- e^{dtK/2} X[b][l] e^{-dtK/2} for all b, l (src/dqmc.c:420-431)
-template <typename T>
-static int ue_half_wrap(dqmc_engine* e, T* X)
-{
-    const int N = e->N, L = e->L, B = 2 * e->nc;
-    const long NN = (long)N * N;
-    const T one = from_re<T>(1.0), zero = from_re<T>(0.0);
-    T* T2 = dp<T>(e->T2);
-    e->gemm_bdiv = L;
-    RC(gemm<T>(e, OP_N, OP_N, N, N, N, one, X, N, NN, dp<T>(e->ehalf), N, NN, zero, T2, N, NN,
-               B * L));
-    e->gemm_adiv = L;
-    RC(gemm<T>(e, OP_N, OP_N, N, N, N, one, dp<T>(e->iehalf), N, NN, T2, N, NN, zero, X, N, NN,
-               B * L));
-    return 0;
-}
-
-// Only the diagonals of the half-wrapped G(t,t) for all b, l: one GEMM
-// Y = X e^{-dtK/2}-side factor (as in ue_half_wrap) and the diagonal of
-// iehalf Y by column dot products (k_diag_dot); X keeps its off-diagonal
-// entries un-wrapped.
-template <typename T>
-static int ue_half_wrap_diag(dqmc_engine* e, T* X)
-{
-    const int N = e->N, L = e->L, B = 2 * e->nc;
-    const long NN = (long)N * N;
-    const T one = from_re<T>(1.0), zero = from_re<T>(0.0);
-    T* T2 = dp<T>(e->T2);
-    e->gemm_bdiv = L;
-    RC(gemm<T>(e, OP_N, OP_N, N, N, N, one, X, N, NN, dp<T>(e->ehalf), N, NN, zero, T2, N, NN,
-               B * L));
-    k_diag_dot<T><<<B * L, 256, 0, e->st>>>(N, L, dp<T>(e->iehalfT), NN, T2, NN, X, NN);
-    LCHK();
-    return 0;
-}
-
-// the contractions read G(t,t) off the diagonal only for the optional groups
-static bool ue_needs_full_gtt(const dqmc_engine* e)
-{
-    return e->bond || e->thermal || e->pbbonly || e->twobond || e->energy || e->gensus ||
-           (!e->cplx && e->nematic);
+    return ue_expand<T>(e, Gred, sU2, half);
 }
 
 template <typename T>
 static int ue_measure(dqmc_engine* e)
 {
-    RC(ue_calc<T>(e));
-    RC(ue_half_wrap<T>(e, dp<T>(e->G0t)));
-    if (ue_needs_full_gtt(e)) RC(ue_half_wrap<T>(e, dp<T>(e->Gtt)));
-    else RC(ue_half_wrap_diag<T>(e, dp<T>(e->Gtt)));
-    RC(ue_half_wrap<T>(e, dp<T>(e->Gt0)));
+    RC(ue_calc<T>(e, true));
     MeasArgs<T> ma = meas_args<T>(e, 1);
     ma.G0t = dp<T>(e->G0t);
     ma.Gtt = dp<T>(e->Gtt);
@@ -396,7 +446,7 @@ static int ue_hook(dqmc_engine* e, int c, int spin, double* G0t, double* Gtt, do
                                                                e->escale, e->iscale);
     LCHK();
     RC(build_C<T>(e, 0, e->F));
-    RC(ue_calc<T>(e));
+    RC(ue_calc<T>(e, false));
     HIPCHK(hipStreamSynchronize(e->st));
     const size_t bytes = (size_t)e->L * N * N * sizeof(T);
     const int b = 2 * c + spin;

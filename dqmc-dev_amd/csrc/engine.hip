@@ -149,7 +149,7 @@ struct dqmc_engine {
     std::vector<int> sweep, n_sweep, n_sweep_warm;  // host mirrors
     bool stale = true;
     void *expK = 0, *iexpK = 0, *ehalf = 0, *iehalf = 0;  // [B][NN] T
-    void* iehalfT = 0;  // [B][NN] T, transposed iehalf (unequal-time only)
+    void* Bthi = 0;  // [B][L][NN] T, half-wrapped B_l^-1 (unequal-time only)
     double *exp_lambda = 0, *del = 0;                       // [nc][2N]
     int* hs = 0;                                            // [nc][L N]
     uint64_t* rng = 0;                                      // [nc][17]
@@ -1199,12 +1199,6 @@ static int engine_setup(dqmc_engine* e, const dqmc_params* const* P)
     }
     HIPCHK(hipMemcpy(e->d_nsweep, e->n_sweep.data(), nc * 4, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(e->d_warm, e->n_sweep_warm.data(), nc * 4, hipMemcpyHostToDevice));
-    if (e->iehalfT) {  // the unequal-time diagonal half-wrap reads iehalf^T
-        k_transpose_raw<<<nblk(NN * 2 * nc), 256, 0, e->st>>>(N, (int)(e->esz / 8),
-                                                               (const double*)e->iehalf,
-                                                               (double*)e->iehalfT, 2 * nc);
-        LCHK();
-    }
     HIPCHK(hipStreamSynchronize(e->st));
     return 0;
 }
