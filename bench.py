@@ -201,7 +201,8 @@ def bench_config4(args):
     engs = []
     for bk in sorted(groups):
         npad = edges[bk][1] ** 2
-        engs.append(buckets.PaddedBatch(groups[bk], 10 if npad < 200 else 8))
+        # the bucket engines run from concurrent host threads: no graph capture
+        engs.append(buckets.PaddedBatch(groups[bk], 10 if npad < 200 else 8, graphs=False))
         print("config 4: bucket %d: %d chains padded to N=%d L=%d" %
               (bk, len(groups[bk]), engs[-1].N, engs[-1].L), file=sys.stderr, flush=True)
     dt, n = _run_engines(engs, args.steps, args.warmup, len(engs))

@@ -153,3 +153,11 @@ __global__ __launch_bounds__(256) void k_trsm_upper(int n, int ncols, const T* U
             if (i < n && c0 + q < ncols) X[i + (long)ldx * (c0 + q)] = x[q][r];
         }
 }
+
+// p[0:n) = 0 (a fill kernel rather than a memset node, so that the sweeps can
+// be captured into HIP graphs)
+__global__ void k_zero(long n, double* p)
+{
+    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x)
+        p[e] = 0.0;
+}

@@ -264,9 +264,9 @@ static int ue_expand(dqmc_engine* e, T* Gred, long sU2, bool half)
 template <typename T>
 static int ue_calc(dqmc_engine* e, bool half)
 {
-    const int N = e->N, L = e->L, F = e->F, B = 2 * e->nc;
+    const int N = e->N, F = e->F, B = 2 * e->nc;
     const int E = 1 + (F - 1) / 2, NE = N * E, NL = NE, N2 = 2 * N;
-    const long NN = (long)N * N, LNN = (long)L * NN, LN = (long)L * N;
+    const long NN = (long)N * N;
     const long sU2 = ue_stride(e);
     T* Gred = dp<T>(e->Gred);
     T* Wa = dp<T>(e->Wa);
@@ -278,7 +278,11 @@ static int ue_calc(dqmc_engine* e, bool half)
     auto BLK = [&](int i, int j) { return Gred + (long)N * i + (long)NL * N * j; };
 
     // calc_o (src/greens.c:247-264)
-    HIPCHK(hipMemsetAsync(Gred, 0, (size_t)B * sU2 * sizeof(T), e->st));
+    {
+        const long nd = (long)B * sU2 * (long)(sizeof(T) / 8);
+        k_zero<<<nblk(nd), 256, 0, e->st>>>(nd, (double*)Gred);
+        LCHK();
+    }
     for (int q = 0; q < E - 1; q++)
         RC(mul_seq_C<T>(e, q * 2, (q + 1) * 2, mone, BLK(q + 1, q), NE, sU2));
     RC(mul_seq_C<T>(e, (E - 1) * 2, 0, one, BLK(0, E - 1), NE, sU2));

@@ -205,6 +205,13 @@ struct dqmc_engine {
     // GEMM timing (bench)
     bool time_gemm = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> gemm_events;
+    // one sweep captured per measurement pattern (bit 0: equal-time, bit 1:
+    // unequal-time) and replayed; off while the GEMMs are timed.  Captured in
+    // relaxed mode: engines driven from other host threads keep making
+    // synchronous calls while one of them captures.
+    bool use_graph = true;
+    hipGraph_t graph[4] = {};
+    hipGraphExec_t gexec[4] = {};
     double gemm_ms = 0.0, gemm_flops = 0.0;
     long long gemm_launches = 0;
 };
@@ -265,13 +272,13 @@ static int gemm(dqmc_engine* e, int opa, int opb, int M, int N, int K, T alpha, 
     hipEvent_t a0 = 0, a1 = 0;
     const bool timed = e->time_gemm && M == e->N && N == e->N && K == e->N && batch == 2 * e->nc;
     if (timed) {
-        hipEventCreate(&a0);
-        hipEventCreate(&a1);
-        hipEventRecord(a0, e->st);
+        (void)hipEventCreate(&a0);
+        (void)hipEventCreate(&a1);
+        (void)hipEventRecord(a0, e->st);
     }
     hipError_t err = gemm_launch<T>(opa, opb, g, batch, e->st, e->gemm_tile);
     if (timed) {
-        hipEventRecord(a1, e->st);
+        (void)hipEventRecord(a1, e->st);
         e->gemm_events.push_back({a0, a1});
         e->gemm_flops = 2.0 * M * N * (double)K * batch * (is_cplx<T>::value ? 4 : 1);
     }
@@ -847,15 +854,83 @@ static int launch_update(dqmc_engine* e, int l)
     return 0;
 }
 
+// One sweep of every chain (device work only: kernel launches on the engine
+// stream, so that it can be captured into a HIP graph): flags, the sweep's
+// random stream, L slices of updates with wraps or recalculations, the
+// equal-time measurements and, if due, the unequal-time ones.
 template <typename T>
-static int run_t(dqmc_engine* e, int n_sweeps, const volatile int* stop)
+static int sweep_body(dqmc_engine* e, bool any_eq, bool any_ue)
 {
     const int N = e->N, L = e->L, nc = e->nc, B = 2 * nc, n = e->n_matmul, F = e->F;
     const long NN = (long)N * N, LN = (long)L * N;
     const T one = from_re<T>(1.0), zero = from_re<T>(0.0);
-    if (e->stale) RC(init_green<T>(e));
     T* G = dp<T>(e->G);
     T* T1 = dp<T>(e->T1);
+    k_sweep_flags<<<(nc + 255) / 256, 256, 0, e->st>>>(nc, e->period_uneqlt, e->d_sweep,
+                                                        e->d_nsweep, e->d_warm, e->active,
+                                                        e->flag_eq, e->flag_ue);
+    LCHK();
+    // the sweep's random stream: site orders and uniforms (dq_rng.hpp)
+    k_rng_raw<<<nc, 64, 0, e->st>>>(N, L, e->draws, e->rng, e->raw, 2L * L * N, e->active,
+                                    e->d_Lc, e->d_Nc);
+    LCHK();
+    k_rng_orders<<<nc, 256, 0, e->st>>>(N, L, e->raw, 2L * L * N, e->del, e->draws, e->order,
+                                        e->unif, e->active, e->d_Lc, e->d_Nc);
+    LCHK();
+    for (int l = 0; l < L; l++) {
+        RC(launch_update<T>(e, l));
+        const int f = l / n;
+        const bool recalc = ((l + 1) % n == 0);
+        if (recalc) {
+            RC(build_C<T>(e, f, 1));
+            RC(calc_eq_g<T>(e, (f + 1) % F, G, dp<T>(e->phb)));
+            k_phase_combine<T><<<(nc + 255) / 256, 256, 0, e->st>>>(nc, dp<T>(e->phb),
+                                                                     dp<T>(e->phase), e->active);
+            LCHK();
+        } else {
+            // G <- B_l G B_l^-1 = expK diag(e_l) [G diag(e'_l) invexpK]  (src/dqmc.c:294-295)
+            RC(gemm<T>(e, OP_N, OP_N, N, N, N, one, G, N, NN, dp<T>(e->iexpK), N, NN, zero, T1,
+                       N, NN, B, nullptr, 0, e->iscale + (long)N * l, LN, nullptr, 0, l, 2));
+            RC(gemm<T>(e, OP_N, OP_N, N, N, N, one, dp<T>(e->expK), N, NN, T1, N, NN, zero, G,
+                       N, NN, B, nullptr, 0, e->escale + (long)N * l, LN, nullptr, 0, l, 1));
+        }
+        if (any_eq && (l + 1) % e->period_eqlt == 0) {
+            // half-wrap e^{dtK/2} G e^{-dtK/2} (src/dqmc.c:364-380), into Y2
+            RC(gemm<T>(e, OP_N, OP_N, N, N, N, one, G, N, NN, dp<T>(e->ehalf), N, NN, zero, T1,
+                       N, NN, B));
+            RC(gemm<T>(e, OP_N, OP_N, N, N, N, one, dp<T>(e->iehalf), N, NN, T1, N, NN, zero,
+                       dp<T>(e->Y2), N, NN, B));
+            MeasArgs<T> ma = meas_args<T>(e, 0);
+            ma.G = dp<T>(e->Y2);
+            ma.sG = NN;
+            if (e->ragged) {  // chains past their own L do not measure
+                k_flag_slice<<<(nc + 255) / 256, 256, 0, e->st>>>(nc, e->flag_eq, e->d_Lc, l,
+                                                                   e->flag_sl);
+                LCHK();
+                ma.flag = e->flag_sl;
+            }
+            const int nb = (std::max(e->num_i, e->num_ij) + 255) / 256;
+            k_meas_eq<T><<<dim3(nb, nc), 256, 0, e->st>>>(ma);
+            LCHK();
+            if (e->energy) {
+                const int nb2 = (std::max(e->num_bs, e->num_bb) + 255) / 256;
+                k_meas_eq_energy<T><<<dim3(nb2, nc), 256, 0, e->st>>>(ma);
+                LCHK();
+            }
+            RC(meas_eq_extra<T>(e, ma));
+        }
+    }
+    if (any_ue) RC(ue_measure<T>(e));
+    k_sweep_advance<<<(nc + 255) / 256, 256, 0, e->st>>>(nc, e->d_sweep, e->active);
+    LCHK();
+    return 0;
+}
+
+template <typename T>
+static int run_t(dqmc_engine* e, int n_sweeps, const volatile int* stop)
+{
+    const int nc = e->nc;
+    if (e->stale) RC(init_green<T>(e));
     for (int s = 0; s < n_sweeps; s++) {
         if (stop && *stop) {
             HIPCHK(hipStreamSynchronize(e->st));
@@ -870,63 +945,30 @@ static int run_t(dqmc_engine* e, int n_sweeps, const volatile int* stop)
                       e->sweep[c] % e->period_uneqlt == 0;
         }
         if (!any_active) break;
-        k_sweep_flags<<<(nc + 255) / 256, 256, 0, e->st>>>(nc, e->period_uneqlt, e->d_sweep,
-                                                            e->d_nsweep, e->d_warm, e->active,
-                                                            e->flag_eq, e->flag_ue);
-        LCHK();
-        // the sweep's random stream: site orders and uniforms (dq_rng.hpp)
-        k_rng_raw<<<nc, 64, 0, e->st>>>(N, L, e->draws, e->rng, e->raw, 2L * L * N, e->active,
-                                        e->d_Lc, e->d_Nc);
-        LCHK();
-        k_rng_orders<<<nc, 256, 0, e->st>>>(N, L, e->raw, 2L * L * N, e->del, e->draws, e->order,
-                                            e->unif, e->active, e->d_Lc, e->d_Nc);
-        LCHK();
-        for (int l = 0; l < L; l++) {
-            RC(launch_update<T>(e, l));
-            const int f = l / n;
-            const bool recalc = ((l + 1) % n == 0);
-            if (recalc) {
-                RC(build_C<T>(e, f, 1));
-                RC(calc_eq_g<T>(e, (f + 1) % F, G, dp<T>(e->phb)));
-                k_phase_combine<T><<<(nc + 255) / 256, 256, 0, e->st>>>(nc, dp<T>(e->phb),
-                                                                         dp<T>(e->phase), e->active);
-                LCHK();
-            } else {
-                // G <- B_l G B_l^-1 = expK diag(e_l) [G diag(e'_l) invexpK]  (src/dqmc.c:294-295)
-                RC(gemm<T>(e, OP_N, OP_N, N, N, N, one, G, N, NN, dp<T>(e->iexpK), N, NN, zero, T1,
-                           N, NN, B, nullptr, 0, e->iscale + (long)N * l, LN, nullptr, 0, l, 2));
-                RC(gemm<T>(e, OP_N, OP_N, N, N, N, one, dp<T>(e->expK), N, NN, T1, N, NN, zero, G,
-                           N, NN, B, nullptr, 0, e->escale + (long)N * l, LN, nullptr, 0, l, 1));
-            }
-            if (any_eq && (l + 1) % e->period_eqlt == 0) {
-                // half-wrap e^{dtK/2} G e^{-dtK/2} (src/dqmc.c:364-380), into Y2
-                RC(gemm<T>(e, OP_N, OP_N, N, N, N, one, G, N, NN, dp<T>(e->ehalf), N, NN, zero, T1,
-                           N, NN, B));
-                RC(gemm<T>(e, OP_N, OP_N, N, N, N, one, dp<T>(e->iehalf), N, NN, T1, N, NN, zero,
-                           dp<T>(e->Y2), N, NN, B));
-                MeasArgs<T> ma = meas_args<T>(e, 0);
-                ma.G = dp<T>(e->Y2);
-                ma.sG = NN;
-                if (e->ragged) {  // chains past their own L do not measure
-                    k_flag_slice<<<(nc + 255) / 256, 256, 0, e->st>>>(nc, e->flag_eq, e->d_Lc, l,
-                                                                       e->flag_sl);
-                    LCHK();
-                    ma.flag = e->flag_sl;
+        const int key = (any_eq ? 1 : 0) | (any_ue ? 2 : 0);
+        if (e->use_graph && !e->time_gemm) {
+            if (!e->gexec[key]) {
+                HIPCHK(hipStreamBeginCapture(e->st, hipStreamCaptureModeRelaxed));
+                const int rc = sweep_body<T>(e, any_eq, any_ue);
+                hipGraph_t g = nullptr;
+                const hipError_t ce = hipStreamEndCapture(e->st, &g);
+                if (rc) {
+                    if (g) (void)hipGraphDestroy(g);
+                    return rc;
                 }
-                const int nb = (std::max(e->num_i, e->num_ij) + 255) / 256;
-                k_meas_eq<T><<<dim3(nb, nc), 256, 0, e->st>>>(ma);
-                LCHK();
-                if (e->energy) {
-                    const int nb2 = (std::max(e->num_bs, e->num_bb) + 255) / 256;
-                    k_meas_eq_energy<T><<<dim3(nb2, nc), 256, 0, e->st>>>(ma);
-                    LCHK();
+                if (ce != hipSuccess)
+                    return set_err(DQMC_ERR_HIP, "sweep capture: %s", hipGetErrorString(ce));
+                e->graph[key] = g;
+                const hipError_t ie = hipGraphInstantiate(&e->gexec[key], g, nullptr, nullptr, 0);
+                if (ie != hipSuccess) {
+                    e->gexec[key] = nullptr;
+                    return set_err(DQMC_ERR_HIP, "sweep graph: %s", hipGetErrorString(ie));
                 }
-                RC(meas_eq_extra<T>(e, ma));
             }
+            HIPCHK(hipGraphLaunch(e->gexec[key], e->st));
+        } else {
+            RC(sweep_body<T>(e, any_eq, any_ue));
         }
-        if (any_ue) RC(ue_measure<T>(e));
-        k_sweep_advance<<<(nc + 255) / 256, 256, 0, e->st>>>(nc, e->d_sweep, e->active);
-        LCHK();
         for (int c = 0; c < nc; c++)
             if (e->sweep[c] < e->n_sweep[c]) e->sweep[c]++;
     }
@@ -1224,12 +1266,12 @@ static int autotune_gemm(dqmc_engine* e)
         e->gemm_tile = cand[c];
         const int reps = 3;
         for (int r = 0; r <= reps && !rc; r++) {
-            if (r == 1) hipEventRecord(a0, e->st);
+            if (r == 1) (void)hipEventRecord(a0, e->st);
             rc = gemm<T>(e, OP_N, OP_N, N, N, N, from_re<T>(1.0), dp<T>(e->expK), N, NN,
                          dp<T>(e->G), N, NN, from_re<T>(0.0), dp<T>(e->X), N, NN, B);
         }
         if (rc) break;
-        hipEventRecord(a1, e->st);
+        (void)hipEventRecord(a1, e->st);
         if (hipEventSynchronize(a1) != hipSuccess) { rc = set_err(DQMC_ERR_HIP, "autotune sync"); break; }
         float ms = 0.f;
         hipEventElapsedTime(&ms, a0, a1);
@@ -1294,6 +1336,7 @@ extern "C" int dqmc_engine_create(dqmc_engine** out, const dqmc_params* const* P
     e->kd = (tune && tune->delay_rank > 0) ? tune->delay_rank : (p0->n_delay > 0 ? p0->n_delay : 16);
     if (e->kd > e->N) e->kd = e->N;
     e->time_gemm = tune && (tune->flags & 1);
+    e->use_graph = !(tune && (tune->flags & 2));
     if (tune && (tune->chain_L || tune->chain_N)) {
         // chains of uneven size, padded by the caller to N and L
         e->ragged = true;
@@ -1345,6 +1388,10 @@ extern "C" void dqmc_engine_destroy(dqmc_engine* e)
     for (auto& ev : e->gemm_events) {
         hipEventDestroy(ev.first);
         hipEventDestroy(ev.second);
+    }
+    for (int k = 0; k < 4; k++) {
+        if (e->gexec[k]) hipGraphExecDestroy(e->gexec[k]);
+        if (e->graph[k]) hipGraphDestroy(e->graph[k]);
     }
     for (void* p : e->allocs) hipFree(p);
     if (e->st) hipStreamDestroy(e->st);

@@ -104,9 +104,10 @@ class PaddedBatch:
     """One engine for chains of uneven N and L (see the module docstring).
 
     sims: simulation dicts of one dtype; n_matmul: the engine's group size
-    (the last group of a chain whose L it does not divide is partial)."""
+    (the last group of a chain whose L it does not divide is partial);
+    graphs: as for :class:`dqmc_amd.engine.Engine`."""
 
-    def __init__(self, sims, n_matmul, device=0, gemm_tile=0):
+    def __init__(self, sims, n_matmul, device=0, gemm_tile=0, graphs=True):
         self.sims = sims
         self.Nc = [_int(s["params"]["N"]) for s in sims]
         self.Lc = [_int(s["params"]["L"]) for s in sims]
@@ -117,7 +118,7 @@ class PaddedBatch:
         self.num_ij = [_int(s["params"]["num_ij"]) for s in sims]
         ni, nij = max(self.num_i), max(self.num_ij)
         self.padded = [pad_sim(s, self.N, self.L, n_matmul, ni, nij) for s in sims]
-        self.engine = _eng.Engine(self.padded, device=device, gemm_tile=gemm_tile,
+        self.engine = _eng.Engine(self.padded, device=device, gemm_tile=gemm_tile, graphs=graphs,
                                   chain_L=self.Lc, chain_N=self.Nc)
 
     @property

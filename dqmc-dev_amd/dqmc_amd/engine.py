@@ -91,10 +91,15 @@ class Engine:
 
     sims: list of simulation dicts ({"params", "state", "meas_eqlt",
     "meas_uneqlt"}) -- all with the same N, L, n_matmul, dtype and
-    measurement set.  State and accumulators are uploaded at construction."""
+    measurement set.  State and accumulators are uploaded at construction.
+
+    graphs: capture each sweep into a HIP graph (one per measurement pattern)
+    and replay it.  HIP's stream capture fails while other host threads make
+    HIP calls, so engines driven concurrently from several threads need
+    graphs=False.  Engines that time their GEMMs (time_gemm) run eagerly."""
 
     def __init__(self, sims, device=0, delay_rank=0, time_gemm=False, gemm_tile=0, chain_L=None,
-                 chain_N=None):
+                 chain_N=None, graphs=True):
         self.L = lib()
         self.sims = sims
         self.cplx = capi.is_complex(sims[0])
@@ -111,7 +116,7 @@ class Engine:
         # own sizes of chains padded to the common N, L (dqmc_amd.buckets)
         self._cl = None if chain_L is None else (C.c_int * len(sims))(*chain_L)
         self._cn = None if chain_N is None else (C.c_int * len(sims))(*chain_N)
-        tune = DqmcTuning(delay_rank, 1 if time_gemm else 0, gemm_tile,
+        tune = DqmcTuning(delay_rank, (1 if time_gemm else 0) | (0 if graphs else 2), gemm_tile,
                           C.cast(self._cl, C.POINTER(C.c_int)) if self._cl is not None else None,
                           C.cast(self._cn, C.POINTER(C.c_int)) if self._cn is not None else None)
         h = C.c_void_p()

@@ -72,6 +72,7 @@ typedef struct group {
     job **j;
     int n;
     int bucket; /* size bucket (dq_bucket_of), 0 for a group of one shape */
+    int alone;  /* the only group: its sweeps may be captured into HIP graphs */
 } group;
 
 static pthread_mutex_t g_log_mu = PTHREAD_MUTEX_INITIALIZER;
@@ -154,7 +155,9 @@ static void *run_group(void *arg)
     int *Lc = uneven ? calloc(n, sizeof *Lc) : NULL, *Nc = uneven ? calloc(n, sizeof *Nc) : NULL;
     const dqmc_params **P = calloc(n, sizeof *P);
     dqmc_engine *e = NULL;
-    dqmc_tuning tune = {0, 0, 0, NULL, NULL};
+    /* HIP stream capture fails while other host threads make HIP calls, so
+     * graphs are used only when this group is the only one */
+    dqmc_tuning tune = {0, g->alone ? 0 : 2, 0, NULL, NULL};
     int rc = 0;
     for (int c = 0; c < n && !rc; c++) {
         job *j = g->j[c];
@@ -368,6 +371,7 @@ void dqmc_wrapper_batch(int n, const char *const *files, const char *const *logs
         pthread_t *th = calloc(ng > 0 ? ng : 1, sizeof *th);
         int *joined = calloc(ng > 0 ? ng : 1, sizeof *joined);
         const int64_t deadline = max_time_ns > 0 ? wall_start + max_time_ns : 0;
+        for (int q = 0; q < ng; q++) gs[q].alone = (ng == 1);
         for (int q = 0; q < ng; q++) pthread_create(&th[q], NULL, run_group, &gs[q]);
         if (deadline) {
             /* the time limit is a stop flag raised here; the engine polls the

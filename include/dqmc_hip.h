@@ -113,7 +113,11 @@ typedef struct dqmc_engine dqmc_engine;
 /* Tuning knobs picked by the autotuner; zero fields mean "engine default". */
 typedef struct dqmc_tuning {
     int delay_rank; /* rank of the delayed Metropolis update (register slots) */
-    int flags;      /* bit 0: time the N x N x N GEMMs with HIP events */
+    int flags;      /* bit 0: time the N x N x N GEMMs with HIP events (the
+                       sweeps then run launch by launch); bit 1: do not capture
+                       the sweeps into HIP graphs (needed when other host
+                       threads make HIP calls while this engine runs: HIP's
+                       stream capture fails then) */
     int gemm_tile;  /* C tile of the MFMA GEMM: 0 = time the shapes at creation
                        and keep the fastest, 1 = 64x64, 2 = 128x128, 3 = 128x64 */
     /* Chains of uneven size in one engine (both NULL: every chain has the

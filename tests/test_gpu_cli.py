@@ -25,6 +25,14 @@ def _run(args, timeout=120):
     return subprocess.run(args, capture_output=True, text=True, timeout=timeout)
 
 
+def _logs(files):
+    out = ""
+    for fn in files:
+        if os.path.exists(fn + ".log"):
+            out += "\n--- %s.log\n%s" % (fn, open(fn + ".log").read()[-1500:])
+    return out
+
+
 def _check_against_oracle(fn, sim, eq_keys=("density", "double_occ", "nn", "zz", "pair_sw"),
                           ue_keys=(), eq_tol=1e-9):
     r = h5.read_sim(fn)
@@ -65,7 +73,7 @@ def test_dqmc_stack_batches_two_shapes(tmp_path):
         f.write("\n".join(fa + fb) + "\n")
     r = _run([os.path.join(BIN, "dqmc_stack"), "-B", "8", stack])
     assert r.returncode == 0, r.stdout + r.stderr
-    assert r.stdout.count("completed:") == 5, r.stdout
+    assert r.stdout.count("completed:") == 5, r.stdout + _logs(fa + fb)
     assert open(stack).read().strip() == ""
     for fn, s in zip(fa, a):
         _check_against_oracle(fn, s, ue_keys=("gt0", "nn", "zz"))
@@ -121,7 +129,7 @@ def test_dqmc_stack_pads_uneven_sizes(tmp_path):
         f.write("\n".join(files) + "\n")
     r = _run([os.path.join(BIN, "dqmc_stack"), "-B", "8", stack])
     assert r.returncode == 0, r.stdout + r.stderr
-    assert r.stdout.count("completed:") == len(files), r.stdout
+    assert r.stdout.count("completed:") == len(files), r.stdout + _logs(files)
     for fn in files:
         assert "padded from" in open(fn + ".log").read(), fn
     for fn, s in zip(files, real + cplx):
