@@ -254,7 +254,7 @@ def main():
                     help="first run one device-to-device copy of MB MiB (a known-size kernel that "
                          "calibrates the FETCH_SIZE / WRITE_SIZE counters of a profiler pass)")
     ap.add_argument("--gemm-tile", type=int, default=0,
-                    help="GEMM C tile: 0 = autotune, 1 = 64x64, 2 = 128x128, 3 = 128x64")
+                    help="GEMM C tile: 0 = autotune, 1 = 64x64, 2 = 128x64 (K step 16), 3 = 128x64 (K step 8)")
     ap.add_argument("--config", type=int, default=2, choices=(1, 2, 3, 4, 5),
                     help="workload: 2 = the headline N=256 L=80 batch (default); 1 = 4x4 "
                          "single chain; 3 = complex Peierls N=144; 4 = 128 uneven chains; "
@@ -283,7 +283,7 @@ def main():
         torch.cuda.synchronize()
         del src, dst
 
-    tiles = {1: "64x64", 2: "128x128", 3: "128x64"}
+    tiles = {1: "64x64", 2: "128x64/16", 3: "128x64/8"}
 
     def timed(nch, steps, warmup, time_gemm):
         sims = workload(nch)

@@ -10,8 +10,8 @@
 //   rank-k flush G += A B^T, Q^H products, half-wraps, T updates.
 // B_l is never materialised: the HS-field scaling is applied on the fly.
 //
-// Tiling for CDNA4: 256 threads = 4 wave64s per C tile (64x64, 128x64 or
-// 128x128), each wave a 2-D array of v_mfma_f64_16x16x4 blocks; K is staged
+// Tiling for CDNA4: 256 threads = 4 wave64s per C tile (64x64 or 128x64),
+// each wave a 2-D array of FP64 MFMA blocks; K is staged
 // through double-buffered LDS.  Complex (zd) uses four real MFMAs per block on
 // planar (re/im) LDS copies.
 // Operand lane layout of v_mfma_f64_16x16x4f64 (wave64):
@@ -394,15 +394,18 @@ static hipError_t gemm_launch_tile(int opa, int opb, const GemmArgs<T>& g, int b
 // Launch with tile shape `tile` (GT_*); GT_AUTO and shapes a product is too
 // small for fall back to 64 x 64.  The 4x4x4 form is used (M4): on the
 // 256 x 256 x 256 batch-512 wraps it takes 382 us against 412 us for the
-// 16x16x4 form, whose path stays selectable through the template argument.  GT_128 (128 x 128) is accepted as a request
-// but runs as 128 x 64: its accumulators left one wave per SIMD, and it was
-// the slowest shape at every size measured.
+// 16x16x4 form, whose path stays selectable through the template argument.
+// GT_128 is the 128 x 64 tile with K steps of 16 (GT_128x64 steps by 8); a
+// 128 x 128 tile needs 268 registers (one wave per SIMD) and was the slowest
+// shape measured.
 template <typename T>
 static hipError_t gemm_launch(int opa, int opb, const GemmArgs<T>& g, int batch, hipStream_t s,
                               int tile)
 {
     constexpr bool Z = is_cplx<T>::value;
-    if ((tile == GT_128x64 || tile == GT_128) && g.M > 64)
+    if (tile == GT_128 && g.M > 64)  // 128 x 64 with K steps of 16 (8 complex)
+        return gemm_launch_tile<T, 4, 2, Z ? 8 : 16, 1>(opa, opb, g, batch, s);
+    if (tile == GT_128x64 && g.M > 64)
         return gemm_launch_tile<T, 4, 2, 8, 1>(opa, opb, g, batch, s);
     return gemm_launch_tile<T, 2, 2, Z ? 8 : 16, 1>(opa, opb, g, batch, s);
 }

@@ -119,7 +119,8 @@ typedef struct dqmc_tuning {
                        threads make HIP calls while this engine runs: HIP's
                        stream capture fails then) */
     int gemm_tile;  /* C tile of the MFMA GEMM: 0 = time the shapes at creation
-                       and keep the fastest, 1 = 64x64, 2 = 128x128, 3 = 128x64 */
+                       and keep the fastest, 1 = 64x64, 2 = 128x64 with K
+                       steps of 16, 3 = 128x64 with K steps of 8 */
     /* Chains of uneven size in one engine (both NULL: every chain has the
      * params' N and L).  chain_L[c] <= L and chain_N[c] <= N are chain c's own
      * slice and site counts; its params are padded to the common N and L by
