@@ -175,10 +175,13 @@ def bench_config4(args):
     U in {2, 4, 8}, mu in [-1.5, 0], t'=0, seed 99 + chain (host draws from
     numpy default_rng(99)).  Chains go to five size buckets (sides 6-8, 9-11,
     12-14, 15-17, 18-20), each one engine padded to its largest N and L
-    (dqmc_amd.buckets: inert padding sites, identity padding slices); the
-    bucket's n_matmul is 10 (8 for N > 200) with a partial last group where it
-    does not divide a chain's L, delay rank min(32, N), eq measurements every
-    n_matmul slices.  The engines run concurrently."""
+    (dqmc_amd.buckets: inert padding sites, identity padding slices).  Each
+    bucket's n_matmul is autotuned before the timed region
+    (buckets.tune_n_matmul: the fastest of 5, 8, 10, 16, 20 whose wrap drift
+    stays below 1e-3, the drift of the reference's own headline setting;
+    --no-tune: 10, or 8 for N > 200), with a partial last
+    group where it does not divide a chain's L; delay rank min(32, N), eq
+    measurements every n_matmul slices.  The engines run concurrently."""
     from dqmc_amd import buckets, gen
     nch = args.chains if args.chains != 256 else 128
     r = __import__("numpy").random.default_rng(99)
@@ -198,13 +201,20 @@ def bench_config4(args):
         sim = gen.create_batch(Nfiles=1, seed=99 + c, **kw)[0]
         bk = next(i for i, (lo, hi) in enumerate(edges) if lo <= side <= hi)
         groups.setdefault(bk, []).append(sim)
-    engs = []
+    engs, tuned = [], []
     for bk in sorted(groups):
         npad = edges[bk][1] ** 2
+        nm = 10 if npad < 200 else 8
+        if not args.no_tune:
+            nm, table = buckets.tune_n_matmul(groups[bk])
+            tuned.append({"bucket": bk, "n_matmul": nm,
+                          "trials": [[n, round(t, 4), d] for n, t, d in table]})
+            print("config 4: bucket %d n_matmul trials (n, s/sweep, drift): %s -> %d"
+                  % (bk, table, nm), file=sys.stderr, flush=True)
         # the bucket engines run from concurrent host threads: no graph capture
-        engs.append(buckets.PaddedBatch(groups[bk], 10 if npad < 200 else 8, graphs=False))
-        print("config 4: bucket %d: %d chains padded to N=%d L=%d" %
-              (bk, len(groups[bk]), engs[-1].N, engs[-1].L), file=sys.stderr, flush=True)
+        engs.append(buckets.PaddedBatch(groups[bk], nm, graphs=False))
+        print("config 4: bucket %d: %d chains padded to N=%d L=%d, n_matmul %d" %
+              (bk, len(groups[bk]), engs[-1].N, engs[-1].L, nm), file=sys.stderr, flush=True)
     dt, n = _run_engines(engs, args.steps, args.warmup, len(engs))
     for e in engs:
         e.close()
@@ -212,7 +222,9 @@ def bench_config4(args):
                  args.steps, args.warmup, dt,
                  "%d real chains, side 6..20, L 20..100, U in {2,4,8}, mu in [-1.5,0], "
                  "seed 99+chain; %d size buckets (one padded engine each), run concurrently"
-                 % (nch, len(engs)), {"buckets": [[e.nc, e.N, e.L] for e in engs]}, nch)
+                 % (nch, len(engs)),
+                 {"buckets": [[e.nc, e.N, e.L, e.n_matmul] for e in engs], "n_matmul_tuning": tuned},
+                 nch)
 
 
 def bench_config5(args):
@@ -259,6 +271,8 @@ def main():
                     help="workload: 2 = the headline N=256 L=80 batch (default); 1 = 4x4 "
                          "single chain; 3 = complex Peierls N=144; 4 = 128 uneven chains; "
                          "5 = adjoint gradients")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="--config 4: fixed n_matmul per bucket instead of the autotuner")
     ap.add_argument("--threads", type=int, default=8,
                     help="host threads driving the per-shape engines of --config 4")
     args = ap.parse_args()

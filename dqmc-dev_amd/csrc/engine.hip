@@ -212,6 +212,11 @@ struct dqmc_engine {
     bool use_graph = true;
     hipGraph_t graph[4] = {};
     hipGraphExec_t gexec[4] = {};
+    // CHECK_G_WRP (src/dqmc.c:277-287): at every recalculation the wrapped G
+    // (into Gw) against the recomputed one; per chain the largest |difference|
+    bool check_drift = false;
+    void* Gw = 0;
+    unsigned long long* d_drift = 0;  // [nc] bits of a non-negative double
     double gemm_ms = 0.0, gemm_flops = 0.0;
     long long gemm_launches = 0;
 };
@@ -854,6 +859,23 @@ static int launch_update(dqmc_engine* e, int l)
     return 0;
 }
 
+// d[b / 2] <- max(d[b / 2], max |Gw[b] - G[b]|) for active chains; the bits of
+// a non-negative double order as the values, so the maximum is an integer one
+template <typename T>
+__global__ __launch_bounds__(256) void k_drift(int N, const T* Gw, const T* G, long NN,
+                                               const int* active, unsigned long long* d)
+{
+    const int b = blockIdx.y;
+    if (!active[b >> 1]) return;
+    const T* x = Gw + b * NN;
+    const T* y = G + b * NN;
+    double m = 0.0;
+    for (long e = blockIdx.x * 256L + threadIdx.x; e < NN; e += (long)gridDim.x * 256)
+        m = fmax(m, absv(x[e] - y[e]));
+    for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
+    if ((threadIdx.x & 63) == 0) atomicMax(d + (b >> 1), (unsigned long long)__double_as_longlong(m));
+}
+
 // One sweep of every chain (device work only: kernel launches on the engine
 // stream, so that it can be captured into a HIP graph): flags, the sweep's
 // random stream, L slices of updates with wraps or recalculations, the
@@ -882,8 +904,20 @@ static int sweep_body(dqmc_engine* e, bool any_eq, bool any_ue)
         const int f = l / n;
         const bool recalc = ((l + 1) % n == 0);
         if (recalc) {
+            if (e->check_drift) {  // the wrap this slice would have had
+                RC(gemm<T>(e, OP_N, OP_N, N, N, N, one, G, N, NN, dp<T>(e->iexpK), N, NN, zero, T1,
+                           N, NN, B, nullptr, 0, e->iscale + (long)N * l, LN, nullptr, 0, l, 2));
+                RC(gemm<T>(e, OP_N, OP_N, N, N, N, one, dp<T>(e->expK), N, NN, T1, N, NN, zero,
+                           dp<T>(e->Gw), N, NN, B, nullptr, 0, e->escale + (long)N * l, LN, nullptr,
+                           0, l, 1));
+            }
             RC(build_C<T>(e, f, 1));
             RC(calc_eq_g<T>(e, (f + 1) % F, G, dp<T>(e->phb)));
+            if (e->check_drift) {
+                k_drift<T><<<dim3(nblk(NN, 256 * 16), B), 256, 0, e->st>>>(N, dp<T>(e->Gw), G, NN,
+                                                                          e->active, e->d_drift);
+                LCHK();
+            }
             k_phase_combine<T><<<(nc + 255) / 256, 256, 0, e->st>>>(nc, dp<T>(e->phb),
                                                                      dp<T>(e->phase), e->active);
             LCHK();
@@ -1061,6 +1095,10 @@ static int engine_setup(dqmc_engine* e, const dqmc_params* const* P)
     DALLOC(e->phb2, B * es);
     DALLOC(e->phase, nc * es);
     DALLOC(e->nacc, nc * 8);
+    if (e->check_drift) {
+        DALLOC(e->Gw, B * NN * es);
+        DALLOC(e->d_drift, nc * 8);
+    }
     DALLOC(e->raw, (long)nc * 2 * L * N * 8);
     DALLOC(e->order, (long)nc * L * N * 4);
     DALLOC(e->unif, (long)nc * L * N * 8);
@@ -1337,6 +1375,7 @@ extern "C" int dqmc_engine_create(dqmc_engine** out, const dqmc_params* const* P
     if (e->kd > e->N) e->kd = e->N;
     e->time_gemm = tune && (tune->flags & 1);
     e->use_graph = !(tune && (tune->flags & 2));
+    e->check_drift = tune && (tune->flags & 4);
     if (tune && (tune->chain_L || tune->chain_N)) {
         // chains of uneven size, padded by the caller to N and L
         e->ragged = true;
@@ -1550,6 +1589,20 @@ extern "C" int64_t dqmc_engine_accepts(dqmc_engine* e)
     int64_t s = 0;
     for (auto x : v) s += (int64_t)x;
     return s;
+}
+
+extern "C" int dqmc_engine_drift(dqmc_engine* e, double* max_abs)
+{
+    if (!e || !max_abs) return set_err(DQMC_ERR_ARG, "bad arguments");
+    if (!e->check_drift) return set_err(DQMC_ERR_ARG, "engine created without the drift check");
+    HIPCHK(hipSetDevice(e->device));
+    HIPCHK(hipStreamSynchronize(e->st));
+    std::vector<unsigned long long> v(e->nc, 0ull);
+    HIPCHK(hipMemcpy(v.data(), e->d_drift, e->nc * 8, hipMemcpyDeviceToHost));
+    for (int c = 0; c < e->nc; c++) memcpy(&max_abs[c], &v[c], 8);
+    std::fill(v.begin(), v.end(), 0ull);
+    HIPCHK(hipMemcpy(e->d_drift, v.data(), e->nc * 8, hipMemcpyHostToDevice));
+    return 0;
 }
 
 extern "C" int dqmc_engine_profile(dqmc_engine* e, const char** names, double* ms, int max)

@@ -105,9 +105,9 @@ class PaddedBatch:
 
     sims: simulation dicts of one dtype; n_matmul: the engine's group size
     (the last group of a chain whose L it does not divide is partial);
-    graphs: as for :class:`dqmc_amd.engine.Engine`."""
+    graphs, check_drift: as for :class:`dqmc_amd.engine.Engine`."""
 
-    def __init__(self, sims, n_matmul, device=0, gemm_tile=0, graphs=True):
+    def __init__(self, sims, n_matmul, device=0, gemm_tile=0, graphs=True, check_drift=False):
         self.sims = sims
         self.Nc = [_int(s["params"]["N"]) for s in sims]
         self.Lc = [_int(s["params"]["L"]) for s in sims]
@@ -119,6 +119,7 @@ class PaddedBatch:
         ni, nij = max(self.num_i), max(self.num_ij)
         self.padded = [pad_sim(s, self.N, self.L, n_matmul, ni, nij) for s in sims]
         self.engine = _eng.Engine(self.padded, device=device, gemm_tile=gemm_tile, graphs=graphs,
+                                  check_drift=check_drift,
                                   chain_L=self.Lc, chain_N=self.Nc)
 
     @property
@@ -155,3 +156,47 @@ class PaddedBatch:
         G, ph = self.engine.green(c, spin)
         n = self.Nc[c]
         return G[:n, :n].copy(), ph
+
+
+def tune_n_matmul(sims, candidates=(5, 8, 10, 16, 20), tol=1e-3, sweeps=2, device=0):
+    """Stabilization interval for one bucket of chains (no counterpart in the
+    reference, where n_matmul is an input).
+
+    For every candidate n no larger than the shortest chain's L: a trial
+    engine times `sweeps` sweeps after a first one, and a second trial engine
+    with the wrap drift check (the reference's CHECK_G_WRP,
+    ``src/dqmc.c:277-287``) runs two sweeps (the drift grows as the field
+    leaves its random start) and then reports the largest
+    |G_wrapped - G_recomputed| of one more sweep over the chains.  Candidates
+    whose drift exceeds `tol` (absolute on G) are dropped and the fastest
+    remaining one is returned, with the table of (n, seconds per sweep,
+    drift).  If none passes, the smallest candidate is returned.  The default
+    tolerance is the drift of the reference's own headline setting
+    (n_matmul = 10 at U = 6, beta = 8, 16x16: 1e-4 to 1e-3 once equilibrated,
+    for the reference algorithm and this engine alike).  The trial chains
+    are copies: `sims` is unchanged."""
+    import time
+    lmin = min(_int(s["params"]["L"]) for s in sims)
+    cands = [n for n in candidates if n <= lmin] or [min(candidates)]
+    table = []
+    for n in cands:
+        pb = PaddedBatch(sims, n, device=device)
+        try:
+            pb.run(1)
+            t0 = time.perf_counter()
+            pb.run(sweeps)
+            dt = (time.perf_counter() - t0) / sweeps
+        finally:
+            pb.close()
+        pb = PaddedBatch(sims, n, device=device, check_drift=True)
+        try:
+            pb.run(2)
+            pb.engine.drift()
+            pb.run(1)
+            drift = float(np.max(pb.engine.drift()))
+        finally:
+            pb.close()
+        table.append((n, dt, drift))
+    ok = [r for r in table if r[2] <= tol]
+    pick = min(ok, key=lambda r: r[1])[0] if ok else min(cands)
+    return pick, table

@@ -65,6 +65,8 @@ def lib():
         L.dqmc_engine_gemm_tile.restype = C.c_int
         L.dqmc_probe_mfma.argtypes = [C.c_int, P(C.c_double)]
         L.dqmc_probe_mfma.restype = C.c_int
+        L.dqmc_engine_drift.argtypes = [C.c_void_p, P(C.c_double)]
+        L.dqmc_engine_drift.restype = C.c_int
         for f in ("set_state", "get_state", "set_meas", "get_meas", "run", "green", "calc_eq_g",
                   "calc_ue_g", "gemm_stats"):
             getattr(L, "dqmc_engine_" + f).restype = C.c_int
@@ -96,10 +98,14 @@ class Engine:
     graphs: capture each sweep into a HIP graph (one per measurement pattern)
     and replay it.  HIP's stream capture fails while other host threads make
     HIP calls, so engines driven concurrently from several threads need
-    graphs=False.  Engines that time their GEMMs (time_gemm) run eagerly."""
+    graphs=False.  Engines that time their GEMMs (time_gemm) run eagerly.
+
+    check_drift: at every recalculation also wrap G and compare it with the
+    recomputed one (the reference's CHECK_G_WRP, src/dqmc.c:277-287); read
+    with :meth:`drift`."""
 
     def __init__(self, sims, device=0, delay_rank=0, time_gemm=False, gemm_tile=0, chain_L=None,
-                 chain_N=None, graphs=True):
+                 chain_N=None, graphs=True, check_drift=False):
         self.L = lib()
         self.sims = sims
         self.cplx = capi.is_complex(sims[0])
@@ -116,7 +122,8 @@ class Engine:
         # own sizes of chains padded to the common N, L (dqmc_amd.buckets)
         self._cl = None if chain_L is None else (C.c_int * len(sims))(*chain_L)
         self._cn = None if chain_N is None else (C.c_int * len(sims))(*chain_N)
-        tune = DqmcTuning(delay_rank, (1 if time_gemm else 0) | (0 if graphs else 2), gemm_tile,
+        flags = (1 if time_gemm else 0) | (0 if graphs else 2) | (4 if check_drift else 0)
+        tune = DqmcTuning(delay_rank, flags, gemm_tile,
                           C.cast(self._cl, C.POINTER(C.c_int)) if self._cl is not None else None,
                           C.cast(self._cn, C.POINTER(C.c_int)) if self._cn is not None else None)
         h = C.c_void_p()
@@ -126,6 +133,14 @@ class Engine:
         for c, s in enumerate(sims):
             self.set_state(c, s["state"])
             self.set_meas(c, s["meas_eqlt"], s.get("meas_uneqlt") or None)
+
+    def drift(self):
+        """Per chain, the largest |G_wrapped - G_recomputed| over both spins
+        and all recalculations since the previous call (needs check_drift)."""
+        out = np.zeros(len(self.sims))
+        _check(self.L.dqmc_engine_drift(self.h, out.ctypes.data_as(C.POINTER(C.c_double))),
+               "dqmc_engine_drift")
+        return out
 
     def close(self):
         if getattr(self, "h", None):

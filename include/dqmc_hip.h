@@ -117,7 +117,8 @@ typedef struct dqmc_tuning {
                        sweeps then run launch by launch); bit 1: do not capture
                        the sweeps into HIP graphs (needed when other host
                        threads make HIP calls while this engine runs: HIP's
-                       stream capture fails then) */
+                       stream capture fails then); bit 2: the wrap drift check
+                       (dqmc_engine_drift) */
     int gemm_tile;  /* C tile of the MFMA GEMM: 0 = time the shapes at creation
                        and keep the fastest, 1 = 64x64, 2 = 128x64 with K
                        steps of 16, 3 = 128x64 with K steps of 8 */
@@ -218,6 +219,13 @@ int dqmc_test_gemm(int is_complex, int opa, int opb, int M, int N, int K, const 
  * form 1: the 4x4x4 one, which the GEMM tiles use).  Roofline context for
  * the published peak. */
 int dqmc_probe_mfma(int form, double *tflops);
+
+/* Wrap drift (CHECK_G_WRP, src/dqmc.c:277-287; a compile-time check there):
+ * with dqmc_tuning.flags bit 2 the engine also wraps G at every
+ * recalculation slice and compares it with the recomputed G.  max_abs[c] <-
+ * the largest |G_wrapped - G_recomputed| of chain c over both spins since the
+ * previous call (which resets it).  Used to pick the stabilization interval. */
+int dqmc_engine_drift(dqmc_engine *e, double *max_abs);
 
 /* The GEMM tile shape the engine uses (dqmc_tuning.gemm_tile values 1..3). */
 int dqmc_engine_gemm_tile(dqmc_engine *e);

@@ -81,3 +81,47 @@ def test_uneven_sizes_refuse_unequal_time():
                          period_uneqlt=1)[0]
     with pytest.raises(ValueError):
         buckets.PaddedBatch([a], 10)
+
+
+def test_wrap_drift_check_leaves_the_chain_unchanged():
+    # the CHECK_G_WRP comparison only reads G: the chain is the same bit for
+    # bit; the drift (wrap rounding, ~1e-8..1e-7 at n_matmul = 10, U = 4 on
+    # 6x6) is small, grows with the interval, and reading it resets it
+    sims = [_sim(6, 40, 10, 31 + k) for k in range(3)]
+    a = engine.Engine(sims, check_drift=True)
+    b = engine.Engine(sims)
+    try:
+        a.run(3)
+        b.run(3)
+        for c in range(3):
+            sa, sb = a.get_state(c), b.get_state(c)
+            assert np.array_equal(sa["hs"], sb["hs"]) and np.array_equal(sa["rng"], sb["rng"])
+            assert np.array_equal(a.green(c, 0)[0], b.green(c, 0)[0])
+        d = a.drift()
+        print("drift", d, flush=True)
+        assert d.shape == (3,) and np.all(d > 0) and np.all(d < 1e-5)
+        assert np.all(a.drift() == 0.0)
+    finally:
+        a.close()
+        b.close()
+    wide = engine.Engine([_sim(6, 40, 20, 31, U=8.0)], check_drift=True)
+    narrow = engine.Engine([_sim(6, 40, 5, 31, U=8.0)], check_drift=True)
+    try:
+        wide.run(2)
+        narrow.run(2)
+        dw, dn = wide.drift()[0], narrow.drift()[0]
+        print("drift n=20 %.3g, n=5 %.3g" % (dw, dn), flush=True)
+        assert dw > dn > 0
+    finally:
+        wide.close()
+        narrow.close()
+
+
+def test_n_matmul_autotuner_picks_a_stable_interval():
+    sims = [_sim(4, 20, 10, 41), _sim(5, 40, 10, 42), _sim(6, 30, 10, 43)]
+    pick, table = buckets.tune_n_matmul(sims, candidates=(5, 10, 20, 40), sweeps=1)
+    print("table", table, "pick", pick, flush=True)
+    assert [n for n, _, _ in table] == [5, 10, 20]  # 40 > the shortest L
+    assert all(t > 0 and d > 0 for _, t, d in table)
+    ok = [(n, t) for n, t, d in table if d <= 1e-3]
+    assert pick == (min(ok, key=lambda x: x[1])[0] if ok else 5)
