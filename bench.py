@@ -2,9 +2,9 @@
 
 Workload (the reference's headline configuration): 16x16 periodic square
 Hubbard model (N=256), t=1, t'=-0.25, U=6, mu=-1, beta=8, dt=0.1 (L=80),
-n_matmul=10, delay rank 32 requested (the N <= 256 Metropolis kernel keeps 16
-register-resident delay slots; the rank changes only the rounding of G, not
-the Markov chain), equal-time measurements every 8 slices,
+n_matmul=10, delay rank 32 (the N <= 256 Metropolis kernel keeps the rank-32
+A factor in registers and B in LDS; the rank changes only the rounding of G,
+not the Markov chain), equal-time measurements every 8 slices,
 unequal-time G(t,0)/G(0,t) + susceptibilities every 2 sweeps, real FP64.
 A "step" is one full sweep (L slices of Metropolis updates + wraps/recalcs +
 measurements) of every chain in the batch.  `value` is the aggregate
@@ -28,8 +28,8 @@ sys.path.insert(0, os.path.join(ROOT, "dqmc-dev_amd"))
 PEAK_FP64_TFLOPS = 78.6  # MI355X dense FP64 (matrix = vector)
 # HBM bytes per 256 x 256 x 256 matrix product of the dominant GEMM, from the
 # FETCH_SIZE / WRITE_SIZE counter passes scaled by a 512 MiB copy
-# (profiles/k_gemm_N256_L80_256chains.md): 1.0012 x the 3 x 512 KiB it must move.
-GEMM_HBM_BYTES_PER_MATRIX_N256 = 1574810
+# (profiles/k_gemm_N256_L80_256chains.md): 1.0007 x the 3 x 512 KiB it must move.
+GEMM_HBM_BYTES_PER_MATRIX_N256 = 1573916
 
 
 def workload(nchains, seed=7):
@@ -345,7 +345,7 @@ def main():
         "dtype": "f64",
         "data": "synthetic (package generator, seed 7 + jump per chain)",
         "config": {"workload": "16x16 square Hubbard N=256 L=80 t'=-0.25 U=6 mu=-1 beta=8 "
-                               "n_matmul=10 delay=32 (16 register slots) eq every 8 slices, "
+                               "n_matmul=10 delay rank 32 (A in registers, B in LDS) eq every 8 slices, "
                                "uneq every 2 sweeps",
                    "chains": nch, "parallelism": "1 GPU, chains batched"},
         "single_chain_sweeps_per_s": None if single is None else round(single, 4),
