@@ -157,7 +157,7 @@ def bench_config3(args):
     eng = engine.Engine(gen.create_batch(Nfiles=1, seed=3, **kw))
     dt1, n1 = _run_engines([eng], max(2, args.steps), 1, 1)
     eng.close()
-    nb = args.chains if args.chains != 256 else 128
+    nb = args.chains  # 256: one chain (one Metropolis workgroup) per CU
     engb = engine.Engine(gen.create_batch(Nfiles=nb, seed=3, **kw))
     dtb, nbs = _run_engines([engb], args.steps, args.warmup, 1)
     engb.close()
