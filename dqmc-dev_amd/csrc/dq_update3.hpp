@@ -22,37 +22,7 @@
 
 #define UPD3_THREADS 256
 
-// G[j][c] += sum_m A[j][m] B[m][c] for the thread's row j (j < N), all c;
-// returns the new G[j][j].  FB columns of the row stay in flight at once.
-template <typename T, int KD, int W>
-__device__ inline T upd3_flush(int N, int j, T* G, const T (&A)[KD], const T (*B)[W], int kused)
-{
-    constexpr int FB = 8;
-    T gjj = from_re<T>(0.0);
-    for (int c0 = 0; c0 < N; c0 += FB) {
-        T g[FB];
-#pragma unroll
-        for (int t = 0; t < FB; t++)
-            if (c0 + t < N) g[t] = G[j + (long)N * (c0 + t)];
-#pragma unroll
-        for (int m = 0; m < KD; m++) {
-            if (m < kused) {  // uniform
-#pragma unroll
-                for (int t = 0; t < FB; t++) g[t] += A[m] * B[m][(c0 + t) & (W - 1)];
-            }
-        }
-#pragma unroll
-        for (int t = 0; t < FB; t++) {
-            if (c0 + t < N) {
-                G[j + (long)N * (c0 + t)] = g[t];
-                if (c0 + t == j) gjj = g[t];
-            }
-        }
-    }
-    return gjj;
-}
-
-// The same flush on the FP64 matrix cores (real builds): per wave the rows
+// The rank-k flush G += A B^T on the FP64 matrix cores (real builds): per wave the rows
 // row0 + [64w, 64w + 64) and all columns in chunks of 32, as the transposed
 // product
 //   G^T[c][r] += sum_m B[m][c] A[r][m]
@@ -61,9 +31,9 @@ __device__ inline T upd3_flush(int N, int j, T* G, const T (&A)[KD], const T (*B
 // is gathered once per flush from the owning lanes' registers, and lane l of
 // the result holds G[r0 + l%16][c0 + 4e + l/16], so 16 lanes move 128
 // contiguous bytes of the column-major G.  The diagonal goes to d[].
-// (Operand layouts as in dq_gemm.hpp.)  The VALU flush above needs one LDS
-// broadcast per FMA and is bound by the LDS return path; this one needs one
-// LDS read per 64 FMAs per lane.
+// (Operand layouts as in dq_gemm.hpp.)  A VALU flush (thread j updating its
+// row) needs one LDS broadcast of B per FMA and is bound by the LDS return
+// path; this one needs one LDS read per 64 FMAs per lane.
 template <int KD, int W>
 __device__ inline void upd3_flush_mfma(int N, int j, int row0, double* G, const double (&A)[KD],
                                        const double (*B)[W], int kused, double* d)
@@ -122,6 +92,76 @@ __device__ inline void upd3_flush_mfma(int N, int j, int row0, double* G, const 
                         if (r == cc) d[r] = acc[mb][b][e];
                     }
                 }
+    }
+}
+
+// The complex flush on the matrix cores: as upd3_flush_mfma with the real and
+// imaginary parts as separate real operands, four MFMAs per block,
+//   re += Br Ar - Bi Ai,   im += Br Ai + Bi Ar,
+// one 16-column block per chunk (the accumulators are twice the real ones).
+template <int KD, int W>
+__device__ inline void upd3_flush_mfma_z(int N, int j, int row0, zd* G, const zd (&A)[KD],
+                                         const zd (*B)[W], int kused, zd* d)
+{
+    constexpr int KB = KD / 4;
+    const int lane = j & 63, w = j >> 6;
+    double ar[4][KB], ai[4][KB];
+#pragma unroll
+    for (int b = 0; b < 4; b++) {
+        const int src = 16 * b + (lane & 15);
+#pragma unroll
+        for (int kb = 0; kb < KB; kb++) {
+            double vr = 0.0, vi = 0.0;
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const double xr = __shfl(A[4 * kb + q].x, src, 64);
+                const double xi = __shfl(A[4 * kb + q].y, src, 64);
+                if ((lane >> 4) == q) {
+                    vr = xr;
+                    vi = xi;
+                }
+            }
+            ar[b][kb] = vr;
+            ai[b][kb] = vi;
+        }
+    }
+    const int rl = row0 + 64 * w + (lane & 15);
+    for (int c0 = 0; c0 < N; c0 += 16) {
+        dbl4 cr[4], ci[4];
+#pragma unroll
+        for (int b = 0; b < 4; b++)
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                const int r = rl + 16 * b, cc = c0 + 4 * e + (lane >> 4);
+                const zd g = (r < N && cc < N) ? G[r + (long)N * cc] : mkz(0.0, 0.0);
+                cr[b][e] = g.x;
+                ci[b][e] = g.y;
+            }
+#pragma unroll
+        for (int kb = 0; kb < KB; kb++) {
+            if (4 * kb < kused) {  // uniform
+                const zd bz = B[4 * kb + (lane >> 4)][(c0 + (lane & 15)) & (W - 1)];
+                const double br = bz.x, bi = bz.y;
+#pragma unroll
+                for (int b = 0; b < 4; b++) {
+                    cr[b] = mfma(br, ar[b][kb], cr[b]);
+                    cr[b] = mfma(-bi, ai[b][kb], cr[b]);
+                    ci[b] = mfma(br, ai[b][kb], ci[b]);
+                    ci[b] = mfma(bi, ar[b][kb], ci[b]);
+                }
+            }
+        }
+#pragma unroll
+        for (int b = 0; b < 4; b++)
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                const int r = rl + 16 * b, cc = c0 + 4 * e + (lane >> 4);
+                if (r < N && cc < N) {
+                    const zd g = mkz(cr[b][e], ci[b][e]);
+                    G[r + (long)N * cc] = g;
+                    if (r == cc) d[r] = g;
+                }
+            }
     }
 }
 
@@ -266,11 +306,9 @@ __global__ __launch_bounds__(NT) void k_update_lds(Upd2Args<T> a)
 #pragma unroll
             for (int r = 0; r < R; r++) {
                 const int j = tid + NT * r;
-                if constexpr (is_cplx<T>::value) {
-                    if (j < N) {
-                        du[j] = upd3_flush<T, KD, W>(N, j, Gu, Au[r], Bu, k);
-                        dd[j] = upd3_flush<T, KD, W>(N, j, Gd, Ad[r], Bd, k);
-                    }
+                if constexpr (is_cplx<T>::value) {  // every lane takes part
+                    upd3_flush_mfma_z<KD, W>(N, tid, NT * r, Gu, Au[r], Bu, k, du);
+                    upd3_flush_mfma_z<KD, W>(N, tid, NT * r, Gd, Ad[r], Bd, k, dd);
                 } else {  // every lane takes part (the A fragments are gathered)
                     upd3_flush_mfma<KD, W>(N, tid, NT * r, Gu, Au[r], Bu, k, du);
                     upd3_flush_mfma<KD, W>(N, tid, NT * r, Gd, Ad[r], Bd, k, dd);
