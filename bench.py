@@ -56,20 +56,56 @@ def flop_model(N, L, F, n_matmul, accepts_per_sweep, n_delay, q_eq, ue_fraction)
     return 2 * per_spin * N3
 
 
-def cpu_baseline(seconds_budget=20.0):
-    """The oracle (CPU restatement, MKL sequential, 2 OpenMP threads -- one per
-    spin -- as the reference's design point) on a bounded sample of the same
-    workload: one chain, a few sweeps including one unequal-time sweep."""
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import oracle
-    sim = workload(1)[0]
-    # sample: 8 sweeps (four with the unequal-time measurement)
+_CPU_CHAIN = r"""
+import os, sys, time, json
+sys.path[:0] = [os.path.join(%(root)r, "dqmc-dev_amd"), os.path.join(%(root)r, "oracle"), %(root)r]
+import bench, oracle
+sim = bench.workload(%(nproc)d)[%(k)d]
+while time.time() < %(start)r:  # all chains start together
+    time.sleep(0.01)
+t0 = time.time()
+out = oracle.run(sim, max_sweeps=%(sweeps)d)
+print(json.dumps({"sweeps": out["sweeps_run"], "seconds": time.time() - t0}))
+"""
+
+
+def cpu_baseline(sweeps=4, cpu_share=16):
+    """The oracle (CPU restatement, MKL sequential, 2 OpenMP threads per chain
+    -- one per spin, the reference's design point) as the reference's stack
+    mode runs it: independent chains in concurrent processes, one per two of
+    the host's cores (the box's CPU share, at most the cores visible), each a
+    bounded sample of `sweeps` sweeps of the same workload (every second one
+    with the unequal-time measurement, plus the initial G).  Started as child
+    processes that begin their sweeps at a common time; the aggregate is the
+    sum of the chains' rates, each measured while all of them run."""
+    import json
+    import subprocess
+    try:
+        ncpu = len(os.sched_getaffinity(0))
+    except AttributeError:
+        ncpu = os.cpu_count() or 2
+    nproc = max(1, min(ncpu, cpu_share) // 2)
+    env = dict(os.environ, OMP_NUM_THREADS="2", MKL_NUM_THREADS="1")
     t0 = time.time()
-    out = oracle.run(sim, max_sweeps=8)
-    dt = time.time() - t0
-    return {"value": out["sweeps_run"] / dt, "unit": "sweeps/s", "cores": 2, "kind": "port",
-            "sample": "1 chain x %d sweeps (every 2nd with unequal-time), incl. initial G; %.1f s"
-                      % (out["sweeps_run"], dt)}
+    start = t0 + 8.0  # after every child has imported numpy and built its input
+    procs = [subprocess.Popen([sys.executable, "-c", _CPU_CHAIN % dict(root=ROOT, nproc=nproc, k=k,
+                                                                       sweeps=sweeps, start=start)],
+                              stdout=subprocess.PIPE, env=env, text=True)
+             for k in range(nproc)]
+    res = []
+    for pr in procs:
+        outp, _ = pr.communicate()
+        if pr.returncode != 0:
+            raise RuntimeError("CPU baseline process failed")
+        res.append(json.loads(outp.strip().splitlines()[-1]))
+    rates = [r["sweeps"] / r["seconds"] for r in res]
+    single = sum(rates) / len(rates)
+    return {"value": sum(rates), "unit": "sweeps/s", "cores": 2 * nproc, "kind": "port",
+            "sample": "%d chains x %d sweeps (every 2nd with unequal-time, incl. initial G) in %d "
+                      "concurrent processes of 2 OpenMP threads (stack mode), %.1f s each; "
+                      "per chain %.3f sweeps/s" % (nproc, sweeps, nproc,
+                                                   max(r["seconds"] for r in res), single),
+            "single_chain_value": round(single, 4)}
 
 
 # --------------------------------------------------------------------------
