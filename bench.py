@@ -56,6 +56,15 @@ def flop_model(N, L, F, n_matmul, accepts_per_sweep, n_delay, q_eq, ue_fraction)
     return 2 * per_spin * N3
 
 
+def _model(flops_per_sweep, sweeps_per_s, prefix=""):
+    """Model FLOP rate of a line: algorithmic flops per chain-sweep (flop_model)
+    times chain-sweeps/s, and its fraction of the FP64 matrix peak."""
+    tf = flops_per_sweep * sweeps_per_s / 1e12
+    return {prefix + "model_gflop_per_sweep": round(flops_per_sweep / 1e9, 4),
+            prefix + "model_tflops": round(tf, 4),
+            prefix + "model_frac_fp64_peak": round(tf / PEAK_FP64_TFLOPS, 5)}
+
+
 _CPU_CHAIN = r"""
 import os, sys, time, json
 sys.path[:0] = [os.path.join(%(root)r, "dqmc-dev_amd"), os.path.join(%(root)r, "oracle"), %(root)r]
@@ -175,11 +184,15 @@ def bench_config1(args):
     engb = engine.Engine(sims)
     dtb, nbs = _run_engines([engb], args.steps * 25, args.warmup, 1)
     engb.close()
+    # F = 5, eq every 8 slices (5 per sweep), no unequal-time
+    fl = flop_model(16, 40, 5, 8, 0.5 * 16 * 40, 8, 5, 0.0)
     return _line("DQMC sweeps/s, 4x4 Hubbard N=16 L=40 FP64 (single chain)", n1 / dt1,
                  "sweeps/s", 400, 100, dt1,
                  "4x4 square Hubbard N=16 L=40 U=4 mu=0 beta=4 n_matmul=8 delay=8 seed 1234, "
-                 "100 warmup + 400 measurement sweeps, eq every 8 slices", 
-                 {"batched_chains": nb, "batched_sweeps_per_s": round(nbs / dtb, 2)}, 1)
+                 "100 warmup + 400 measurement sweeps, eq every 8 slices",
+                 dict({"batched_chains": nb, "batched_sweeps_per_s": round(nbs / dtb, 2),
+                       "model_acceptance_assumed": 0.5},
+                      **_model(fl, n1 / dt1), **_model(fl, nbs / dtb, "batched_")), 1)
 
 
 def bench_config3(args):
@@ -202,8 +215,14 @@ def bench_config3(args):
                  "12x12 square Hubbard, nflux=1 (Landau gauge Peierls phases), complex128, "
                  "N=144 L=60 U=4 mu=-0.5 beta=6 n_matmul=10 delay=16, eq every 10 slices, "
                  "uneq + bond correlators every 2 sweeps",
-                 {"single_chain_sweeps_per_s": round(n1 / dt1, 4),
-                  "batched_over_single": round(nbs / dtb / (n1 / dt1), 3), "dtype": "c128"}, nb)
+                 dict({"single_chain_sweeps_per_s": round(n1 / dt1, 4),
+                       "batched_over_single": round(nbs / dtb / (n1 / dt1), 3), "dtype": "c128",
+                       "model_acceptance_assumed": 0.5},
+                      # complex arithmetic: 4 real flops per complex multiply-add pair;
+                      # F = 6, eq every 10 slices, unequal-time every 2 sweeps (bond
+                      # correlators not counted)
+                      **_model(4 * flop_model(144, 60, 6, 10, 0.5 * 144 * 60, 16, 6, 0.5),
+                               nbs / dtb)), nb)
 
 
 def bench_config4(args):
@@ -252,6 +271,15 @@ def bench_config4(args):
         print("config 4: bucket %d: %d chains padded to N=%d L=%d, n_matmul %d" %
               (bk, len(groups[bk]), engs[-1].N, engs[-1].L, nm), file=sys.stderr, flush=True)
     dt, n = _run_engines(engs, args.steps, args.warmup, len(engs))
+    # each chain's own work (its N, L, the bucket's n_matmul, eq every n_matmul
+    # slices); padding is overhead, not counted
+    fl = 0.0
+    for e in engs:
+        for Nc, Lc in zip(e.Nc, e.Lc):
+            Fc = -(-Lc // e.n_matmul)
+            fl += flop_model(Nc, Lc, Fc, e.n_matmul, 0.5 * Nc * Lc, min(32, Nc),
+                             Lc // e.n_matmul, 0.0)
+    fl /= nch
     for e in engs:
         e.close()
     return _line("DQMC sweeps/s, 128 uneven real chains (aggregate)", n / dt, "sweeps/s",
@@ -259,8 +287,9 @@ def bench_config4(args):
                  "%d real chains, side 6..20, L 20..100, U in {2,4,8}, mu in [-1.5,0], "
                  "seed 99+chain; %d size buckets (one padded engine each), run concurrently"
                  % (nch, len(engs)),
-                 {"buckets": [[e.nc, e.N, e.L, e.n_matmul] for e in engs], "n_matmul_tuning": tuned},
-                 nch)
+                 dict({"buckets": [[e.nc, e.N, e.L, e.n_matmul] for e in engs],
+                       "n_matmul_tuning": tuned, "model_acceptance_assumed": 0.5},
+                      **_model(fl, n / dt)), nch)
 
 
 def bench_config5(args):
