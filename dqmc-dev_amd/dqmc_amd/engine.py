@@ -225,7 +225,8 @@ class Engine:
         return int(self.L.dqmc_engine_accepts(self.h))
 
     def gemm_tile(self):
-        """GEMM C-tile shape in use (1 = 64x64, 2 = 128x128, 3 = 128x64)."""
+        """GEMM C-tile shape in use (1 = 64x64, 2 = 128x64 with K steps of 16,
+        3 = 128x64 with K steps of 8)."""
         return self.L.dqmc_engine_gemm_tile(self.h)
 
     def gemm_stats(self):
