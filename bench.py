@@ -176,12 +176,12 @@ def bench_config1(args):
     kw = dict(Nx=4, Ny=4, tp=0.0, U=4.0, mu=0.0, dt=0.1, L=40, n_matmul=8, n_delay=8,
               period_eqlt=8, n_sweep_warm=100, n_sweep_meas=400)
     sim = gen.create_batch(Nfiles=1, seed=1234, **kw)
-    eng = engine.Engine(sim)
+    eng = engine.Engine(sim, graphs=not args.no_graphs)
     dt1, n1 = _run_engines([eng], 400, 100, 1)
     eng.close()
     nb = args.chains if args.chains != 256 else 1024
     sims = gen.create_batch(Nfiles=nb, seed=1234, **kw)
-    engb = engine.Engine(sims)
+    engb = engine.Engine(sims, graphs=not args.no_graphs)
     dtb, nbs = _run_engines([engb], args.steps * 25, args.warmup, 1)
     engb.close()
     # F = 5, eq every 8 slices (5 per sweep), no unequal-time
@@ -203,11 +203,11 @@ def bench_config3(args):
     kw = dict(Nx=12, Ny=12, tp=0.0, U=4.0, mu=-0.5, dt=0.1, L=60, nflux=1, n_matmul=10,
               n_delay=16, period_eqlt=10, period_uneqlt=2, n_sweep_warm=0,
               n_sweep_meas=100000, meas_bond_corr=1)
-    eng = engine.Engine(gen.create_batch(Nfiles=1, seed=3, **kw))
+    eng = engine.Engine(gen.create_batch(Nfiles=1, seed=3, **kw), graphs=not args.no_graphs)
     dt1, n1 = _run_engines([eng], max(2, args.steps), 1, 1)
     eng.close()
     nb = args.chains  # 256: one chain (one Metropolis workgroup) per CU
-    engb = engine.Engine(gen.create_batch(Nfiles=nb, seed=3, **kw))
+    engb = engine.Engine(gen.create_batch(Nfiles=nb, seed=3, **kw), graphs=not args.no_graphs)
     dtb, nbs = _run_engines([engb], args.steps, args.warmup, 1)
     engb.close()
     return _line("DQMC sweeps/s, complex Peierls 12x12 N=144 L=60 (batched chains, aggregate)",
@@ -261,7 +261,7 @@ def bench_config4(args):
         npad = edges[bk][1] ** 2
         nm = 10 if npad < 200 else 8
         if not args.no_tune:
-            nm, table = buckets.tune_n_matmul(groups[bk])
+            nm, table = buckets.tune_n_matmul(groups[bk], graphs=not args.no_graphs)
             tuned.append({"bucket": bk, "n_matmul": nm,
                           "trials": [[n, round(t, 4), d] for n, t, d in table]})
             print("config 4: bucket %d n_matmul trials (n, s/sweep, drift): %s -> %d"
@@ -336,6 +336,9 @@ def main():
                     help="workload: 2 = the headline N=256 L=80 batch (default); 1 = 4x4 "
                          "single chain; 3 = complex Peierls N=144; 4 = 128 uneven chains; "
                          "5 = adjoint gradients")
+    ap.add_argument("--no-graphs", action="store_true",
+                    help="launch the sweeps kernel by kernel instead of replaying HIP graphs "
+                         "(for profilers: rocprofv3 --kernel-trace crashed on graph launches)")
     ap.add_argument("--no-tune", action="store_true",
                     help="--config 4: fixed n_matmul per bucket instead of the autotuner")
     ap.add_argument("--threads", type=int, default=8,
@@ -366,7 +369,8 @@ def main():
 
     def timed(nch, steps, warmup, time_gemm):
         sims = workload(nch)
-        eng = engine.Engine(sims, device=0, time_gemm=time_gemm, gemm_tile=args.gemm_tile)
+        eng = engine.Engine(sims, device=0, time_gemm=time_gemm, gemm_tile=args.gemm_tile,
+                            graphs=not args.no_graphs)
         tile = eng.gemm_tile()
         eng.run(0)  # initial G outside the timed region
         eng.run(warmup)

@@ -158,7 +158,8 @@ class PaddedBatch:
         return G[:n, :n].copy(), ph
 
 
-def tune_n_matmul(sims, candidates=(5, 8, 10, 16, 20), tol=1e-3, sweeps=2, device=0):
+def tune_n_matmul(sims, candidates=(5, 8, 10, 16, 20), tol=1e-3, sweeps=2, device=0,
+                  graphs=True):
     """Stabilization interval for one bucket of chains (no counterpart in the
     reference, where n_matmul is an input).
 
@@ -180,7 +181,7 @@ def tune_n_matmul(sims, candidates=(5, 8, 10, 16, 20), tol=1e-3, sweeps=2, devic
     cands = [n for n in candidates if n <= lmin] or [min(candidates)]
     table = []
     for n in cands:
-        pb = PaddedBatch(sims, n, device=device)
+        pb = PaddedBatch(sims, n, device=device, graphs=graphs)
         try:
             pb.run(1)
             t0 = time.perf_counter()
@@ -188,7 +189,7 @@ def tune_n_matmul(sims, candidates=(5, 8, 10, 16, 20), tol=1e-3, sweeps=2, devic
             dt = (time.perf_counter() - t0) / sweeps
         finally:
             pb.close()
-        pb = PaddedBatch(sims, n, device=device, check_drift=True)
+        pb = PaddedBatch(sims, n, device=device, check_drift=True, graphs=graphs)
         try:
             pb.run(2)
             pb.engine.drift()
