@@ -312,6 +312,7 @@ def bench_config5(args):
         gt = float(np.sum(gK * prob.t_pattern))
         gtp = float(np.sum(gK * prob.tp_pattern))
     dt = time.perf_counter() - t0
+    prob.close()
     return _line("DQMC adjoint evaluations/s, 10x10 N=100 L=50 (forward + gradients per HS field)",
                  nf * reps / dt, "fields/s", reps, 1, dt,
                  "10x10 square Hubbard N=100 L=50 t'=-0.2 U=4 mu=-0.3 beta=5, 64 HS fields "

@@ -1042,6 +1042,27 @@ static void csr_of_map(const int32_t* map, int n_in, int nbins, std::vector<int>
         if (map[q] >= 0 && map[q] < nbins) ix[fill[map[q]]++] = q;
 }
 
+// chain c's numeric parameters (the exponentials, exp_lambda, Delta and the
+// Peierls phases) onto the engine stream
+static int upload_numeric(dqmc_engine* e, int c, const dqmc_params* p)
+{
+    const long N = e->N, NN = N * N;
+    const size_t mb = NN * e->esz;
+    const void* src[8] = {p->exp_Ku, p->exp_Kd, p->inv_exp_Ku, p->inv_exp_Kd,
+                          p->exp_halfKu, p->exp_halfKd, p->inv_exp_halfKu, p->inv_exp_halfKd};
+    void* dst[4] = {e->expK, e->iexpK, e->ehalf, e->iehalf};
+    for (int k = 0; k < 8; k++)
+        HIPCHK(hipMemcpyAsync((char*)dst[k / 2] + (2 * c + (k & 1)) * mb, src[k], mb,
+                              hipMemcpyHostToDevice, e->st));
+    HIPCHK(hipMemcpyAsync(e->exp_lambda + c * 2L * N, p->exp_lambda, 2L * N * 8, hipMemcpyHostToDevice, e->st));
+    HIPCHK(hipMemcpyAsync(e->del + c * 2L * N, p->del, 2L * N * 8, hipMemcpyHostToDevice, e->st));
+    if (e->cplx) {
+        HIPCHK(hipMemcpyAsync((char*)e->peierlsu + c * mb, p->peierlsu, mb, hipMemcpyHostToDevice, e->st));
+        HIPCHK(hipMemcpyAsync((char*)e->peierlsd + c * mb, p->peierlsd, mb, hipMemcpyHostToDevice, e->st));
+    }
+    return 0;
+}
+
 static int engine_setup(dqmc_engine* e, const dqmc_params* const* P)
 {
     const dqmc_params* p0 = P[0];
@@ -1193,20 +1214,8 @@ static int engine_setup(dqmc_engine* e, const dqmc_params* const* P)
     for (int c = 0; c < nc; c++) {
         const dqmc_params* p = P[c];
         const size_t mb = NN * es;
-        HIPCHK(hipMemcpyAsync((char*)e->expK + (2 * c) * mb, p->exp_Ku, mb, hipMemcpyHostToDevice, e->st));
-        HIPCHK(hipMemcpyAsync((char*)e->expK + (2 * c + 1) * mb, p->exp_Kd, mb, hipMemcpyHostToDevice, e->st));
-        HIPCHK(hipMemcpyAsync((char*)e->iexpK + (2 * c) * mb, p->inv_exp_Ku, mb, hipMemcpyHostToDevice, e->st));
-        HIPCHK(hipMemcpyAsync((char*)e->iexpK + (2 * c + 1) * mb, p->inv_exp_Kd, mb, hipMemcpyHostToDevice, e->st));
-        HIPCHK(hipMemcpyAsync((char*)e->ehalf + (2 * c) * mb, p->exp_halfKu, mb, hipMemcpyHostToDevice, e->st));
-        HIPCHK(hipMemcpyAsync((char*)e->ehalf + (2 * c + 1) * mb, p->exp_halfKd, mb, hipMemcpyHostToDevice, e->st));
-        HIPCHK(hipMemcpyAsync((char*)e->iehalf + (2 * c) * mb, p->inv_exp_halfKu, mb, hipMemcpyHostToDevice, e->st));
-        HIPCHK(hipMemcpyAsync((char*)e->iehalf + (2 * c + 1) * mb, p->inv_exp_halfKd, mb, hipMemcpyHostToDevice, e->st));
-        HIPCHK(hipMemcpyAsync(e->exp_lambda + c * 2L * N, p->exp_lambda, 2L * N * 8, hipMemcpyHostToDevice, e->st));
-        HIPCHK(hipMemcpyAsync(e->del + c * 2L * N, p->del, 2L * N * 8, hipMemcpyHostToDevice, e->st));
-        if (e->cplx) {
-            HIPCHK(hipMemcpyAsync((char*)e->peierlsu + c * mb, p->peierlsu, mb, hipMemcpyHostToDevice, e->st));
-            HIPCHK(hipMemcpyAsync((char*)e->peierlsd + c * mb, p->peierlsd, mb, hipMemcpyHostToDevice, e->st));
-        }
+        (void)mb;
+        RC(upload_numeric(e, c, p));
         csr_of_map(p->map_i, N, e->num_i, rp, ix);
         HIPCHK(hipMemcpy(e->rp_i + c * (e->num_i + 1L), rp.data(), rp.size() * 4, hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(e->ix_i + c * (long)N, ix.data(), ix.size() * 4, hipMemcpyHostToDevice));
@@ -1319,6 +1328,25 @@ static int autotune_gemm(dqmc_engine* e)
     hipEventDestroy(a1);
     e->gemm_tile = pick;
     return rc;
+}
+
+extern "C" int dqmc_engine_set_params(dqmc_engine* e, int c, const dqmc_params* p)
+{
+    if (!e || !p || c < 0 || c >= e->nc) return set_err(DQMC_ERR_ARG, "bad arguments");
+    if (p->N != e->N || p->L != e->L || p->n_matmul != e->n_matmul ||
+        (p->is_complex != 0) != (e->cplx != 0))
+        return set_err(DQMC_ERR_ARG, "parameters of another shape than the engine's");
+    if (e->ragged) return set_err(DQMC_ERR_UNSUPPORTED, "padded chains keep their parameters");
+    HIPCHK(hipSetDevice(e->device));
+    HIPCHK(hipStreamSynchronize(e->st));
+    RC(upload_numeric(e, c, p));
+    // sites that consume a uniform (Delta's not both zero, src/updates.c:21)
+    int nd = 0;
+    for (int i = 0; i < e->N; i++) nd += !(p->del[i] == 0.0 && p->del[i + e->N] == 0.0);
+    HIPCHK(hipMemcpyAsync(e->draws + c, &nd, 4, hipMemcpyHostToDevice, e->st));
+    HIPCHK(hipStreamSynchronize(e->st));
+    e->stale = true;  // G is recomputed before the next sweep
+    return 0;
 }
 
 extern "C" int dqmc_engine_adjoint(dqmc_engine* e, double* logdet, double* W, double* trG,

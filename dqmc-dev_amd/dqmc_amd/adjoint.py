@@ -30,6 +30,29 @@ from . import engine as _eng
 from . import gen
 
 
+class _SymExp:
+    """exp(c A) and the Frechet derivative of exp at A for a symmetric A,
+    from one eigendecomposition A = V diag(a) V^T: exp(c A) = V diag(e^{c a})
+    V^T, and (Daleckii-Krein) L(A, Z) = V (Phi o V^T Z V) V^T with the divided
+    differences Phi_ij = (e^{a_i} - e^{a_j}) / (a_i - a_j), e^{a_i} on ties."""
+
+    def __init__(self, A):
+        self.A = A
+        self.a, self.V = np.linalg.eigh(A)
+
+    def exp(self, c=1.0):
+        return (self.V * np.exp(c * self.a)) @ self.V.T
+
+    def frechet(self, Z):
+        a, V = self.a, self.V
+        d = a[:, None] - a[None, :]
+        small = np.abs(d) < 1e-8
+        dd = np.where(small, 1.0, d)
+        # e^{a_j} expm1(d) / d, its Taylor form for near ties
+        phi = np.exp(a)[None, :] * np.where(small, 1.0 + 0.5 * d + d * d / 6.0, np.expm1(dd) / dd)
+        return V @ (phi * (V.T @ Z @ V)) @ V.T
+
+
 def lambda_of_U(U, dt):
     """HS coupling of the reference's cosh form and d lambda / d U."""
     a = np.exp(0.5 * U * dt) + np.sqrt(np.expm1(U * dt))
@@ -54,6 +77,13 @@ class AdjointProblem:
                                   n_sweep_meas=1)
         self.t_pattern = gen.hopping_square(Nx, Ny, t=1.0, tp=0.0)[0].real
         self.tp_pattern = gen.hopping_square(Nx, Ny, t=0.0, tp=1.0)[0].real
+        self._eng = None  # built on the first evaluation, then reused
+        self._sym = None  # eigendecomposition of -dt K_s of the last evaluation
+
+    def close(self):
+        if self._eng is not None:
+            self._eng.close()
+            self._eng = None
 
     def hopping(self, t=1.0, tp=0.0):
         return t * self.t_pattern + tp * self.tp_pattern
@@ -61,12 +91,18 @@ class AdjointProblem:
     def _sims(self, K, mu, lam):
         N, dt = self.N, self.dt
         Ks = np.asarray(K, dtype=np.float64) - mu * np.eye(N)
-        E = expm(-dt * Ks)
+        # a symmetric hopping matrix (every lattice the generator builds):
+        # one eigendecomposition gives the four exponentials and, in the
+        # chain rule, the Frechet derivative
+        self._sym = _SymExp(-dt * Ks) if np.array_equal(Ks, Ks.T) else None
+        ex = self._sym.exp if self._sym is not None else (lambda c: expm(-c * dt * Ks))
+        E = ex(1.0)
+        self._Einv = ex(-1.0)  # E^-1, reused by the chain rule
         base = dict(self._tmpl["params"])
         base["exp_Ku"] = base["exp_Kd"] = np.ascontiguousarray(E.T)
-        base["inv_exp_Ku"] = base["inv_exp_Kd"] = np.ascontiguousarray(expm(dt * Ks).T)
-        base["exp_halfKu"] = base["exp_halfKd"] = np.ascontiguousarray(expm(-0.5 * dt * Ks).T)
-        base["inv_exp_halfKu"] = base["inv_exp_halfKd"] = np.ascontiguousarray(expm(0.5 * dt * Ks).T)
+        base["inv_exp_Ku"] = base["inv_exp_Kd"] = np.ascontiguousarray(self._Einv.T)
+        base["exp_halfKu"] = base["exp_halfKd"] = np.ascontiguousarray(ex(0.5).T)
+        base["inv_exp_halfKu"] = base["inv_exp_halfKd"] = np.ascontiguousarray(ex(-0.5).T)
         el = np.exp(lam) * np.ones(N)
         base["exp_lambda"] = np.array((1.0 / el, el))
         base["del"] = np.array((el ** 2 - 1.0, el ** -2 - 1.0))
@@ -82,34 +118,42 @@ class AdjointProblem:
         """Forward + adjoint sums on the GPU.  Returns (logdet [nf] summed over
         spins, dict of per-field pieces)."""
         sims, E = self._sims(K, mu, lam)
-        eng = _eng.Engine(sims, device=self.device)
-        try:
-            nf, N = len(sims), self.N
-            L = eng.L
-            logdet = np.zeros(2 * nf)
-            W = np.zeros(2 * nf * N * N)
-            trG = np.zeros(2 * nf)
-            sG = np.zeros(2 * nf)
-            dp = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))
-            L.dqmc_engine_adjoint.argtypes = [C.c_void_p] + [C.POINTER(C.c_double)] * 4
-            rc = L.dqmc_engine_adjoint(eng.h, dp(logdet), dp(W), dp(trG), dp(sG))
-            if rc != 0:
-                raise _eng.DqmcError("dqmc_engine_adjoint: %s" % L.dqmc_last_error().decode())
-        finally:
-            eng.close()
+        if self._eng is None:
+            # the fields are fixed: one engine for every evaluation, its
+            # parameters replaced when K, mu or lambda change
+            self._eng = _eng.Engine(sims, device=self.device, gemm_tile=1)
+        else:  # every field shares the parameters: one struct for all chains
+            self._eng.set_params(range(len(sims)), sims[0]["params"])
+        eng = self._eng
+        nf, N = len(sims), self.N
+        L = eng.L
+        logdet = np.zeros(2 * nf)
+        W = np.zeros(2 * nf * N * N)
+        trG = np.zeros(2 * nf)
+        sG = np.zeros(2 * nf)
+        dp = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))
+        L.dqmc_engine_adjoint.argtypes = [C.c_void_p] + [C.POINTER(C.c_double)] * 4
+        rc = L.dqmc_engine_adjoint(eng.h, dp(logdet), dp(W), dp(trG), dp(sG))
+        if rc != 0:
+            raise _eng.DqmcError("dqmc_engine_adjoint: %s" % L.dqmc_last_error().decode())
         # column-major buffers -> (row, col) matrices
         Wm = W.reshape(nf, 2, N, N).transpose(0, 1, 3, 2)
         return logdet.reshape(nf, 2).sum(axis=1), {
             "logdet": logdet.reshape(nf, 2), "W": Wm, "trG": trG.reshape(nf, 2),
-            "sG": sG.reshape(nf, 2), "E": E}
+            "sG": sG.reshape(nf, 2), "E": E, "Einv": self._Einv}
 
-    def grad_K_from(self, K, mu, W_weighted, E):
+    def grad_K_from(self, K, mu, W_weighted, E, Einv=None):
         """d/dK of sum_f w_f sum_s logdet, given sum_f w_f W_f,s summed over s
-        (both spins share K_s = K - mu I here)."""
+        (both spins share K_s = K - mu I here); Einv: E^-1 if at hand."""
         dt = self.dt
         A = -dt * (np.asarray(K, dtype=np.float64) - mu * np.eye(self.N))
-        Z = W_weighted @ np.linalg.inv(E)
-        Lf = expm_frechet(A, Z, compute_expm=False)
+        Z = W_weighted @ (np.linalg.inv(E) if Einv is None else Einv)
+        if np.array_equal(A, A.T):
+            # the decomposition of the last evaluation when it is of this A
+            sym = self._sym if self._sym is not None and np.array_equal(self._sym.A, A) else _SymExp(A)
+            Lf = sym.frechet(Z)
+        else:
+            Lf = expm_frechet(A, Z, compute_expm=False)
         return -dt * Lf.T
 
 
@@ -118,7 +162,7 @@ def logdet_and_grads(prob, K, mu, lam):
     to K (full matrix), mu and lambda (per field for mu and lambda)."""
     f, d = prob.run(K, mu, lam)
     Wsum = d["W"].sum(axis=(0, 1))
-    gK = prob.grad_K_from(K, mu, Wsum, d["E"])
+    gK = prob.grad_K_from(K, mu, Wsum, d["E"], d.get("Einv"))
     N, L, dt = prob.N, prob.L, prob.dt
     gmu = dt * (2 * N * L - d["trG"].sum(axis=1))
     glam = d["sG"].sum(axis=1)
@@ -144,7 +188,7 @@ try:
             prob, d = ctx.prob, ctx.d
             w = g.detach().cpu().numpy().astype(np.float64)
             Ww = np.tensordot(w, d["W"].sum(axis=1), axes=(0, 0))
-            gK = prob.grad_K_from(ctx.K, ctx.mu, Ww, d["E"])
+            gK = prob.grad_K_from(ctx.K, ctx.mu, Ww, d["E"], d.get("Einv"))
             N, L, dt = prob.N, prob.L, prob.dt
             gmu = float(np.dot(w, dt * (2 * N * L - d["trG"].sum(axis=1))))
             glam = float(np.dot(w, d["sG"].sum(axis=1)))

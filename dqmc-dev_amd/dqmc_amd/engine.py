@@ -67,6 +67,8 @@ def lib():
         L.dqmc_probe_mfma.restype = C.c_int
         L.dqmc_engine_drift.argtypes = [C.c_void_p, P(C.c_double)]
         L.dqmc_engine_drift.restype = C.c_int
+        L.dqmc_engine_set_params.argtypes = [C.c_void_p, C.c_int, P(capi.DqmcParams)]
+        L.dqmc_engine_set_params.restype = C.c_int
         for f in ("set_state", "get_state", "set_meas", "get_meas", "run", "green", "calc_eq_g",
                   "calc_ue_g", "gemm_stats"):
             getattr(L, "dqmc_engine_" + f).restype = C.c_int
@@ -156,6 +158,16 @@ class Engine:
     @property
     def nc(self):
         return len(self.sims)
+
+    def set_params(self, c, params):
+        """Replace chain c's numeric parameters (exponentials, exp_lambda, del,
+        Peierls phases) with those of a params dict of the same shape; c may
+        be a sequence of chains that all get these parameters."""
+        ps, keep = capi.make_params(params)
+        for k in (c if hasattr(c, "__iter__") else (c,)):
+            _check(self.L.dqmc_engine_set_params(self.h, int(k), C.byref(ps)),
+                   "dqmc_engine_set_params")
+            self._keep[k] = (ps, keep)
 
     def set_state(self, c, st):
         rng = np.ascontiguousarray(st["rng"], dtype=np.uint64)

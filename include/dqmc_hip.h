@@ -220,6 +220,15 @@ int dqmc_test_gemm(int is_complex, int opa, int opb, int M, int N, int K, const 
  * the published peak. */
 int dqmc_probe_mfma(int form, double *tflops);
 
+/* Replace chain c's numeric parameters (the exponentials, exp_lambda, del
+ * and the Peierls phases) in an existing engine; N, L, n_matmul and the
+ * element type must be the engine's, and the maps are kept.  No counterpart
+ * in the reference, whose parameters are fixed per file; it serves callers
+ * that evaluate the same HS fields for many parameter values (the adjoint's
+ * parameter scans) without rebuilding the engine.  G is recomputed before
+ * the next sweep.  Not for padded (chain_L / chain_N) engines. */
+int dqmc_engine_set_params(dqmc_engine *e, int chain, const dqmc_params *params);
+
 /* Wrap drift (CHECK_G_WRP, src/dqmc.c:277-287; a compile-time check there):
  * with dqmc_tuning.flags bit 2 the engine also wraps G at every
  * recalculation slice and compares it with the recomputed G.  max_abs[c] <-

@@ -67,3 +67,42 @@ def test_lambda_of_U_derivative():
         assert abs(fd - dl) < 1e-8 * abs(dl)
         # cosh(lambda) = e^{U dt / 2} (gen_1band_unified_hub.py:925-927)
         assert abs(np.cosh(lam) - np.exp(0.5 * U * 0.1)) < 1e-12
+
+
+def test_symmetric_exponential_and_frechet_match_scipy():
+    """The eigendecomposition route (symmetric K) against scipy's expm and
+    expm_frechet (Al-Mohy and Higham), on a lattice hopping matrix whose
+    spectrum is degenerate (exact ties in the divided differences) and on a
+    random symmetric one."""
+    from scipy.linalg import expm_frechet
+    rs = np.random.RandomState(7)
+    K_lat = gen.hopping_square(4, 4, t=1.0, tp=-0.25)[0].real
+    B = rs.randn(16, 16)
+    for K in (K_lat, 0.5 * (B + B.T)):
+        A = -0.1 * (K - 0.3 * np.eye(16))
+        s = adjoint._SymExp(A)
+        for c in (1.0, -1.0, 0.5, -0.5):
+            assert np.abs(s.exp(c) - expm(c * A)).max() < 1e-13
+        Z = rs.randn(16, 16)
+        ref = expm_frechet(A, Z, compute_expm=False)
+        assert np.abs(s.frechet(Z) - ref).max() < 1e-12 * np.abs(ref).max()
+
+
+def test_symmetric_K_gradient_matches_finite_differences():
+    """grad_K_from on a symmetric K (eigendecomposition route) against central
+    differences of the brute-force log det."""
+    rs = np.random.RandomState(5)
+    prob = adjoint.AdjointProblem(2, 3, L=6, dt=0.1, fields=rs.randint(0, 2, (1, 6, 6)), n_matmul=2)
+    K = prob.hopping(1.0, -0.2)
+    mu, lam = 0.2, 0.5
+    field = prob.fields[0]
+    b, E = _brute(prob, K, mu, lam, field)
+    gK = prob.grad_K_from(K, mu, b["W"], E, np.linalg.inv(E))
+    f = lambda K_: _brute(prob, K_, mu, lam, field)[0]["logdet"]
+    h = 1e-6
+    for (i, j) in [(0, 1), (1, 0), (2, 2), (5, 3)]:
+        Kp, Km = K.copy(), K.copy()
+        Kp[i, j] += h
+        Km[i, j] -= h
+        fd = (f(Kp) - f(Km)) / (2 * h)
+        assert abs(fd - gK[i, j]) < 1e-7 * max(1.0, abs(fd)), (i, j, fd, gK[i, j])

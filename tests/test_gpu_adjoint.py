@@ -40,6 +40,26 @@ def test_engine_pieces_match_brute_force():
         assert abs(d["sG"][k].sum() - b["sG"]) < 1e-9 * max(1.0, abs(b["sG"]))
 
 
+def test_set_params_matches_a_fresh_engine():
+    # an evaluation after dqmc_engine_set_params equals one on an engine built
+    # with those parameters, bit for bit
+    Nx, Ny, L = 4, 4, 20
+    fields = _fields(Nx, Ny, L, 3, 12)
+    a = adjoint.AdjointProblem(Nx, Ny, L, 0.1, fields, n_matmul=10)
+    b = adjoint.AdjointProblem(Nx, Ny, L, 0.1, fields, n_matmul=10)
+    try:
+        lam = adjoint.lambda_of_U(4.0, 0.1)[0]
+        a.run(a.hopping(1.0, -0.2), -0.3, lam)  # builds a's engine
+        f2, d2 = a.run(a.hopping(0.9, 0.1), 0.2, 1.1 * lam)  # replaces its parameters
+        g2, e2 = b.run(b.hopping(0.9, 0.1), 0.2, 1.1 * lam)  # a fresh engine
+        assert np.array_equal(f2, g2)
+        for k in ("W", "trG", "sG", "logdet"):
+            assert np.array_equal(d2[k], e2[k]), k
+    finally:
+        a.close()
+        b.close()
+
+
 @pytest.fixture(scope="module")
 def cfg5():
     Nx = Ny = 10
