@@ -235,7 +235,9 @@ def bench_config4(args):
     (buckets.tune_n_matmul: the fastest of 5, 8, 10, 16, 20 whose wrap drift
     stays below 1e-3, the drift of the reference's own headline setting;
     --no-tune: 10, or 8 for N > 200), with a partial last
-    group where it does not divide a chain's L; delay rank min(32, N), eq
+    group where it does not divide a chain's L; then the delay rank
+    (buckets.tune_delay_rank: the fastest compiled rank of 4, 8, 16, 32;
+    --no-tune: each file's n_delay = min(32, N), as compiled); eq
     measurements every n_matmul slices.  The engines run concurrently."""
     from dqmc_amd import buckets, gen
     nch = args.chains if args.chains != 256 else 128
@@ -266,10 +268,18 @@ def bench_config4(args):
                           "trials": [[n, round(t, 4), d] for n, t, d in table]})
             print("config 4: bucket %d n_matmul trials (n, s/sweep, drift): %s -> %d"
                   % (bk, table, nm), file=sys.stderr, flush=True)
+        kd = 0
+        if not args.no_tune:
+            kd, ktable = buckets.tune_delay_rank(groups[bk], n_matmul=nm,
+                                                 graphs=not args.no_graphs)
+            tuned[-1].update(delay_rank=kd, rank_trials=[[k, round(t, 4)] for k, t in ktable])
+            print("config 4: bucket %d delay rank trials (rank, s/sweep): %s -> %d"
+                  % (bk, ktable, kd), file=sys.stderr, flush=True)
         # the bucket engines run from concurrent host threads: no graph capture
-        engs.append(buckets.PaddedBatch(groups[bk], nm, graphs=False))
-        print("config 4: bucket %d: %d chains padded to N=%d L=%d, n_matmul %d" %
-              (bk, len(groups[bk]), engs[-1].N, engs[-1].L, nm), file=sys.stderr, flush=True)
+        engs.append(buckets.PaddedBatch(groups[bk], nm, graphs=False, delay_rank=kd))
+        print("config 4: bucket %d: %d chains padded to N=%d L=%d, n_matmul %d, delay rank %d" %
+              (bk, len(groups[bk]), engs[-1].N, engs[-1].L, nm, engs[-1].engine.delay_rank()),
+              file=sys.stderr, flush=True)
     dt, n = _run_engines(engs, args.steps, args.warmup, len(engs))
     # each chain's own work (its N, L, the bucket's n_matmul, eq every n_matmul
     # slices); padding is overhead, not counted
@@ -277,9 +287,10 @@ def bench_config4(args):
     for e in engs:
         for Nc, Lc in zip(e.Nc, e.Lc):
             Fc = -(-Lc // e.n_matmul)
-            fl += flop_model(Nc, Lc, Fc, e.n_matmul, 0.5 * Nc * Lc, min(32, Nc),
+            fl += flop_model(Nc, Lc, Fc, e.n_matmul, 0.5 * Nc * Lc, e.engine.delay_rank(),
                              Lc // e.n_matmul, 0.0)
     fl /= nch
+    blist = [[e.nc, e.N, e.L, e.n_matmul, e.engine.delay_rank()] for e in engs]
     for e in engs:
         e.close()
     return _line("DQMC sweeps/s, 128 uneven real chains (aggregate)", n / dt, "sweeps/s",
@@ -287,8 +298,8 @@ def bench_config4(args):
                  "%d real chains, side 6..20, L 20..100, U in {2,4,8}, mu in [-1.5,0], "
                  "seed 99+chain; %d size buckets (one padded engine each), run concurrently"
                  % (nch, len(engs)),
-                 dict({"buckets": [[e.nc, e.N, e.L, e.n_matmul] for e in engs],
-                       "n_matmul_tuning": tuned, "model_acceptance_assumed": 0.5},
+                 dict({"buckets": blist,  # chains, N, L, n_matmul, delay rank
+                       "tuning": tuned, "model_acceptance_assumed": 0.5},
                       **_model(fl, n / dt)), nch)
 
 

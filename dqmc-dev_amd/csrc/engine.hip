@@ -815,10 +815,29 @@ static int meas_eq_extra(dqmc_engine* e, const MeasArgs<T>& ma)
 // unequal-time Green's functions and measurements
 #include "dq_ue_host.hpp"
 
-// Metropolis sweep of slice l for every chain; the delay rank is the register
-// capacity of the kernel variant (real: 16 slots for N <= 256, 8 for N <= 512;
-// complex: half of that).  The flush work per acceptance does not depend on the
-// rank (rounds cost ~rank, there are ~1/rank of them); the gemv shrinks with it.
+// Delay ranks of the compiled Metropolis variants (powers of two between the
+// bounds below):
+//   N <= 64, registers:                  real 8..16, complex 4..8
+//   N <= 256, A in registers, B in LDS:  real 8..32, complex 8..16
+//   N <= 512:                            real 8..16 (512 threads, B in LDS),
+//                                        complex 4 (registers, 2 rows a thread)
+// A request takes the largest compiled rank not above it, the smallest below
+// them all; N > 512 runs the global-memory kernel at any rank <= N.
+static int upd_rank(int N, bool cplx, int want)
+{
+    if (N > UPD2_MAXN) return std::max(1, std::min(want, N));
+    int lo, hi;
+    if (N <= 64) { lo = cplx ? 4 : 8; hi = cplx ? 8 : 16; }
+    else if (N <= UPD3_THREADS) { lo = 8; hi = cplx ? 16 : 32; }
+    else { lo = cplx ? 4 : 8; hi = cplx ? 4 : 16; }
+    int k = lo;
+    while (2 * k <= hi && 2 * k <= want) k *= 2;
+    return k;
+}
+
+// Metropolis sweep of slice l for every chain at the engine's delay rank
+// (upd_rank).  The flush work per acceptance does not depend on the rank
+// (rounds cost ~rank, there are ~1/rank of them); the gemv grows with it.
 template <typename T>
 static int launch_update(dqmc_engine* e, int l)
 {
@@ -834,14 +853,23 @@ static int launch_update(dqmc_engine* e, int l)
         a.n_acc = e->nacc; a.active = e->active;
         a.Lc = e->ragged ? e->d_Lc : nullptr;
         constexpr bool Z = is_cplx<T>::value;
-        // small N: the register-only kernel (rank 16 / 8 complex; its flushes
-        // are cheap there); up to N = 256 rank 32 / 16 with B in LDS and the
-        // flush on the matrix cores; up to N = 512 the same with 512 threads,
-        // rank 16 (real; complex keeps the register kernel, rank 4)
-        if (N <= 64) k_update_reg<T, 1, Z ? 8 : 16><<<nc, UPD2_THREADS, 0, e->st>>>(a);
-        else if (N <= UPD3_THREADS) k_update_lds<T, 1, Z ? 16 : 32><<<nc, UPD3_THREADS, 0, e->st>>>(a);
-        else if constexpr (!Z) k_update_lds<T, 1, 16, 512><<<nc, 512, 0, e->st>>>(a);
-        else k_update_reg<T, 2, 4><<<nc, UPD2_THREADS, 0, e->st>>>(a);
+        const int kd = e->kd;
+        // small N: the register-only kernel (its flushes are cheap there); up
+        // to N = 256 B in LDS and the flush on the matrix cores; up to N = 512
+        // the same with 512 threads (real; complex keeps the register kernel)
+        if (N <= 64) {
+            if (kd == (Z ? 8 : 16)) k_update_reg<T, 1, Z ? 8 : 16><<<nc, UPD2_THREADS, 0, e->st>>>(a);
+            else k_update_reg<T, 1, Z ? 4 : 8><<<nc, UPD2_THREADS, 0, e->st>>>(a);
+        } else if (N <= UPD3_THREADS) {
+            if (!Z && kd == 32) k_update_lds<T, 1, Z ? 16 : 32><<<nc, UPD3_THREADS, 0, e->st>>>(a);
+            else if (kd == 16) k_update_lds<T, 1, 16><<<nc, UPD3_THREADS, 0, e->st>>>(a);
+            else k_update_lds<T, 1, 8><<<nc, UPD3_THREADS, 0, e->st>>>(a);
+        } else if constexpr (!Z) {
+            if (kd == 16) k_update_lds<T, 1, 16, 512><<<nc, 512, 0, e->st>>>(a);
+            else k_update_lds<T, 1, 8, 512><<<nc, 512, 0, e->st>>>(a);
+        } else {
+            k_update_reg<T, 2, 4><<<nc, UPD2_THREADS, 0, e->st>>>(a);
+        }
         LCHK();
         return 0;
     }
@@ -1363,6 +1391,11 @@ extern "C" int dqmc_engine_gemm_tile(dqmc_engine* e)
     return e ? e->gemm_tile : 0;
 }
 
+extern "C" int dqmc_engine_delay_rank(dqmc_engine* e)
+{
+    return e ? e->kd : 0;
+}
+
 extern "C" int dqmc_engine_create(dqmc_engine** out, const dqmc_params* const* P, int n_chains,
                                   int device, const dqmc_tuning* tune)
 {
@@ -1399,8 +1432,11 @@ extern "C" int dqmc_engine_create(dqmc_engine** out, const dqmc_params* const* P
     e->num_plaq_acc = p0->num_plaq_accum; e->num_b_acc = p0->num_b_accum;
     e->num_b2_acc = p0->num_b2_accum;
     e->esz = e->cplx ? 16 : 8;
-    e->kd = (tune && tune->delay_rank > 0) ? tune->delay_rank : (p0->n_delay > 0 ? p0->n_delay : 16);
-    if (e->kd > e->N) e->kd = e->N;
+    // the file's n_delay (the flush at k == n_delay, src/updates.c:58), or the
+    // tuning's rank, rounded to a compiled variant
+    e->kd = upd_rank(e->N, e->cplx,
+                     (tune && tune->delay_rank > 0) ? tune->delay_rank
+                                                    : (p0->n_delay > 0 ? p0->n_delay : 1 << 30));
     e->time_gemm = tune && (tune->flags & 1);
     e->use_graph = !(tune && (tune->flags & 2));
     e->check_drift = tune && (tune->flags & 4);

@@ -107,7 +107,8 @@ class PaddedBatch:
     (the last group of a chain whose L it does not divide is partial);
     graphs, check_drift: as for :class:`dqmc_amd.engine.Engine`."""
 
-    def __init__(self, sims, n_matmul, device=0, gemm_tile=0, graphs=True, check_drift=False):
+    def __init__(self, sims, n_matmul, device=0, gemm_tile=0, graphs=True, check_drift=False,
+                 delay_rank=0):
         self.sims = sims
         self.Nc = [_int(s["params"]["N"]) for s in sims]
         self.Lc = [_int(s["params"]["L"]) for s in sims]
@@ -119,6 +120,7 @@ class PaddedBatch:
         ni, nij = max(self.num_i), max(self.num_ij)
         self.padded = [pad_sim(s, self.N, self.L, n_matmul, ni, nij) for s in sims]
         self.engine = _eng.Engine(self.padded, device=device, gemm_tile=gemm_tile, graphs=graphs,
+                                  delay_rank=delay_rank,
                                   check_drift=check_drift,
                                   chain_L=self.Lc, chain_N=self.Nc)
 
@@ -201,3 +203,43 @@ def tune_n_matmul(sims, candidates=(5, 8, 10, 16, 20), tol=1e-3, sweeps=2, devic
     ok = [r for r in table if r[2] <= tol]
     pick = min(ok, key=lambda r: r[1])[0] if ok else min(cands)
     return pick, table
+
+
+def tune_delay_rank(sims, n_matmul=None, candidates=(4, 8, 16, 32), sweeps=2, device=0,
+                    graphs=True):
+    """Delay rank of the Metropolis kernel for one bucket of chains (the
+    reference takes n_delay from the input file, ``src/updates.c:7``).
+
+    Every candidate is rounded to a compiled variant of the engine's kernel
+    (Engine.delay_rank); each distinct rank gets a trial engine on copies of
+    the chains (`sims` is unchanged) that times `sweeps` sweeps after a first
+    one.  The rank changes only the rounding of G, not the chain, so the
+    fastest one is returned, with the table of (rank, seconds per sweep).
+    n_matmul: the bucket's stabilization interval (default: the chains' own;
+    chains of one shape at their own interval run unpadded, with every
+    measurement they carry)."""
+    import time
+    own = {(_int(s["params"]["N"]), _int(s["params"]["L"]), _int(s["params"]["n_matmul"]))
+           for s in sims}
+    if n_matmul is None:
+        n_matmul = _int(sims[0]["params"]["n_matmul"])
+    plain = len(own) == 1 and next(iter(own))[2] == n_matmul
+    table, seen = [], set()
+    for want in candidates:
+        if plain:
+            eng = _eng.Engine(sims, device=device, graphs=graphs, delay_rank=want)
+        else:
+            eng = PaddedBatch(sims, n_matmul, device=device, graphs=graphs,
+                              delay_rank=want).engine
+        try:
+            k = eng.delay_rank()
+            if k in seen:
+                continue
+            seen.add(k)
+            eng.run(1)
+            t0 = time.perf_counter()
+            eng.run(sweeps)
+            table.append((k, (time.perf_counter() - t0) / sweeps))
+        finally:
+            eng.close()
+    return min(table, key=lambda r: r[1])[0], table

@@ -63,6 +63,8 @@ def lib():
         L.dqmc_acc_len.restype = C.c_long
         L.dqmc_engine_gemm_tile.argtypes = [C.c_void_p]
         L.dqmc_engine_gemm_tile.restype = C.c_int
+        L.dqmc_engine_delay_rank.argtypes = [C.c_void_p]
+        L.dqmc_engine_delay_rank.restype = C.c_int
         L.dqmc_probe_mfma.argtypes = [C.c_int, P(C.c_double)]
         L.dqmc_probe_mfma.restype = C.c_int
         L.dqmc_engine_drift.argtypes = [C.c_void_p, P(C.c_double)]
@@ -240,6 +242,11 @@ class Engine:
         """GEMM C-tile shape in use (1 = 64x64, 2 = 128x64 with K steps of 16,
         3 = 128x64 with K steps of 8)."""
         return self.L.dqmc_engine_gemm_tile(self.h)
+
+    def delay_rank(self):
+        """Delay rank of the Metropolis kernel in use (the params' n_delay or
+        the requested rank, rounded to a compiled variant)."""
+        return self.L.dqmc_engine_delay_rank(self.h)
 
     def gemm_stats(self):
         ms = C.c_double(0)

@@ -112,7 +112,10 @@ typedef struct dqmc_engine dqmc_engine;
 
 /* Tuning knobs picked by the autotuner; zero fields mean "engine default". */
 typedef struct dqmc_tuning {
-    int delay_rank; /* rank of the delayed Metropolis update (register slots) */
+    int delay_rank; /* rank of the delayed Metropolis update; 0 = the params'
+                       n_delay.  Rounded to a compiled variant: powers of two,
+                       real 8..16 (N <= 64), 8..32 (N <= 256), 8..16 (N <= 512);
+                       complex 4..8, 8..16, 4; any rank <= N above N = 512 */
     int flags;      /* bit 0: time the N x N x N GEMMs with HIP events (the
                        sweeps then run launch by launch); bit 1: do not capture
                        the sweeps into HIP graphs (needed when other host
@@ -238,6 +241,10 @@ int dqmc_engine_drift(dqmc_engine *e, double *max_abs);
 
 /* The GEMM tile shape the engine uses (dqmc_tuning.gemm_tile values 1..3). */
 int dqmc_engine_gemm_tile(dqmc_engine *e);
+
+/* The delay rank the engine's Metropolis kernel runs at (n_delay of
+ * src/updates.c:7, after rounding to a compiled variant). */
+int dqmc_engine_delay_rank(dqmc_engine *e);
 
 const char *dqmc_last_error(void);
 const char *dqmc_version(void);

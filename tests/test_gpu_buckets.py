@@ -125,3 +125,18 @@ def test_n_matmul_autotuner_picks_a_stable_interval():
     assert all(t > 0 and d > 0 for _, t, d in table)
     ok = [(n, t) for n, t, d in table if d <= 1e-3]
     assert pick == (min(ok, key=lambda x: x[1])[0] if ok else 5)
+
+
+def test_delay_rank_autotuner_times_each_compiled_rank():
+    sims = [_sim(9, 20, 10, 44), _sim(10, 40, 10, 45)]  # N = 81, 100: ranks 8, 16, 32
+    hs0 = [np.array(s["state"]["hs"]) for s in sims]
+    pick, table = buckets.tune_delay_rank(sims, n_matmul=10, candidates=(4, 8, 12, 16, 32), sweeps=1)
+    print("table", table, "pick", pick, flush=True)
+    assert [k for k, _ in table] == [8, 16, 32]  # 4 -> 8 and 12 -> 8 share a variant
+    assert all(t > 0 for _, t in table)
+    assert pick == min(table, key=lambda r: r[1])[0]
+    assert all(np.array_equal(np.asarray(s["state"]["hs"]), h) for s, h in zip(sims, hs0))
+    # chains of one shape at their own interval run unpadded
+    one = [_sim(10, 20, 10, 46), _sim(10, 20, 10, 47)]
+    pick1, table1 = buckets.tune_delay_rank(one, candidates=(16, 32), sweeps=1)
+    assert [k for k, _ in table1] == [16, 32] and pick1 in (16, 32)

@@ -151,6 +151,51 @@ def test_trajectory_per_update_kernel(nflux, side):
         eng.close()
 
 
+_ORACLE_RUNS = {}
+
+
+# every compiled delay rank of every Metropolis kernel (requested rank ->
+# rank in use): the trajectory is the oracle's whatever the rank
+@pytest.mark.parametrize("nflux,side,want,rank", [
+    (0, 6, 8, 8), (0, 6, 32, 16), (1, 6, 4, 4), (1, 6, 16, 8),
+    (0, 10, 8, 8), (0, 10, 20, 16), (0, 10, 64, 32), (1, 10, 8, 8), (1, 10, 16, 16),
+    (1, 10, 2, 8), (0, 17, 8, 8), (0, 17, 16, 16), (1, 17, 16, 4)])
+def test_trajectory_per_delay_rank(nflux, side, want, rank):
+    sim = gen.create_batch(seed=32, Nx=side, Ny=side, nflux=nflux, U=6.0, mu=-0.8, tp=-0.2,
+                           L=10, n_matmul=5, period_eqlt=5, n_sweep_warm=1, n_sweep_meas=2)[0]
+    print("side", side, "nflux", nflux, "rank", want, flush=True)
+    eng = engine.Engine([sim], delay_rank=want)
+    try:
+        assert eng.delay_rank() == rank
+        eng.run(3)
+        st = eng.get_state(0)
+        if (nflux, side) not in _ORACLE_RUNS:
+            _ORACLE_RUNS[(nflux, side)] = oracle.run(sim)
+        out = _ORACLE_RUNS[(nflux, side)]
+        assert np.array_equal(st["hs"], out["state"]["hs"])
+        assert np.array_equal(st["rng"], out["state"]["rng"])
+        eq, _ = eng.get_meas(0)
+        for k in ("density", "double_occ", "g00", "nn", "zz"):
+            assert rel(eq[k], out["meas_eqlt"][k]) < 1e-9, k
+        # the sweep ended on a recalculation: G is the stabilized product's
+        s2 = dict(sim, state=dict(sim["state"], hs=st["hs"]))
+        for spin in (0, 1):
+            G, _ = eng.green(0, spin)
+            assert rel(G, oracle.calc_eq_g(s2, spin, 0)[0]) < 1e-10
+    finally:
+        eng.close()
+
+
+def test_delay_rank_defaults_to_the_files_n_delay():
+    sim = gen.create_batch(seed=33, Nx=10, Ny=10, U=4.0, L=10, n_matmul=5, n_delay=16,
+                           period_eqlt=5, n_sweep_warm=1, n_sweep_meas=1)[0]
+    eng = engine.Engine([sim])
+    try:
+        assert eng.delay_rank() == 16
+    finally:
+        eng.close()
+
+
 @pytest.mark.parametrize("nflux", [0, 1])
 @pytest.mark.parametrize("Lnm", [(20, 10), (40, 10), (40, 8), (80, 10)])
 def test_unequal_time(nflux, Lnm):
