@@ -1358,20 +1358,52 @@ static int autotune_gemm(dqmc_engine* e)
     return rc;
 }
 
+// d[c n + i] = d[i] for chains c = 1 .. nc-1 (n doubles per chain)
+__global__ __launch_bounds__(256) void k_bcast_chain0(double* d, long n)
+{
+    const long c = blockIdx.y + 1;
+    for (long i = blockIdx.x * 256L + threadIdx.x; i < n; i += (long)gridDim.x * 256)
+        d[c * n + i] = d[i];
+}
+__global__ __launch_bounds__(256) void k_fill_i32(int* d, int n, int v)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) d[i] = v;
+}
+
 extern "C" int dqmc_engine_set_params(dqmc_engine* e, int c, const dqmc_params* p)
 {
-    if (!e || !p || c < 0 || c >= e->nc) return set_err(DQMC_ERR_ARG, "bad arguments");
+    if (!e || !p || c < -1 || c >= e->nc) return set_err(DQMC_ERR_ARG, "bad arguments");
     if (p->N != e->N || p->L != e->L || p->n_matmul != e->n_matmul ||
         (p->is_complex != 0) != (e->cplx != 0))
         return set_err(DQMC_ERR_ARG, "parameters of another shape than the engine's");
     if (e->ragged) return set_err(DQMC_ERR_UNSUPPORTED, "padded chains keep their parameters");
     HIPCHK(hipSetDevice(e->device));
     HIPCHK(hipStreamSynchronize(e->st));
-    RC(upload_numeric(e, c, p));
+    const bool all = c < 0;
+    RC(upload_numeric(e, all ? 0 : c, p));
+    if (all && e->nc > 1) {  // one upload, then chain 0's blocks to every chain
+        const long N = e->N, NN = N * N, ed = e->esz / 8;
+        struct { void* d; long n; } arr[] = {
+            {e->expK, 2 * NN * ed}, {e->iexpK, 2 * NN * ed}, {e->ehalf, 2 * NN * ed},
+            {e->iehalf, 2 * NN * ed}, {e->exp_lambda, 2 * N}, {e->del, 2 * N},
+            {e->cplx ? e->peierlsu : nullptr, NN * ed}, {e->cplx ? e->peierlsd : nullptr, NN * ed}};
+        for (auto& a : arr) {
+            if (!a.d) continue;
+            const long nb = std::min<long>(nblk(a.n, 256), 64);
+            k_bcast_chain0<<<dim3(nb, e->nc - 1), 256, 0, e->st>>>((double*)a.d, a.n);
+            LCHK();
+        }
+    }
     // sites that consume a uniform (Delta's not both zero, src/updates.c:21)
     int nd = 0;
     for (int i = 0; i < e->N; i++) nd += !(p->del[i] == 0.0 && p->del[i + e->N] == 0.0);
-    HIPCHK(hipMemcpyAsync(e->draws + c, &nd, 4, hipMemcpyHostToDevice, e->st));
+    if (all) {
+        k_fill_i32<<<nblk(e->nc, 256), 256, 0, e->st>>>(e->draws, e->nc, nd);
+        LCHK();
+    } else {
+        HIPCHK(hipMemcpyAsync(e->draws + c, &nd, 4, hipMemcpyHostToDevice, e->st));
+    }
     HIPCHK(hipStreamSynchronize(e->st));
     e->stale = true;  // G is recomputed before the next sweep
     return 0;

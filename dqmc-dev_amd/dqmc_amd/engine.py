@@ -164,11 +164,15 @@ class Engine:
     def set_params(self, c, params):
         """Replace chain c's numeric parameters (exponentials, exp_lambda, del,
         Peierls phases) with those of a params dict of the same shape; c may
-        be a sequence of chains that all get these parameters."""
+        be a sequence of chains that all get these parameters (every chain:
+        one upload, copied on the device)."""
         ps, keep = capi.make_params(params)
-        for k in (c if hasattr(c, "__iter__") else (c,)):
-            _check(self.L.dqmc_engine_set_params(self.h, int(k), C.byref(ps)),
+        cs = [int(k) for k in (c if hasattr(c, "__iter__") else (c,))]
+        every = sorted(cs) == list(range(len(self.sims)))
+        for k in ([-1] if every else cs):
+            _check(self.L.dqmc_engine_set_params(self.h, k, C.byref(ps)),
                    "dqmc_engine_set_params")
+        for k in cs:
             self._keep[k] = (ps, keep)
 
     def set_state(self, c, st):

@@ -224,8 +224,9 @@ int dqmc_test_gemm(int is_complex, int opa, int opb, int M, int N, int K, const 
 int dqmc_probe_mfma(int form, double *tflops);
 
 /* Replace chain c's numeric parameters (the exponentials, exp_lambda, del
- * and the Peierls phases) in an existing engine; N, L, n_matmul and the
- * element type must be the engine's, and the maps are kept.  No counterpart
+ * and the Peierls phases) in an existing engine; chain -1: every chain (one
+ * upload, copied on the device).  N, L, n_matmul and the element type must be
+ * the engine's, and the maps are kept.  No counterpart
  * in the reference, whose parameters are fixed per file; it serves callers
  * that evaluate the same HS fields for many parameter values (the adjoint's
  * parameter scans) without rebuilding the engine.  G is recomputed before

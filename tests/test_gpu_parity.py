@@ -345,3 +345,42 @@ def test_graph_replay_matches_eager(nflux):
     finally:
         eg.close()
         ee.close()
+
+
+def test_set_params_per_chain_and_every_chain_match_fresh_engines():
+    """dqmc_engine_set_params on one chain, and on every chain at once (chain
+    -1: one upload copied on the device), gives the chains of an engine built
+    with those parameters (same field and random stream: same seeds)."""
+    kw = dict(Nx=4, Ny=4, mu=-0.3, tp=-0.1, L=20, n_matmul=5, period_eqlt=5, n_sweep_warm=1,
+              n_sweep_meas=4)
+    a = gen.create_batch(Nfiles=2, seed=51, U=4.0, **kw)
+    b = gen.create_batch(Nfiles=2, seed=51, U=6.0, **kw)
+
+    def same(e1, e2, c):
+        s1, s2 = e1.get_state(c), e2.get_state(c)
+        assert np.array_equal(s1["hs"], s2["hs"]) and np.array_equal(s1["rng"], s2["rng"])
+        for spin in (0, 1):
+            assert rel(e1.green(c, spin)[0], e2.green(c, spin)[0]) < 1e-12
+
+    e1 = engine.Engine(a)
+    e2 = engine.Engine([a[0], b[1]])
+    e3 = engine.Engine(b)
+    try:
+        e1.set_params(1, b[1]["params"])
+        e1.run(3)
+        e2.run(3)
+        same(e1, e2, 0)
+        same(e1, e2, 1)
+        e4 = engine.Engine(a)
+        try:
+            e4.set_params(range(2), b[0]["params"])  # U=6 exponentials for both
+            e4.run(3)
+            e3.run(3)
+            same(e4, e3, 0)
+            same(e4, e3, 1)  # b[1] differs from b[0] only in its field and stream
+        finally:
+            e4.close()
+    finally:
+        e1.close()
+        e2.close()
+        e3.close()
