@@ -20,6 +20,7 @@ struct MeasArgs {
     int energy, bond;
     const T* G; long sG;              // half-wrapped equal-time G[b] (eq) / G(t,t) etc (ue)
     const T* G0t; const T* Gtt; const T* Gt0; long sU; // ue arrays [b][L*N*N]
+    const T* G0tT;                    // G(0,t)^T per slice, same layout (bond-pair kernels)
     const T* phase;                   // [c]
     const int* flag;                  // [c]: measure this chain now
     const int* rp_i; const int* ix_i; long srp_i, six_i;     // CSR of map_i  [c]

@@ -35,11 +35,12 @@ enum { UE_uuuu = 7, UE_dddd = 8, UE_uudd = 9, UE_dduu = 10, UE_pair_b2b2 = 16, U
 enum { EQ_uuuu = 11, EQ_dddd = 12, EQ_uudd = 13, EQ_dduu = 14, EQ_chi = 15, EQ_j = 21,
        EQ_j2 = 22, EQ_jn = 23 };
 
-// Green's functions of one spin at one slice (the unequal-time blocks; for
-// equal time all four are the same matrix and dt = 1)
+// Green's functions of one spin at one slice (the unequal-time blocks).  G(0,t)
+// is read from its transpose, otT[a + N b] = G0t[b, a]: a wave's lanes share
+// the bond at 0 and differ in the bond at t, so the reads go down one column.
 template <typename T>
 struct SpinSlice {
-    const T *tt, *oo, *ot, *t0;  // G(t,t), G(0,0), G(0,t), G(t,0)
+    const T *tt, *oo, *otT, *t0;  // G(t,t), G(0,0), G(0,t)^T, G(t,0)
     int N;
     double dt;                   // 1 at t = 0 (the equal-time delta terms)
 };
@@ -69,7 +70,7 @@ __device__ inline void pair_vals(const SpinSlice<T>& g, const BondOp<T>& a, cons
     for (int p = 0; p < 2; p++)
 #pragma unroll
         for (int q = 0; q < 2; q++) {
-            ot[q][p] = g.ot[sb[q] + N * sa[p]];
+            ot[q][p] = g.otT[sa[p] + N * sb[q]];
             t0[p][q] = g.t0[sa[p] + N * sb[q]];
         }
     // connected <c+_{sa[pa]} c_{sa[qa]} (t)  c+_{sb[pb]} c_{sb[qb]} (0)>
@@ -204,7 +205,7 @@ __global__ __launch_bounds__(256) void k_meas_ue_pairs(MeasArgs<T> a)
         const long bo = (2 * c + s) * a.sU;
         g[s].tt = a.Gtt + bo + NN * tt;
         g[s].oo = a.Gtt + bo;
-        g[s].ot = a.G0t + bo + NN * tt;
+        g[s].otT = a.G0tT + bo + NN * tt;
         g[s].t0 = a.Gt0 + bo + NN * tt;
         g[s].N = N;
         g[s].dt = (tt == 0) ? 1.0 : 0.0;

@@ -154,6 +154,29 @@ __global__ __launch_bounds__(256) void k_trsm_upper(int n, int ncols, const T* U
         }
 }
 
+// Y[b] = X[b]^T for the N x N matrices b = blockIdx.x / tn^2 (stride NN), one
+// 32 x 32 tile per workgroup through LDS (tn = ceil(N / 32) tiles a side)
+template <typename T>
+__global__ __launch_bounds__(256) void k_transpose_batch(int N, int tn, const T* X, T* Y, long NN)
+{
+    __shared__ T s[32][33];
+    const int tiles = tn * tn;
+    const long b = blockIdx.x / tiles;
+    const int t = blockIdx.x % tiles, ti = t % tn, tj = t / tn;
+    const T* x = X + b * NN;
+    T* y = Y + b * NN;
+    const int lx = threadIdx.x & 31, ly = threadIdx.x >> 5;
+    for (int k = ly; k < 32; k += 8) {
+        const int r = 32 * ti + lx, c = 32 * tj + k;
+        if (r < N && c < N) s[k][lx] = x[r + (long)N * c];
+    }
+    __syncthreads();
+    for (int k = ly; k < 32; k += 8) {  // y(r, c) = x(c, r) = s[lx][k]
+        const int r = 32 * tj + lx, c = 32 * ti + k;
+        if (r < N && c < N) y[r + (long)N * c] = s[lx][k];
+    }
+}
+
 // p[0:n) = 0 (a fill kernel rather than a memset node, so that the sweeps can
 // be captured into HIP graphs)
 __global__ void k_zero(long n, double* p)

@@ -408,6 +408,16 @@ static int ue_measure(dqmc_engine* e)
         k_meas_ue<T><<<dim3((e->num_ij + 255) / 256, L, nc), 256, 0, e->st>>>(ma);
     }
     LCHK();
+    if (e->bond || e->thermal || e->pbbonly || e->twobond) {
+        // G(0,t)^T for the bond-pair kernels, into T2 (the expansion's
+        // half-wrapped B_l, free until the next measurement rebuilds it)
+        const int tn = (e->N + 31) / 32;
+        const long count = 2L * nc * L;
+        k_transpose_batch<T><<<(unsigned)(count * tn * tn), 256, 0, e->st>>>(
+            e->N, tn, dp<T>(e->G0t), dp<T>(e->T2), (long)e->N * e->N);
+        LCHK();
+        ma.G0tT = dp<T>(e->T2);
+    }
     if (e->bond || e->thermal || e->pbbonly) {
         k_meas_ue_pairs<T, PK_BB><<<dim3((e->num_bb + 255) / 256, L, nc), 256, 0, e->st>>>(ma);
         LCHK();
