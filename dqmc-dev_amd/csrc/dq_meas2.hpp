@@ -48,7 +48,8 @@ struct SpinSlice {
 template <typename T>
 struct BondOp {
     int x0, x1;
-    T pf, pb;  // phases of c+_x0 c_x1 and c+_x1 c_x0
+    T pf, pb;   // phases of c+_x0 c_x1 and c+_x1 c_x0
+    T f, bk, n; // (tables) <F>, <B> and the bond density at the operator's slice
 };
 
 template <typename T>
@@ -58,8 +59,9 @@ struct PairVals {
 };
 
 // per-spin pieces of bond operator a at t and bond operator b at 0; DA / DB:
-// also the density pieces of a / b
-template <typename T, bool DA, bool DB>
+// also the density pieces of a / b; TAB: the hop terms and densities come
+// with the operators (k_bond_tab, the same expressions) instead of from G
+template <typename T, bool DA, bool DB, bool TAB = false>
 __device__ inline void pair_vals(const SpinSlice<T>& g, const BondOp<T>& a, const BondOp<T>& b,
                                  PairVals<T>& v)
 {
@@ -77,8 +79,15 @@ __device__ inline void pair_vals(const SpinSlice<T>& g, const BondOp<T>& a, cons
     auto cn = [&](int pa, int qa, int pb, int qb) -> T {
         return (from_re<T>(sa[pa] == sb[qb] ? g.dt : 0.0) - ot[qb][pa]) * t0[qa][pb];
     };
-    const T fa = -(a.pf * g.tt[a.x1 + N * a.x0]), ba = -(a.pb * g.tt[a.x0 + N * a.x1]);
-    const T fb = -(b.pf * g.oo[b.x1 + N * b.x0]), bb = -(b.pb * g.oo[b.x0 + N * b.x1]);
+    T fa, ba, fb, bb;
+    if constexpr (TAB) {
+        fa = a.f; ba = a.bk; fb = b.f; bb = b.bk;
+    } else {
+        fa = -(a.pf * g.tt[a.x1 + N * a.x0]);
+        ba = -(a.pb * g.tt[a.x0 + N * a.x1]);
+        fb = -(b.pf * g.oo[b.x1 + N * b.x0]);
+        bb = -(b.pb * g.oo[b.x0 + N * b.x1]);
+    }
     const T cff = a.pf * b.pf * cn(0, 1, 0, 1), cfb = a.pf * b.pb * cn(0, 1, 1, 0);
     const T cbf = a.pb * b.pf * cn(1, 0, 0, 1), cbb = a.pb * b.pb * cn(1, 0, 1, 0);
     v.ja = fa - ba;
@@ -88,11 +97,13 @@ __device__ inline void pair_vals(const SpinSlice<T>& g, const BondOp<T>& a, cons
     v.cjj = cff - cfb - cbf + cbb;
     v.ckk = cff + cfb + cbf + cbb;
     if (DA) {
-        v.na = from_re<T>(2.0) - g.tt[a.x0 + N * a.x0] - g.tt[a.x1 + N * a.x1];
+        if constexpr (TAB) v.na = a.n;
+        else v.na = from_re<T>(2.0) - g.tt[a.x0 + N * a.x0] - g.tt[a.x1 + N * a.x1];
         v.cnj = b.pf * (cn(0, 0, 0, 1) + cn(1, 1, 0, 1)) - b.pb * (cn(0, 0, 1, 0) + cn(1, 1, 1, 0));
     }
     if (DB) {
-        v.nb = from_re<T>(2.0) - g.oo[b.x0 + N * b.x0] - g.oo[b.x1 + N * b.x1];
+        if constexpr (TAB) v.nb = b.n;
+        else v.nb = from_re<T>(2.0) - g.oo[b.x0 + N * b.x0] - g.oo[b.x1 + N * b.x1];
         v.cjn = a.pf * (cn(0, 1, 0, 0) + cn(0, 1, 1, 1)) - a.pb * (cn(1, 0, 0, 0) + cn(1, 0, 1, 1));
     }
     if (DA && DB) v.cnn = cn(0, 0, 0, 0) + cn(0, 0, 1, 1) + cn(1, 1, 0, 0) + cn(1, 1, 1, 1);
@@ -161,6 +172,55 @@ __device__ inline BondOp<T> bond1(const int* bonds, int nb, int b, const T* P, l
     o.pb = pf<T>(P, o.x1 + N * o.x0);
     return o;
 }
+// bond b of spin s from the tables of k_bond_tab: phases bph (the chain's)
+// and hop terms ht = [f, b, n][b] of the operator's slice and spin
+template <typename T>
+__device__ inline BondOp<T> bond1_tab(const int* bonds, int nb, int b, const T* bph, int s,
+                                      const T* ht)
+{
+    BondOp<T> o;
+    o.x0 = bonds[b];
+    o.x1 = bonds[b + nb];
+    o.pf = bph[(2 * s) * nb + b];
+    o.pb = bph[(2 * s + 1) * nb + b];
+    o.f = ht[b];
+    o.bk = ht[nb + b];
+    o.n = ht[2 * nb + b];
+    return o;
+}
+
+// The tables read by bond1_tab, per (chain, slice t, bond b): for each spin
+// <F> = -pf G(t,t)[x1, x0], <B> = -pb G(t,t)[x0, x1], the bond density
+// 2 - G(t,t)[x0, x0] - G(t,t)[x1, x1] (as pair_vals forms them from G), and
+// at t = 0 the phases.  A wave of the pair kernel differs in the bond at t,
+// so reading these by bond index replaces gathers over G and the phases.
+template <typename T>
+__global__ __launch_bounds__(256) void k_bond_tab(MeasArgs<T> a, T* btab, T* bph)
+{
+    const int c = blockIdx.z;
+    if (!a.flag[c]) return;
+    const int tt = blockIdx.y, nb = a.num_b;
+    const int b = blockIdx.x * 256 + threadIdx.x;
+    if (b >= nb) return;
+    const long N = a.N, NN = N * N;
+    const int* bonds = a.bonds + c * a.sbonds;
+    const int x0 = bonds[b], x1 = bonds[b + nb];
+#pragma unroll
+    for (int s = 0; s < 2; s++) {
+        const T* P = (s ? a.peierlsd : a.peierlsu) + c * a.sP;
+        const T pfv = pf<T>(P, x0 + N * x1), pbv = pf<T>(P, x1 + N * x0);
+        const T* g = a.Gtt + (2 * c + s) * a.sU + NN * tt;
+        T* h = btab + ((long)(2 * c + s) * a.L + tt) * 3 * nb;
+        h[b] = -(pfv * g[x1 + N * x0]);
+        h[nb + b] = -(pbv * g[x0 + N * x1]);
+        h[2 * nb + b] = from_re<T>(2.0) - g[x0 + N * x0] - g[x1 + N * x1];
+        if (tt == 0) {
+            bph[((long)c * 4 + 2 * s) * nb + b] = pfv;
+            bph[((long)c * 4 + 2 * s + 1) * nb + b] = pbv;
+        }
+    }
+}
+
 // two-hop bond c: pp = [pp_u, ppr_u, pp_d, ppr_d][c]
 template <typename T>
 __device__ inline BondOp<T> bond2(const int* b2s, int nb2, int c, const T* pp, int s)
@@ -225,9 +285,13 @@ __global__ __launch_bounds__(256) void k_meas_ue_pairs(MeasArgs<T> a)
             const int b = lin % nb, cc = lin / nb;
 #pragma unroll
             for (int s = 0; s < 2; s++) {
-                const BondOp<T> A = bond1<T>(bonds, nb, b, P[s], N), B = bond1<T>(bonds, nb, cc, P[s], N);
-                if (thermal) pair_vals<T, true, true>(g[s], A, B, v[s]);
-                else pair_vals<T, false, false>(g[s], A, B, v[s]);
+                const T* bph = a.bph + (long)c * 4 * nb;
+                const T* ht = a.btab + ((long)(2 * c + s) * a.L + tt) * 3 * nb;
+                const T* h0 = a.btab + (long)(2 * c + s) * a.L * 3 * nb;
+                const BondOp<T> A = bond1_tab<T>(bonds, nb, b, bph, s, ht),
+                                B = bond1_tab<T>(bonds, nb, cc, bph, s, h0);
+                if (thermal) pair_vals<T, true, true, true>(g[s], A, B, v[s]);
+                else pair_vals<T, false, false, true>(g[s], A, B, v[s]);
             }
             if (bond || pbb) {
                 const int i0 = bonds[b], i1 = bonds[b + nb], j0 = bonds[cc], j1 = bonds[cc + nb];

@@ -419,6 +419,11 @@ static int ue_measure(dqmc_engine* e)
         ma.G0tT = dp<T>(e->T2);
     }
     if (e->bond || e->thermal || e->pbbonly) {
+        ma.btab = dp<T>(e->btab);
+        ma.bph = dp<T>(e->bph);
+        k_bond_tab<T><<<dim3((e->num_b + 255) / 256, L, nc), 256, 0, e->st>>>(ma, dp<T>(e->btab),
+                                                                           dp<T>(e->bph));
+        LCHK();
         k_meas_ue_pairs<T, PK_BB><<<dim3((e->num_bb + 255) / 256, L, nc), 256, 0, e->st>>>(ma);
         LCHK();
     }
