@@ -305,7 +305,6 @@ __global__ __launch_bounds__(NT) void k_update_lds(Upd2Args<T> a)
         if (k > 0) {
 #pragma unroll
             for (int r = 0; r < R; r++) {
-                const int j = tid + NT * r;
                 if constexpr (is_cplx<T>::value) {  // every lane takes part
                     upd3_flush_mfma_z<KD, W>(N, tid, NT * r, Gu, Au[r], Bu, k, du);
                     upd3_flush_mfma_z<KD, W>(N, tid, NT * r, Gd, Ad[r], Bd, k, dd);

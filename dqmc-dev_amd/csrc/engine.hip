@@ -1089,6 +1089,13 @@ static int upload_numeric(dqmc_engine* e, int c, const dqmc_params* p)
         HIPCHK(hipMemcpyAsync((char*)e->peierlsu + c * mb, p->peierlsu, mb, hipMemcpyHostToDevice, e->st));
         HIPCHK(hipMemcpyAsync((char*)e->peierlsd + c * mb, p->peierlsd, mb, hipMemcpyHostToDevice, e->st));
     }
+    const long nb2 = e->num_b2;  // phases of the two-hop bonds
+    if (nb2 > 0 && p->pp_u && p->ppr_u && p->pp_d && p->ppr_d) {
+        const double* src[4] = {p->pp_u, p->ppr_u, p->pp_d, p->ppr_d};
+        for (int q = 0; q < 4; q++)
+            HIPCHK(hipMemcpyAsync((char*)e->pp + ((long)c * 4 + q) * nb2 * e->esz, src[q],
+                                  nb2 * e->esz, hipMemcpyHostToDevice, e->st));
+    }
     return 0;
 }
 
@@ -1286,12 +1293,7 @@ static int engine_setup(dqmc_engine* e, const dqmc_params* const* P)
             HIPCHK(hipMemcpy(e->degen2 + c * PM_COUNT, d2, sizeof d2, hipMemcpyHostToDevice));
             if (nb2 > 0 && p->bond2s)
                 HIPCHK(hipMemcpy(e->bond2s + c * 2 * nb2, p->bond2s, 2 * nb2 * 4, hipMemcpyHostToDevice));
-            if (nb2 > 0 && p->pp_u && p->ppr_u && p->pp_d && p->ppr_d) {
-                const double* src[4] = {p->pp_u, p->ppr_u, p->pp_d, p->ppr_d};
-                for (int q = 0; q < 4; q++)
-                    HIPCHK(hipMemcpy((char*)e->pp + ((long)c * 4 + q) * nb2 * es, src[q], nb2 * es,
-                                     hipMemcpyHostToDevice));
-            }
+            // (the two-hop bond phases pp came with upload_numeric)
             if (e->num_plaq > 0 && p->plaqs)
                 HIPCHK(hipMemcpy(e->plaqs + c * 3L * e->num_plaq, p->plaqs, 3L * e->num_plaq * 4,
                                  hipMemcpyHostToDevice));
@@ -1354,11 +1356,11 @@ static int autotune_gemm(dqmc_engine* e)
         (void)hipEventRecord(a1, e->st);
         if (hipEventSynchronize(a1) != hipSuccess) { rc = set_err(DQMC_ERR_HIP, "autotune sync"); break; }
         float ms = 0.f;
-        hipEventElapsedTime(&ms, a0, a1);
+        (void)hipEventElapsedTime(&ms, a0, a1);
         if (ms < best) { best = ms; pick = cand[c]; }
     }
-    hipEventDestroy(a0);
-    hipEventDestroy(a1);
+    (void)hipEventDestroy(a0);
+    (void)hipEventDestroy(a1);
     e->gemm_tile = pick;
     return rc;
 }
@@ -1392,7 +1394,8 @@ extern "C" int dqmc_engine_set_params(dqmc_engine* e, int c, const dqmc_params* 
         struct { void* d; long n; } arr[] = {
             {e->expK, 2 * NN * ed}, {e->iexpK, 2 * NN * ed}, {e->ehalf, 2 * NN * ed},
             {e->iehalf, 2 * NN * ed}, {e->exp_lambda, 2 * N}, {e->del, 2 * N},
-            {e->cplx ? e->peierlsu : nullptr, NN * ed}, {e->cplx ? e->peierlsd : nullptr, NN * ed}};
+            {e->cplx ? e->peierlsu : nullptr, NN * ed}, {e->cplx ? e->peierlsd : nullptr, NN * ed},
+            {e->num_b2 > 0 && p->pp_u ? e->pp : nullptr, 4L * e->num_b2 * ed}};
         for (auto& a : arr) {
             if (!a.d) continue;
             const long nb = std::min<long>(nblk(a.n, 256), 64);
@@ -1523,18 +1526,18 @@ extern "C" int dqmc_engine_create(dqmc_engine** out, const dqmc_params* const* P
 extern "C" void dqmc_engine_destroy(dqmc_engine* e)
 {
     if (!e) return;
-    hipSetDevice(e->device);
-    if (e->st) hipStreamSynchronize(e->st);
+    (void)hipSetDevice(e->device);
+    if (e->st) (void)hipStreamSynchronize(e->st);
     for (auto& ev : e->gemm_events) {
-        hipEventDestroy(ev.first);
-        hipEventDestroy(ev.second);
+        (void)hipEventDestroy(ev.first);
+        (void)hipEventDestroy(ev.second);
     }
     for (int k = 0; k < 4; k++) {
-        if (e->gexec[k]) hipGraphExecDestroy(e->gexec[k]);
-        if (e->graph[k]) hipGraphDestroy(e->graph[k]);
+        if (e->gexec[k]) (void)hipGraphExecDestroy(e->gexec[k]);
+        if (e->graph[k]) (void)hipGraphDestroy(e->graph[k]);
     }
-    for (void* p : e->allocs) hipFree(p);
-    if (e->st) hipStreamDestroy(e->st);
+    for (void* p : e->allocs) (void)hipFree(p);
+    if (e->st) (void)hipStreamDestroy(e->st);
     delete e;
 }
 
@@ -1683,8 +1686,8 @@ extern "C" int dqmc_engine_calc_ue_g(dqmc_engine* e, int c, int spin, double* G0
 extern "C" int64_t dqmc_engine_accepts(dqmc_engine* e)
 {
     if (!e) return -1;
-    hipSetDevice(e->device);
-    hipStreamSynchronize(e->st);
+    (void)hipSetDevice(e->device);
+    (void)hipStreamSynchronize(e->st);
     std::vector<unsigned long long> v(e->nc);
     if (hipMemcpy(v.data(), e->nacc, e->nc * 8, hipMemcpyDeviceToHost) != hipSuccess) return -1;
     int64_t s = 0;
@@ -1720,10 +1723,10 @@ extern "C" int dqmc_engine_gemm_stats(dqmc_engine* e, double* avg_ms, int64_t* l
     double tot = 0.0;
     for (auto& ev : e->gemm_events) {
         float t = 0.f;
-        hipEventElapsedTime(&t, ev.first, ev.second);
+        (void)hipEventElapsedTime(&t, ev.first, ev.second);
         tot += t;
-        hipEventDestroy(ev.first);
-        hipEventDestroy(ev.second);
+        (void)hipEventDestroy(ev.first);
+        (void)hipEventDestroy(ev.second);
     }
     const long long n = (long long)e->gemm_events.size();
     e->gemm_events.clear();
@@ -1838,18 +1841,18 @@ extern "C" int dqmc_probe_mfma(int form, double* tflops)
     (void)hipEventCreate(&e1);
     float ms = 0.f;
     for (int r = 0; r < 3 && !rc; r++) {  // the last of three launches is timed
-        hipEventRecord(e0, st);
+        (void)hipEventRecord(e0, st);
         if (form == 0) k_mfma_loop<0><<<blocks, 256, 0, st>>>(out, iters, 1e-3);
         else k_mfma_loop<1><<<blocks, 256, 0, st>>>(out, iters, 1e-3);
-        hipEventRecord(e1, st);
+        (void)hipEventRecord(e1, st);
         if (hipEventSynchronize(e1) != hipSuccess || hipGetLastError() != hipSuccess)
             rc = set_err(DQMC_ERR_HIP, "probe launch");
-        hipEventElapsedTime(&ms, e0, e1);
+        (void)hipEventElapsedTime(&ms, e0, e1);
     }
     const double flop_per_inst = form == 0 ? 2.0 * 16 * 16 * 4 : 4 * 2.0 * 4 * 4 * 4;
     if (!rc) *tflops = (double)blocks * 4 * iters * 8 * flop_per_inst / (ms * 1e-3) * 1e-12;
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
     (void)hipStreamDestroy(st);
     (void)hipFree(out);
     return rc;

@@ -223,8 +223,9 @@ int dqmc_test_gemm(int is_complex, int opa, int opb, int M, int N, int K, const 
  * the published peak. */
 int dqmc_probe_mfma(int form, double *tflops);
 
-/* Replace chain c's numeric parameters (the exponentials, exp_lambda, del
- * and the Peierls phases) in an existing engine; chain -1: every chain (one
+/* Replace chain c's numeric parameters (the exponentials, exp_lambda, del,
+ * the Peierls phases and those of the two-hop bonds) in an existing engine;
+ * chain -1: every chain (one
  * upload, copied on the device).  N, L, n_matmul and the element type must be
  * the engine's, and the maps are kept.  No counterpart
  * in the reference, whose parameters are fixed per file; it serves callers

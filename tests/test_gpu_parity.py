@@ -384,3 +384,29 @@ def test_set_params_per_chain_and_every_chain_match_fresh_engines():
         e1.close()
         e2.close()
         e3.close()
+
+
+def test_set_params_refreshes_two_hop_phases():
+    """Complex chains with the two-hop bond groups: an engine built at nflux=1
+    and given nflux=2's parameters (Peierls phases and two-hop phases pp)
+    measures what an engine built at nflux=2 measures."""
+    kw = dict(Nx=4, Ny=4, mu=-0.3, tp=-0.1, U=4.0, L=10, n_matmul=5, period_eqlt=5,
+              period_uneqlt=1, n_sweep_warm=0, n_sweep_meas=4, meas_2bond_corr=1,
+              meas_thermal=1)
+    a = gen.create_batch(Nfiles=2, seed=61, nflux=1, **kw)
+    b = gen.create_batch(Nfiles=2, seed=61, nflux=2, **kw)
+    e1 = engine.Engine(a)
+    e2 = engine.Engine(b)
+    try:
+        e1.set_params(range(2), b[0]["params"])
+        e1.run(2)
+        e2.run(2)
+        for c in range(2):
+            assert np.array_equal(e1.get_state(c)["hs"], e2.get_state(c)["hs"])
+            _, u1 = e1.get_meas(c)
+            _, u2 = e2.get_meas(c)
+            for k in ("j2j2", "pair_b2b2", "j2j", "jnjn", "zz"):
+                assert rel(u1[k], u2[k]) < 1e-12, k
+    finally:
+        e1.close()
+        e2.close()
