@@ -238,7 +238,8 @@ def bench_config4(args):
     group where it does not divide a chain's L; then the delay rank
     (buckets.tune_delay_rank: the fastest compiled rank of 4, 8, 16, 32;
     --no-tune: each file's n_delay = min(32, N), as compiled); eq
-    measurements every n_matmul slices.  The engines run concurrently."""
+    measurements every n_matmul slices.  The engines run concurrently, each
+    sweep one HIP-graph launch per engine (captured before the threads start)."""
     from dqmc_amd import buckets, gen
     nch = args.chains if args.chains != 256 else 128
     r = __import__("numpy").random.default_rng(99)
@@ -275,11 +276,19 @@ def bench_config4(args):
             tuned[-1].update(delay_rank=kd, rank_trials=[[k, round(t, 4)] for k, t in ktable])
             print("config 4: bucket %d delay rank trials (rank, s/sweep): %s -> %d"
                   % (bk, ktable, kd), file=sys.stderr, flush=True)
-        # the bucket engines run from concurrent host threads: no graph capture
-        engs.append(buckets.PaddedBatch(groups[bk], nm, graphs=False, delay_rank=kd))
+        engs.append(buckets.PaddedBatch(groups[bk], nm, graphs=not args.no_graphs,
+                                        delay_rank=kd))
         print("config 4: bucket %d: %d chains padded to N=%d L=%d, n_matmul %d, delay rank %d" %
               (bk, len(groups[bk]), engs[-1].N, engs[-1].L, nm, engs[-1].engine.delay_rank()),
               file=sys.stderr, flush=True)
+    if not args.no_graphs:
+        # capture each engine's sweep graph here, one engine at a time (stream
+        # capture fails while another host thread makes HIP calls); every sweep
+        # has the same launch pattern (equal-time measurements, no warm-up, no
+        # unequal-time), so the concurrent sweeps below only replay graphs
+        for e in engs:
+            e.run(0)
+            e.run(1)
     dt, n = _run_engines(engs, args.steps, args.warmup, len(engs))
     # each chain's own work (its N, L, the bucket's n_matmul, eq every n_matmul
     # slices); padding is overhead, not counted
