@@ -418,7 +418,7 @@ static int ue_measure(dqmc_engine* e)
         LCHK();
         ma.G0tT = dp<T>(e->T2);
     }
-    if (e->bond || e->thermal || e->pbbonly) {
+    if ((e->bond || e->thermal || e->pbbonly) && e->num_b > 0 && e->num_bb > 0) {
         ma.btab = dp<T>(e->btab);
         ma.bph = dp<T>(e->bph);
         k_bond_tab<T><<<dim3((e->num_b + 255) / 256, L, nc), 256, 0, e->st>>>(ma, dp<T>(e->btab),
