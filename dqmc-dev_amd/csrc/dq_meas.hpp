@@ -23,6 +23,7 @@ struct MeasArgs {
     const T* G0tT;                    // G(0,t)^T per slice, same layout (bond-pair kernels)
     const T* btab;                    // [b][t][f, b, n][num_b]: bond hop terms, densities
     const T* bph;                     // [c][pf_u, pb_u, pf_d, pb_d][num_b]: bond phases
+    const T* b2tab;                   // [b][t][f, b, n][num_b2]: the same for two-hop bonds
     const T* phase;                   // [c]
     const int* flag;                  // [c]: measure this chain now
     const int* rp_i; const int* ix_i; long srp_i, six_i;     // CSR of map_i  [c]

@@ -221,6 +221,47 @@ __global__ __launch_bounds__(256) void k_bond_tab(MeasArgs<T> a, T* btab, T* bph
     }
 }
 
+// two-hop bond c of spin s: phases from pp = [pp_u, ppr_u, pp_d, ppr_d][c]
+// and hop terms ht = [f, b, n][c] of the operator's slice (k_bond2_tab)
+template <typename T>
+__device__ inline BondOp<T> bond2_tab(const int* b2s, int nb2, int c, const T* pp, int s,
+                                      const T* ht)
+{
+    BondOp<T> o;
+    o.x0 = b2s[c];
+    o.x1 = b2s[c + nb2];
+    o.pf = pp[(2 * s) * nb2 + c];
+    o.pb = pp[(2 * s + 1) * nb2 + c];
+    o.f = ht[c];
+    o.bk = ht[nb2 + c];
+    o.n = ht[2 * nb2 + c];
+    return o;
+}
+
+// k_bond_tab for the two-hop bonds (phases from pp, no phase table)
+template <typename T>
+__global__ __launch_bounds__(256) void k_bond2_tab(MeasArgs<T> a, T* b2tab)
+{
+    const int c = blockIdx.z;
+    if (!a.flag[c]) return;
+    const int tt = blockIdx.y, nb2 = a.num_b2;
+    const int b = blockIdx.x * 256 + threadIdx.x;
+    if (b >= nb2) return;
+    const long N = a.N, NN = N * N;
+    const int* b2s = a.bond2s + c * a.sbond2s;
+    const T* pp = a.pp + c * a.spp;
+    const int x0 = b2s[b], x1 = b2s[b + nb2];
+#pragma unroll
+    for (int s = 0; s < 2; s++) {
+        const T pfv = pp[(2 * s) * nb2 + b], pbv = pp[(2 * s + 1) * nb2 + b];
+        const T* g = a.Gtt + (2 * c + s) * a.sU + NN * tt;
+        T* h = b2tab + ((long)(2 * c + s) * a.L + tt) * 3 * nb2;
+        h[b] = -(pfv * g[x1 + N * x0]);
+        h[nb2 + b] = -(pbv * g[x0 + N * x1]);
+        h[2 * nb2 + b] = from_re<T>(2.0) - g[x0 + N * x0] - g[x1 + N * x1];
+    }
+}
+
 // two-hop bond c: pp = [pp_u, ppr_u, pp_d, ppr_d][c]
 template <typename T>
 __device__ inline BondOp<T> bond2(const int* b2s, int nb2, int c, const T* pp, int s)
@@ -255,10 +296,20 @@ __global__ __launch_bounds__(256) void k_meas_ue_pairs(MeasArgs<T> a)
     const T phase = a.phase[c];
     T* acc = a.acc + c * a.sacc;
     const long* off = a.off;
-    const T* P[2] = {a.peierlsu + c * a.sP, a.peierlsd + c * a.sP};
     const int* bonds = a.bonds + c * a.sbonds;
     const int* b2s = a.bond2s + c * a.sbond2s;
     const T* pp = a.pp + c * a.spp;
+    // per-bond tables (k_bond_tab, k_bond2_tab) at slice t and at 0, per spin
+    const T* bph = a.bph ? a.bph + (long)c * 4 * nb : nullptr;
+    const T *T1t[2], *T1z[2], *T2t[2], *T2z[2];
+#pragma unroll
+    for (int s = 0; s < 2; s++) {
+        const long bt = (long)(2 * c + s) * a.L;
+        T1t[s] = a.btab ? a.btab + (bt + tt) * 3 * nb : nullptr;
+        T1z[s] = a.btab ? a.btab + bt * 3 * nb : nullptr;
+        T2t[s] = a.b2tab ? a.b2tab + (bt + tt) * 3 * nb2 : nullptr;
+        T2z[s] = a.b2tab ? a.b2tab + bt * 3 * nb2 : nullptr;
+    }
     SpinSlice<T> g[2];
 #pragma unroll
     for (int s = 0; s < 2; s++) {
@@ -285,11 +336,8 @@ __global__ __launch_bounds__(256) void k_meas_ue_pairs(MeasArgs<T> a)
             const int b = lin % nb, cc = lin / nb;
 #pragma unroll
             for (int s = 0; s < 2; s++) {
-                const T* bph = a.bph + (long)c * 4 * nb;
-                const T* ht = a.btab + ((long)(2 * c + s) * a.L + tt) * 3 * nb;
-                const T* h0 = a.btab + (long)(2 * c + s) * a.L * 3 * nb;
-                const BondOp<T> A = bond1_tab<T>(bonds, nb, b, bph, s, ht),
-                                B = bond1_tab<T>(bonds, nb, cc, bph, s, h0);
+                const BondOp<T> A = bond1_tab<T>(bonds, nb, b, bph, s, T1t[s]),
+                                B = bond1_tab<T>(bonds, nb, cc, bph, s, T1z[s]);
                 if (thermal) pair_vals<T, true, true, true>(g[s], A, B, v[s]);
                 else pair_vals<T, false, false, true>(g[s], A, B, v[s]);
             }
@@ -317,9 +365,10 @@ __global__ __launch_bounds__(256) void k_meas_ue_pairs(MeasArgs<T> a)
             const int b = lin % nb2, cc = lin / nb2;
 #pragma unroll
             for (int s = 0; s < 2; s++) {
-                const BondOp<T> A = bond2<T>(b2s, nb2, b, pp, s), B = bond1<T>(bonds, nb, cc, P[s], N);
-                if (thermal) pair_vals<T, false, true>(g[s], A, B, v[s]);
-                else pair_vals<T, false, false>(g[s], A, B, v[s]);
+                const BondOp<T> A = bond2_tab<T>(b2s, nb2, b, pp, s, T2t[s]),
+                                B = bond1_tab<T>(bonds, nb, cc, bph, s, T1z[s]);
+                if (thermal) pair_vals<T, false, true, true>(g[s], A, B, v[s]);
+                else pair_vals<T, false, false, true>(g[s], A, B, v[s]);
             }
             if (thermal) s0 += pre * corr_jjn(v);
             if (two) s1 += pre * corr_jj(v, false);
@@ -327,9 +376,10 @@ __global__ __launch_bounds__(256) void k_meas_ue_pairs(MeasArgs<T> a)
             const int b = lin % nb, cc = lin / nb;
 #pragma unroll
             for (int s = 0; s < 2; s++) {
-                const BondOp<T> A = bond1<T>(bonds, nb, b, P[s], N), B = bond2<T>(b2s, nb2, cc, pp, s);
-                if (thermal) pair_vals<T, true, false>(g[s], A, B, v[s]);
-                else pair_vals<T, false, false>(g[s], A, B, v[s]);
+                const BondOp<T> A = bond1_tab<T>(bonds, nb, b, bph, s, T1t[s]),
+                                B = bond2_tab<T>(b2s, nb2, cc, pp, s, T2z[s]);
+                if (thermal) pair_vals<T, true, false, true>(g[s], A, B, v[s]);
+                else pair_vals<T, false, false, true>(g[s], A, B, v[s]);
             }
             if (thermal) s0 += pre * corr_jnj(v);
             if (two) s1 += pre * corr_jj(v, false);
@@ -337,8 +387,9 @@ __global__ __launch_bounds__(256) void k_meas_ue_pairs(MeasArgs<T> a)
             const int b = lin % nb2, cc = lin / nb2;
 #pragma unroll
             for (int s = 0; s < 2; s++) {
-                const BondOp<T> A = bond2<T>(b2s, nb2, b, pp, s), B = bond2<T>(b2s, nb2, cc, pp, s);
-                pair_vals<T, false, false>(g[s], A, B, v[s]);
+                const BondOp<T> A = bond2_tab<T>(b2s, nb2, b, pp, s, T2t[s]),
+                                B = bond2_tab<T>(b2s, nb2, cc, pp, s, T2z[s]);
+                pair_vals<T, false, false, true>(g[s], A, B, v[s]);
             }
             const int i0 = b2s[b], i2 = b2s[b + nb2], j0 = b2s[cc], j2 = b2s[cc + nb2];
             const T* gu = g[0].t0;

@@ -408,6 +408,12 @@ static int ue_measure(dqmc_engine* e)
         k_meas_ue<T><<<dim3((e->num_ij + 255) / 256, L, nc), 256, 0, e->st>>>(ma);
     }
     LCHK();
+    // the pair kernels read the per-bond tables of the maps they walk
+    if ((e->bond || e->thermal || e->pbbonly) && e->num_bb > 0 && !e->btab)
+        return set_err(DQMC_ERR_UNSUPPORTED, "bond-pair measurements without bonds");
+    if ((e->thermal || e->twobond) && (e->num_bb2 > 0 || e->num_b2b > 0 || e->num_b2b2 > 0) &&
+        (!e->btab || !e->b2tab))
+        return set_err(DQMC_ERR_UNSUPPORTED, "two-hop bond measurements need bonds and two-hop bonds");
     if (e->bond || e->thermal || e->pbbonly || e->twobond) {
         // G(0,t)^T for the bond-pair kernels, into T2 (the expansion's
         // half-wrapped B_l, free until the next measurement rebuilds it)
@@ -417,13 +423,22 @@ static int ue_measure(dqmc_engine* e)
             e->N, tn, dp<T>(e->G0t), dp<T>(e->T2), (long)e->N * e->N);
         LCHK();
         ma.G0tT = dp<T>(e->T2);
+        // per-(chain, slice, bond) hop terms, densities and phases
+        if (e->btab) {
+            ma.btab = dp<T>(e->btab);
+            ma.bph = dp<T>(e->bph);
+            k_bond_tab<T><<<dim3((e->num_b + 255) / 256, L, nc), 256, 0, e->st>>>(
+                ma, dp<T>(e->btab), dp<T>(e->bph));
+            LCHK();
+        }
+        if (e->b2tab) {
+            ma.b2tab = dp<T>(e->b2tab);
+            k_bond2_tab<T><<<dim3((e->num_b2 + 255) / 256, L, nc), 256, 0, e->st>>>(
+                ma, dp<T>(e->b2tab));
+            LCHK();
+        }
     }
     if ((e->bond || e->thermal || e->pbbonly) && e->num_b > 0 && e->num_bb > 0) {
-        ma.btab = dp<T>(e->btab);
-        ma.bph = dp<T>(e->bph);
-        k_bond_tab<T><<<dim3((e->num_b + 255) / 256, L, nc), 256, 0, e->st>>>(ma, dp<T>(e->btab),
-                                                                           dp<T>(e->bph));
-        LCHK();
         k_meas_ue_pairs<T, PK_BB><<<dim3((e->num_bb + 255) / 256, L, nc), 256, 0, e->st>>>(ma);
         LCHK();
     }

@@ -189,7 +189,7 @@ struct dqmc_engine {
     long srp2[PM_COUNT] = {}, six2[PM_COUNT] = {};
     int *degen2 = 0, *bond2s = 0, *plaqs = 0;
     void *pp = 0, *gs = 0;
-    void *btab = 0, *bph = 0;  // bond-pair tables of the unequal-time measurement
+    void *btab = 0, *bph = 0, *b2tab = 0;  // bond-pair tables of the unequal-time measurement
     // adjoint outputs (allocated on first use)
     void *adjW = 0, *adjT = 0;
     double* adjL = 0;
@@ -1244,10 +1244,12 @@ static int engine_setup(dqmc_engine* e, const dqmc_params* const* P)
         DALLOC(e->Gtt, B * LNN * es);
         DALLOC(e->Gt0, B * LNN * es);
         RC(ue_alloc(e));
-        if ((e->bond || e->thermal || e->pbbonly) && e->num_b > 0) {
+        if ((e->bond || e->thermal || e->pbbonly || e->twobond) && e->num_b > 0) {
             DALLOC(e->btab, B * L * 3L * e->num_b * es);
             DALLOC(e->bph, (long)nc * 4 * e->num_b * es);
         }
+        if ((e->thermal || e->twobond) && e->num_b2 > 0)
+            DALLOC(e->b2tab, B * L * 3L * e->num_b2 * es);
     }
     // per-chain parameters
     std::vector<int> rp, ix, deg(4 * nc);
