@@ -203,6 +203,8 @@ def bench_config3(args):
     kw = dict(Nx=12, Ny=12, tp=0.0, U=4.0, mu=-0.5, dt=0.1, L=60, nflux=1, n_matmul=10,
               n_delay=16, period_eqlt=10, period_uneqlt=2, n_sweep_warm=0,
               n_sweep_meas=100000, meas_bond_corr=1)
+    if args.all_bond_sets:  # also the thermal and two-hop (2-bond) correlators
+        kw.update(meas_thermal=1, meas_2bond_corr=1)
     eng = engine.Engine(gen.create_batch(Nfiles=1, seed=3, **kw), graphs=not args.no_graphs)
     dt1, n1 = _run_engines([eng], max(2, args.steps), 1, 1)
     eng.close()
@@ -214,7 +216,8 @@ def bench_config3(args):
                  nbs / dtb, "sweeps/s", args.steps, args.warmup, dtb,
                  "12x12 square Hubbard, nflux=1 (Landau gauge Peierls phases), complex128, "
                  "N=144 L=60 U=4 mu=-0.5 beta=6 n_matmul=10 delay=16, eq every 10 slices, "
-                 "uneq + bond correlators every 2 sweeps",
+                 "uneq + bond correlators every 2 sweeps"
+                 + (" (+ thermal and 2-bond sets)" if args.all_bond_sets else ""),
                  dict({"single_chain_sweeps_per_s": round(n1 / dt1, 4),
                        "batched_over_single": round(nbs / dtb / (n1 / dt1), 3), "dtype": "c128",
                        "model_acceptance_assumed": 0.5},
@@ -360,6 +363,8 @@ def main():
     ap.add_argument("--no-graphs", action="store_true",
                     help="launch the sweeps kernel by kernel instead of replaying HIP graphs "
                          "(for profilers: rocprofv3 --kernel-trace crashed on graph launches)")
+    ap.add_argument("--all-bond-sets", action="store_true",
+                    help="--config 3: also measure the thermal and two-hop bond correlators")
     ap.add_argument("--no-tune", action="store_true",
                     help="--config 4: fixed n_matmul per bucket instead of the autotuner")
     ap.add_argument("--threads", type=int, default=8,
