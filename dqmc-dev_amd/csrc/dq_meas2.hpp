@@ -446,8 +446,8 @@ __global__ __launch_bounds__(256) void k_meas_ue_kv(MeasArgs<T> a)
     const long NN = (long)N * N;
     const T phase = a.phase[c];
     T* acc = a.acc + c * a.sacc;
-    const T* P[2] = {a.peierlsu + c * a.sP, a.peierlsd + c * a.sP};
     const int* bonds = a.bonds + c * a.sbonds;
+    const T* bph = a.bph + (long)c * 4 * nb;
     const int* rp = a.rp_bs + c * a.srp_bs;
     const int* ix = a.ix_bs + c * a.six_bs;
     const T pre = phase * (1.0 / a.degen[4 * c + 3]);
@@ -460,16 +460,17 @@ __global__ __launch_bounds__(256) void k_meas_ue_kv(MeasArgs<T> a)
 #pragma unroll
         for (int s = 0; s < 2; s++) {
             const long bo = (2 * c + s) * a.sU;
-            const T* gtt = a.Gtt + bo + NN * tt;
             const T* goo = a.Gtt + bo;
-            const T* got = a.G0t + bo + NN * tt;
+            const T* gotT = a.G0tT + bo + NN * tt;  // G(0,t)^T
             const T* gt0 = a.Gt0 + bo + NN * tt;
-            const BondOp<T> A = bond1<T>(bonds, nb, b, P[s], N);
-            k[s] = -(A.pf * gtt[A.x1 + N * (long)A.x0]) - A.pb * gtt[A.x0 + N * (long)A.x1];
+            // the bond's phases and hop terms at t from k_bond_tab's tables
+            const BondOp<T> A =
+                bond1_tab<T>(bonds, nb, b, bph, s, a.btab + ((long)(2 * c + s) * a.L + tt) * 3 * nb);
+            k[s] = A.f + A.bk;
             n[s] = from_re<T>(1.0) - goo[j + N * (long)j];
             // connected <c+_x c_y (t) n_j (0)> = (dt d_xj - G0t[j,x]) Gt0[y,j]
-            cn[s] = A.pf * ((from_re<T>(A.x0 == j ? dt : 0.0) - got[j + N * (long)A.x0]) * gt0[A.x1 + N * (long)j]) +
-                    A.pb * ((from_re<T>(A.x1 == j ? dt : 0.0) - got[j + N * (long)A.x1]) * gt0[A.x0 + N * (long)j]);
+            cn[s] = A.pf * ((from_re<T>(A.x0 == j ? dt : 0.0) - gotT[A.x0 + N * (long)j]) * gt0[A.x1 + N * (long)j]) +
+                    A.pb * ((from_re<T>(A.x1 == j ? dt : 0.0) - gotT[A.x1 + N * (long)j]) * gt0[A.x0 + N * (long)j]);
         }
         kv += pre * ((k[0] * n[0] + cn[0]) * n[1] + (k[1] * n[1] + cn[1]) * n[0]);
         kn += pre * ((k[0] + k[1]) * (n[0] + n[1]) + cn[0] + cn[1]);

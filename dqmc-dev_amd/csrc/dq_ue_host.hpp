@@ -409,12 +409,13 @@ static int ue_measure(dqmc_engine* e)
     }
     LCHK();
     // the pair kernels read the per-bond tables of the maps they walk
-    if ((e->bond || e->thermal || e->pbbonly) && e->num_bb > 0 && !e->btab)
+    if (((e->bond || e->thermal || e->pbbonly) && e->num_bb > 0 && !e->btab) ||
+        (e->energy && e->num_bs > 0 && !e->btab))
         return set_err(DQMC_ERR_UNSUPPORTED, "bond-pair measurements without bonds");
     if ((e->thermal || e->twobond) && (e->num_bb2 > 0 || e->num_b2b > 0 || e->num_b2b2 > 0) &&
         (!e->btab || !e->b2tab))
         return set_err(DQMC_ERR_UNSUPPORTED, "two-hop bond measurements need bonds and two-hop bonds");
-    if (e->bond || e->thermal || e->pbbonly || e->twobond) {
+    if (e->bond || e->thermal || e->pbbonly || e->twobond || e->energy) {
         // G(0,t)^T for the bond-pair kernels, into T2 (the expansion's
         // half-wrapped B_l, free until the next measurement rebuilds it)
         const int tn = (e->N + 31) / 32;

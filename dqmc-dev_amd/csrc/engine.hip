@@ -1244,7 +1244,7 @@ static int engine_setup(dqmc_engine* e, const dqmc_params* const* P)
         DALLOC(e->Gtt, B * LNN * es);
         DALLOC(e->Gt0, B * LNN * es);
         RC(ue_alloc(e));
-        if ((e->bond || e->thermal || e->pbbonly || e->twobond) && e->num_b > 0) {
+        if ((e->bond || e->thermal || e->pbbonly || e->twobond || e->energy) && e->num_b > 0) {
             DALLOC(e->btab, B * L * 3L * e->num_b * es);
             DALLOC(e->bph, (long)nc * 4 * e->num_b * es);
         }
