@@ -16,6 +16,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1672,6 +1673,111 @@ extern "C" int dqmc_engine_calc_eq_g(dqmc_engine* e, int c, int spin, int f, dou
     HIPCHK(hipSetDevice(e->device));
     return e->cplx ? calc_eq_g_hook<zd>(e, c, spin, f, G, phase)
                    : calc_eq_g_hook<double>(e, c, spin, f, G, phase);
+}
+
+// --------------------------------------------------------------------------
+// calc_eq_g without allocation: the work space of calc_eq_g lives in one
+// file-scope device array per GPU.  In host code `scratch` is only a stand-in
+// for the symbol; its device address comes from hipGetSymbolAddress, once per
+// device, and only that address is passed to kernels.
+// --------------------------------------------------------------------------
+static __device__ __align__(256) unsigned char scratch[DQMC_NOALLOC_SCRATCH_BYTES];
+
+#define NOALLOC_MAX_DEVICES 64
+static std::mutex g_scratch_mu;
+static void* g_scratch_dev[NOALLOC_MAX_DEVICES];
+
+// device address of this GPU's `scratch` (the current device), resolved on
+// the first call on that device
+static int scratch_address(void** out)
+{
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= NOALLOC_MAX_DEVICES)
+        return set_err(DQMC_ERR_UNSUPPORTED, "device %d beyond %d", dev, NOALLOC_MAX_DEVICES);
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void* p = nullptr;
+        const hipError_t err = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (err != hipSuccess || !p)
+            return set_err(DQMC_ERR_HIP, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(err));
+        g_scratch_dev[dev] = p;
+    }
+    *out = g_scratch_dev[dev];
+    return 0;
+}
+
+// Carves calc_eq_g's work space for `batch` matrices of order N out of the
+// region at `base` (nullptr: only count); returns the bytes used.
+static size_t eqg_carve(dqmc_engine* e, char* base, int N, size_t es, int batch)
+{
+    const long NN = (long)N * N, B = batch;
+    size_t off = 0;
+    auto take = [&](size_t bytes) -> void* {
+        void* p = base ? (void*)(base + off) : nullptr;
+        off += (bytes + 255) & ~(size_t)255;
+        return p;
+    };
+    e->sVb = NN;
+    e->sTb = 2L * N * 64 + 64 * 64;
+    e->sWq = 64L * 2 * N;
+    void** mats[] = {&e->X, &e->Y, &e->Y2, &e->Z, &e->Q, &e->Tm, &e->Rs, &e->Tp, &e->Mm};
+    for (void** m : mats) *m = take(B * NN * es);
+    e->tau = take(B * 2 * N * es);
+    e->Vb = take(B * e->sVb * es);
+    e->Tb = take(B * e->sTb * es);
+    e->Wq1 = take(B * e->sWq * es);
+    e->Wq2 = take(B * e->sWq * es);
+    e->detq = take(B * es);
+    e->detm = take(B * es);
+    e->jpvt = (int*)take(B * N * 4);
+    e->perm = (int*)take(B * N * 4);
+    e->dvec = (double*)take(B * N * 8);
+    e->vvec = (double*)take(B * N * 8);
+    return off;
+}
+
+__global__ __launch_bounds__(256) void k_zero_u64(unsigned long long* p, long n)
+{
+    for (long i = blockIdx.x * 256L + threadIdx.x; i < n; i += (long)gridDim.x * 256) p[i] = 0ull;
+}
+
+template <typename T>
+static int calc_eq_g_noalloc_t(int N, int F, int n_chains, int f, const double* C, double* G,
+                               double* phase, hipStream_t st)
+{
+    dqmc_engine tmp;  // no allocation: every buffer points into the caller's or `scratch`
+    dqmc_engine* e = &tmp;
+    e->nc = n_chains; e->N = N; e->L = F; e->F = F; e->n_matmul = 1;
+    e->cplx = is_cplx<T>::value; e->esz = sizeof(T);
+    e->st = st;
+    e->gemm_tile = GT_64;
+    const size_t need = eqg_carve(e, nullptr, N, sizeof(T), 2 * n_chains);
+    if (need > (size_t)DQMC_NOALLOC_SCRATCH_BYTES)
+        return set_err(DQMC_ERR_ARG,
+                       "calc_eq_g of %d matrices of order %d needs %zu bytes of work space; "
+                       "the static scratch holds %zu", 2 * n_chains, N, need,
+                       (size_t)DQMC_NOALLOC_SCRATCH_BYTES);
+    void* base = nullptr;
+    RC(scratch_address(&base));
+    eqg_carve(e, (char*)base, N, sizeof(T), 2 * n_chains);
+    // every call starts from a cleared work space, as a freshly created engine
+    // does: the previous call may have left other contents at other strides
+    const long n64 = (long)(need / 8);
+    k_zero_u64<<<nblk(n64, 256), 256, 0, st>>>((unsigned long long*)base, n64);
+    LCHK();
+    e->C = (void*)C;
+    return calc_eq_g<T>(e, f, (T*)G, (T*)phase);
+}
+
+extern "C" int dqmc_calc_eq_g_noalloc(int is_complex, int N, int F, int n_chains, int f,
+                                      const double* C, double* G, double* phase, void* stream)
+{
+    if (N <= 0 || N > QR_MAXM || F <= 0 || n_chains <= 0 || f < 0 || f >= F || !C || !G || !phase)
+        return set_err(DQMC_ERR_ARG, "bad arguments (N=%d F=%d n_chains=%d f=%d)", N, F, n_chains, f);
+    hipStream_t st = (hipStream_t)stream;
+    return is_complex ? calc_eq_g_noalloc_t<zd>(N, F, n_chains, f, C, G, phase, st)
+                      : calc_eq_g_noalloc_t<double>(N, F, n_chains, f, C, G, phase, st);
 }
 
 extern "C" int dqmc_engine_calc_ue_g(dqmc_engine* e, int c, int spin, double* G0t, double* Gtt,

@@ -248,6 +248,28 @@ int dqmc_engine_gemm_tile(dqmc_engine *e);
  * src/updates.c:7, after rounding to a compiled variant). */
 int dqmc_engine_delay_rank(dqmc_engine *e);
 
+/* Stabilized equal-time Green's functions from caller-held group products,
+ * allocating no memory.  It has no counterpart in the reference.  It is for
+ * callers that cannot allocate device memory: code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc.  It computes what
+ * dqmc_engine_calc_eq_g computes (calc_eq_g, src/greens.c:69-208) for the
+ * 2*n_chains matrices b = 2*chain + spin at once:
+ *   C      device, [2 n_chains][F][N*N] num: C_f = B_{(f+1)n-1} .. B_{fn},
+ *          column-major (the group products of mul_seq, src/greens.c:8-41)
+ *   G      device, [2 n_chains][N*N] num  <- (I + C_{f-1} .. C_f)^-1 (cyclic)
+ *   phase  device, [2 n_chains] num       <- phase of det(I + C_{f-1} .. C_f)
+ *   stream a hipStream_t (NULL: the null stream); the call only queues work
+ *          on it and returns.
+ * The work space is one static device array of DQMC_NOALLOC_SCRATCH_BYTES on
+ * each GPU, held until the process ends and cleared at the start of every
+ * call.  Every call on a GPU shares that GPU's array: calls that overlap on
+ * one GPU race.  A request whose work space exceeds the array fails with
+ * DQMC_ERR_ARG (about 11 N*N num per matrix: real N = 256 fits 5 chains,
+ * complex N = 144 fits 7, real N = 64 fits 59). */
+#define DQMC_NOALLOC_SCRATCH_BYTES (64L << 20)
+int dqmc_calc_eq_g_noalloc(int is_complex, int N, int F, int n_chains, int f, const double *C,
+                           double *G, double *phase, void *stream);
+
 const char *dqmc_last_error(void);
 const char *dqmc_version(void);
 
