@@ -361,11 +361,16 @@ MEAS_TOGGLES = ("meas_bond_corr", "meas_thermal", "meas_2bond_corr", "meas_energ
 def create_1(init_rng=None, geometry="square", bc=1, Nx=4, Ny=4, mu=0.0, tp=0.0, tpp=0.0,
              U=6.0, dt=0.1, L=40, nflux=0, h=0.0, n_delay=16, n_matmul=8,
              n_sweep_warm=200, n_sweep_meas=2000, period_eqlt=8, period_uneqlt=0,
-             trans_sym=1, checkpoint_every=10000, twistx=0.0, twisty=0.0, **toggles):
+             trans_sym=1, checkpoint_every=10000, twistx=0.0, twisty=0.0, alpha=0.5,
+             **toggles):
     """Return the simulation contents as {"params": {...}, "state": {...},
     "meas_eqlt": {...}, "meas_uneqlt": {...}, "metadata": {...}}.
 
-    Defaults and assertions match util/gen_1band_unified_hub.py:265-310."""
+    Defaults and assertions match util/gen_1band_unified_hub.py:265-310.
+    alpha is the gauge of the Peierls phases: 1/2 (symmetric), the value the
+    reference's generator always passes (util/gen_1band_unified_hub.py:643),
+    or 1 (Landau).  With t' = 0 the two are related by a site-diagonal unitary
+    and give the same gauge-invariant physics."""
     for k in toggles:
         if k not in MEAS_TOGGLES:
             raise TypeError("unknown argument %r" % k)
@@ -388,12 +393,12 @@ def create_1(init_rng=None, geometry="square", bc=1, Nx=4, Ny=4, mu=0.0, tp=0.0,
         raise ValueError("Can't measure generalized susceptibility without trans_sym")
     if geometry == "square":
         maps = _square_maps(Nx, Ny, tp, trans_sym)
-        kij, peierls = hopping_square(Nx, Ny, t=1.0, tp=tp, tpp=tpp, nflux=nflux, alpha=0.5,
+        kij, peierls = hopping_square(Nx, Ny, t=1.0, tp=tp, tpp=tpp, nflux=nflux, alpha=alpha,
                                       twistx=twistx, twisty=twisty)
         thermal = _thermal_phases(maps, peierls, tp, N)
     elif geometry == "triangular":
         maps = _square_maps(Nx, Ny, 0.0, trans_sym)  # placeholder maps (trajectory tests only)
-        kij, peierls = hopping_triangular(Nx, Ny, t=1.0, nflux=nflux)
+        kij, peierls = hopping_triangular(Nx, Ny, t=1.0, nflux=nflux, alpha=alpha)
         thermal = np.ones((maps["b2ps"], N), dtype=np.complex128)
     else:
         raise NotImplementedError("geometry %r" % geometry)
@@ -522,6 +527,7 @@ def create_1(init_rng=None, geometry="square", bc=1, Nx=4, Ny=4, mu=0.0, tp=0.0,
             "bps": maps["bps"], "b2ps": maps["b2ps"], "plaq_per_cell": 2, "U": U, "t'": tp,
             "t''": tpp, "nflux": nflux, "twistx": twistx, "twisty": twisty, "mu": mu, "h": h,
             "beta": L * dt, "trans_sym": trans_sym, "geometry": geometry, "bc": bc,
+            "gauge_alpha": alpha,
             "commit": "dqmc-mi355x", "version": 0.1}
     return {"params": p, "state": state, "meas_eqlt": meq, "meas_uneqlt": mue, "metadata": meta}
 

@@ -82,3 +82,58 @@ def test_end_to_end_hs_trajectory(golden, geometry):
         m = out["meas_eqlt"]
         assert m["n_sample"] == 20 * 5
         np.testing.assert_allclose(m["density"] / m["sign"], 1.0, atol=1e-12)
+
+
+def test_complex_build_on_zero_phases_reproduces_the_real_golden(golden):
+    """The complex update path (src/updates.c:7-84 under USE_CPLX: complex
+    ratios, the phase p/|p| of each accepted flip, complex wraps and QRs) on
+    the golden file with every num array complex and imaginary parts zero
+    must walk the same trajectory as the real build: hs_end["square"] of
+    test/hs_ref.py, bit for bit."""
+    import simtools
+    s = simtools.as_complex(gen.create_batch(seed=1234, n_sweep_warm=20, n_sweep_meas=20)[0])
+    out = oracle.run(s)
+    assert np.array_equal(out["state"]["hs"], np.array(golden["hs_end"]["square"]))
+    real = oracle.run(gen.create_batch(seed=1234, n_sweep_warm=20, n_sweep_meas=20)[0])
+    assert [int(v) for v in out["state"]["rng"]] == [int(v) for v in real["state"]["rng"]]
+    assert out["n_acc"] == real["n_acc"]
+    for k in ("density", "double_occ", "nn", "xx", "zz", "pair_sw", "g00"):
+        v = np.asarray(out["meas_eqlt"][k])
+        assert np.abs(v.imag).max() < 1e-12, k
+        w = np.asarray(real["meas_eqlt"][k])
+        assert np.abs(v.real - w).max() <= 1e-9 * np.abs(w).max(), k  # rounding of zgemm vs dgemm
+
+
+def test_complex_trajectory_is_gauge_invariant():
+    """Peierls phases in the symmetric (alpha = 1/2) and the Landau (alpha = 1)
+    gauge are related by a site-diagonal unitary D at t' = 0: exp_K' =
+    D exp_K D^*, so G' = D G D^*, every Metropolis ratio (which reads only
+    diag G) and the weight's phase are unchanged, and the complex trajectory
+    must be identical; gauge-invariant accumulators (phase-weighted hops,
+    densities, spins) agree.  The pair fields are not gauge invariant, and
+    the phases g00 / gt0 attach to far site pairs are not covariant in the
+    reference's construction (simtools.GAUGE_VARIANT); they are skipped.  A
+    misplaced Peierls phase or a conjugation error in the complex path breaks
+    this."""
+    import simtools
+    kw = dict(Nx=4, Ny=6, nflux=1, tp=0.0, U=4.0, mu=-0.4, L=20, n_matmul=5, period_eqlt=5,
+              period_uneqlt=2, n_sweep_warm=2, n_sweep_meas=6, meas_bond_corr=1,
+              meas_energy_corr=1, meas_thermal=1, meas_2bond_corr=1)
+    a = oracle.run(gen.create_batch(seed=77, alpha=0.5, **kw)[0])
+    b = oracle.run(gen.create_batch(seed=77, alpha=1.0, **kw)[0])
+    assert np.array_equal(a["state"]["hs"], b["state"]["hs"])
+    assert a["n_acc"] == b["n_acc"] and a["n_acc"] > 0
+    assert abs(a["phase"] - b["phase"]) < 1e-10
+    checked = 0
+    for grp in ("meas_eqlt", "meas_uneqlt"):
+        for k, v in a[grp].items():
+            if k in ("n_sample",) or k in simtools.GAUGE_VARIANT:
+                continue
+            w = np.asarray(b[grp][k])
+            v = np.asarray(v)
+            assert np.abs(v - w).max() <= 1e-9 * max(np.abs(v).max(), 1.0), (grp, k)
+            checked += 1
+    assert checked > 20
+    # the pair fields do change with the gauge (the test would see a dropped phase)
+    assert np.abs(np.asarray(a["meas_eqlt"]["pair_sw"]) -
+                  np.asarray(b["meas_eqlt"]["pair_sw"])).max() > 1e-6
