@@ -113,3 +113,28 @@ def test_measure_eqlt_particle_hole_half_filling():
     assert np.abs(gd - (np.eye(16) - D @ gu.T @ D)).max() < 1e-9
     m = oracle.measure_eqlt(sim, 1.0, gu, gd)
     assert abs(m["density"][0] - 1.0) < 1e-12
+
+
+@pytest.mark.parametrize("U", [0.0, 4.0])
+def test_stable_unequal_time_referee(U):
+    """The referee used at full size (tests/simtools.ue_referee: UDV forms with
+    a QR after every slice, sums inverted with the scale splitting) agrees
+    with calc_ue_g where that is exact (small N, short blocks), and at U = 0
+    with the closed form G(t,0) = e^{-tK} (I + e^{-beta K})^-1."""
+    import simtools
+    sim = _sim(U=U, period_uneqlt=1, L=40, n_matmul=8)
+    taus = [0, 1, 7, 8, 9, 20, 39]
+    for spin in (0, 1):
+        G0t, Gtt, Gt0 = oracle.calc_ue_g(sim, spin)
+        gt0, g0t, gtt = simtools.ue_referee(sim, spin, taus)
+        for t in taus:
+            assert np.abs(gt0[t] - Gt0[t]).max() < 1e-10 * np.abs(Gt0[t]).max()
+            assert np.abs(gtt[t] - Gtt[t]).max() < 1e-10 * np.abs(Gtt[t]).max()
+            if t:
+                assert np.abs(g0t[t] - G0t[t]).max() < 1e-10 * np.abs(G0t[t]).max()
+        if U == 0.0:
+            K = (sim["params"]["Kd"] if spin else sim["params"]["Ku"]).T
+            G0 = np.linalg.inv(np.eye(16) + expm(-4.0 * K))
+            for t in taus:
+                want = expm(-0.1 * t * K) @ G0
+                assert np.abs(gt0[t] - want).max() < 1e-10 * np.abs(want).max()
