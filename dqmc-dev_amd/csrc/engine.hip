@@ -201,6 +201,11 @@ struct dqmc_engine {
     std::vector<int> Lc, Nc;
     int *d_Lc = 0, *d_Nc = 0;
     int* flag_sl = 0;    // [c] per-slice equal-time measurement flags
+    // equal-time cadence per chain (padded engines: each chain keeps its
+    // file's period_eqlt) and the slices at which any chain measures
+    std::vector<int> pe;
+    int* d_pe = 0;
+    std::vector<char> meas_at;
     // blocked QR workspace: explicit V, compact-WY T factors, GEMM scratch, dets
     void *Vb = 0, *Tb = 0, *Wq1 = 0, *Wq2 = 0, *detq = 0, *detm = 0;
     long sVb = 0, sTb = 0, sWq = 0;
@@ -214,6 +219,10 @@ struct dqmc_engine {
     bool use_graph = true;
     hipGraph_t graph[4] = {};
     hipGraphExec_t gexec[4] = {};
+    // the host queues sweep s only after sweep s - 2 has finished on the
+    // device, so a stop flag set while the sweeps run is seen within two
+    // sweeps (the reference polls before every sweep, src/dqmc.c:224-230)
+    hipEvent_t done[2] = {};
     // CHECK_G_WRP (src/dqmc.c:277-287): at every recalculation the wrapped G
     // (into Gw) against the recomputed one; per chain the largest |difference|
     bool check_drift = false;
@@ -662,11 +671,13 @@ __global__ void k_sweep_flags(int nc, int period_uneqlt, const int* sweep, const
     flag_ue[c] = act && s >= warm[c] && period_uneqlt > 0 && (s % period_uneqlt) == 0;
 }
 
-// equal-time measurement flags of slice l for chains padded to a common L
-__global__ void k_flag_slice(int nc, const int* flag_eq, const int* Lc, int l, int* out)
+// equal-time measurement flags of slice l for chains padded to a common L,
+// each at its own cadence pe[c] (src/dqmc.c:361: (l + 1) % period_eqlt == 0)
+__global__ void k_flag_slice(int nc, const int* flag_eq, const int* Lc, const int* pe, int l,
+                             int* out)
 {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c < nc) out[c] = flag_eq[c] && l < Lc[c];
+    if (c < nc) out[c] = flag_eq[c] && l < Lc[c] && pe[c] > 0 && (l + 1) % pe[c] == 0;
 }
 
 __global__ void k_sweep_advance(int nc, int* sweep, const int* active)
@@ -958,7 +969,7 @@ static int sweep_body(dqmc_engine* e, bool any_eq, bool any_ue)
             RC(gemm<T>(e, OP_N, OP_N, N, N, N, one, dp<T>(e->expK), N, NN, T1, N, NN, zero, G,
                        N, NN, B, nullptr, 0, e->escale + (long)N * l, LN, nullptr, 0, l, 1));
         }
-        if (any_eq && (l + 1) % e->period_eqlt == 0) {
+        if (any_eq && e->meas_at[l]) {
             // half-wrap e^{dtK/2} G e^{-dtK/2} (src/dqmc.c:364-380), into Y2
             RC(gemm<T>(e, OP_N, OP_N, N, N, N, one, G, N, NN, dp<T>(e->ehalf), N, NN, zero, T1,
                        N, NN, B));
@@ -967,9 +978,9 @@ static int sweep_body(dqmc_engine* e, bool any_eq, bool any_ue)
             MeasArgs<T> ma = meas_args<T>(e, 0);
             ma.G = dp<T>(e->Y2);
             ma.sG = NN;
-            if (e->ragged) {  // chains past their own L do not measure
-                k_flag_slice<<<(nc + 255) / 256, 256, 0, e->st>>>(nc, e->flag_eq, e->d_Lc, l,
-                                                                   e->flag_sl);
+            if (e->ragged) {  // chains past their own L, or not due at l, do not measure
+                k_flag_slice<<<(nc + 255) / 256, 256, 0, e->st>>>(nc, e->flag_eq, e->d_Lc, e->d_pe,
+                                                                   l, e->flag_sl);
                 LCHK();
                 ma.flag = e->flag_sl;
             }
@@ -996,6 +1007,7 @@ static int run_t(dqmc_engine* e, int n_sweeps, const volatile int* stop)
     const int nc = e->nc;
     if (e->stale) RC(init_green<T>(e));
     for (int s = 0; s < n_sweeps; s++) {
+        if (s >= 2) HIPCHK(hipEventSynchronize(e->done[s & 1]));  // sweep s - 2 has ended
         if (stop && *stop) {
             HIPCHK(hipStreamSynchronize(e->st));
             return DQMC_INTERRUPTED;
@@ -1004,7 +1016,7 @@ static int run_t(dqmc_engine* e, int n_sweeps, const volatile int* stop)
         for (int c = 0; c < nc; c++) {
             const bool act = e->sweep[c] < e->n_sweep[c];
             any_active |= act;
-            any_eq |= act && e->sweep[c] >= e->n_sweep_warm[c] && e->period_eqlt > 0;
+            any_eq |= act && e->sweep[c] >= e->n_sweep_warm[c] && e->pe[c] > 0;
             any_ue |= act && e->sweep[c] >= e->n_sweep_warm[c] && e->period_uneqlt > 0 &&
                       e->sweep[c] % e->period_uneqlt == 0;
         }
@@ -1022,17 +1034,20 @@ static int run_t(dqmc_engine* e, int n_sweeps, const volatile int* stop)
                 }
                 if (ce != hipSuccess)
                     return set_err(DQMC_ERR_HIP, "sweep capture: %s", hipGetErrorString(ce));
-                e->graph[key] = g;
-                const hipError_t ie = hipGraphInstantiate(&e->gexec[key], g, nullptr, nullptr, 0);
+                hipGraphExec_t x = nullptr;
+                const hipError_t ie = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
                 if (ie != hipSuccess) {
-                    e->gexec[key] = nullptr;
+                    (void)hipGraphDestroy(g);
                     return set_err(DQMC_ERR_HIP, "sweep graph: %s", hipGetErrorString(ie));
                 }
+                e->graph[key] = g;
+                e->gexec[key] = x;
             }
             HIPCHK(hipGraphLaunch(e->gexec[key], e->st));
         } else {
             RC(sweep_body<T>(e, any_eq, any_ue));
         }
+        HIPCHK(hipEventRecord(e->done[s & 1], e->st));
         for (int c = 0; c < nc; c++)
             if (e->sweep[c] < e->n_sweep[c]) e->sweep[c]++;
     }
@@ -1043,10 +1058,12 @@ static int run_t(dqmc_engine* e, int n_sweeps, const volatile int* stop)
 // --------------------------------------------------------------------------
 // C ABI
 // --------------------------------------------------------------------------
-static int same_shape(const dqmc_params* a, const dqmc_params* b)
+// chains of one engine: equal shapes and measurement sets; the equal-time
+// cadence may differ between chains padded to a common size (padded)
+static int same_shape(const dqmc_params* a, const dqmc_params* b, bool padded)
 {
     return a->N == b->N && a->L == b->L && a->F == b->F && a->n_matmul == b->n_matmul &&
-           a->is_complex == b->is_complex && a->period_eqlt == b->period_eqlt &&
+           a->is_complex == b->is_complex && (padded || a->period_eqlt == b->period_eqlt) &&
            a->period_uneqlt == b->period_uneqlt && a->num_i == b->num_i &&
            a->num_ij == b->num_ij && a->meas_energy_corr == b->meas_energy_corr &&
            a->meas_bond_corr == b->meas_bond_corr && a->meas_thermal == b->meas_thermal &&
@@ -1321,6 +1338,8 @@ static int engine_setup(dqmc_engine* e, const dqmc_params* const* P)
         DALLOC(e->d_Lc, nc * 4);
         DALLOC(e->d_Nc, nc * 4);
         DALLOC(e->flag_sl, nc * 4);
+        DALLOC(e->d_pe, nc * 4);
+        HIPCHK(hipMemcpy(e->d_pe, e->pe.data(), nc * 4, hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(e->d_Lc, e->Lc.data(), nc * 4, hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(e->d_Nc, e->Nc.data(), nc * 4, hipMemcpyHostToDevice));
     }
@@ -1450,8 +1469,9 @@ extern "C" int dqmc_engine_create(dqmc_engine** out, const dqmc_params* const* P
         return set_err(DQMC_ERR_ARG, "bad shape N=%d L=%d n_matmul=%d F=%d", p0->N, p0->L,
                        p0->n_matmul, p0->F);
     if (p0->N > UPD_MAXN) return set_err(DQMC_ERR_ARG, "N=%d exceeds %d", p0->N, UPD_MAXN);
+    const bool padded = tune && (tune->chain_L || tune->chain_N);
     for (int c = 1; c < n_chains; c++)
-        if (!same_shape(p0, P[c]))
+        if (!same_shape(p0, P[c], padded))
             return set_err(DQMC_ERR_ARG, "chain %d has a different shape than chain 0", c);
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
@@ -1504,12 +1524,26 @@ extern "C" int dqmc_engine_create(dqmc_engine** out, const dqmc_params* const* P
                            "unequal-time measurements need chains of equal size (chain_L/chain_N)");
         }
     }
+    e->pe.resize(n_chains);
+    e->meas_at.assign(e->L, 0);
+    for (int c = 0; c < n_chains; c++) {
+        e->pe[c] = P[c]->period_eqlt;
+        const int lc = e->ragged ? e->Lc[c] : e->L;
+        for (int l = 0; l < lc; l++)
+            if (e->pe[c] > 0 && (l + 1) % e->pe[c] == 0) e->meas_at[l] = 1;
+    }
     e->sweep.assign(n_chains, 0);
     e->n_sweep.assign(n_chains, 0);
     e->n_sweep_warm.assign(n_chains, 0);
     if (hipStreamCreateWithFlags(&e->st, hipStreamNonBlocking) != hipSuccess) {
         delete e;
         return set_err(DQMC_ERR_HIP, "hipStreamCreate failed");
+    }
+    for (auto& ev : e->done) {
+        if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) {
+            dqmc_engine_destroy(e);
+            return set_err(DQMC_ERR_HIP, "hipEventCreate failed");
+        }
     }
     int rc = engine_setup(e, P);
     if (!rc) {
@@ -1539,6 +1573,8 @@ extern "C" void dqmc_engine_destroy(dqmc_engine* e)
         if (e->gexec[k]) (void)hipGraphExecDestroy(e->gexec[k]);
         if (e->graph[k]) (void)hipGraphDestroy(e->graph[k]);
     }
+    for (auto& ev : e->done)
+        if (ev) (void)hipEventDestroy(ev);
     for (void* p : e->allocs) (void)hipFree(p);
     if (e->st) (void)hipStreamDestroy(e->st);
     delete e;

@@ -15,9 +15,12 @@ engine of the bucket's N and L:
   beyond a chain's own count (up to the largest chain's) stay empty; both
   are dropped again by :meth:`PaddedBatch.meas`.
 
-A chain whose L is a multiple of the engine's ``n_matmul`` and whose
-``period_eqlt`` equals it runs exactly as it would alone (same random
-stream, recalculation points and measurement slices).  Supported: the
+Every chain keeps its own ``period_eqlt`` (the engine measures each padded
+chain at its own cadence), so its accumulators collect the samples a run of
+its own file collects.  A chain whose L is a multiple of the engine's
+``n_matmul`` runs exactly as it would alone (same random stream,
+recalculation points and measurement slices); at another ``n_matmul`` only
+the recalculation points move, which changes G by rounding.  Supported: the
 default equal-time measurements; the optional groups and unequal-time
 measurements need equal sizes.
 """
@@ -51,7 +54,8 @@ def _block(a, n_pad, fill_diag):
 
 
 def pad_sim(sim, N_pad, L_pad, n_matmul, num_i_pad, num_ij_pad):
-    """Padded copy of a simulation dict (params, state, meas_eqlt)."""
+    """Padded copy of a simulation dict (params, state, meas_eqlt); the
+    chain's own period_eqlt is kept."""
     p = sim["params"]
     N, L = _int(p["N"]), _int(p["L"])
     if _int(p.get("period_uneqlt", 0)) > 0:
@@ -79,8 +83,7 @@ def pad_sim(sim, N_pad, L_pad, n_matmul, num_i_pad, num_ij_pad):
     mij[:N, :N] = np.asarray(p["map_ij"]).reshape(N, N)
     q["map_i"], q["map_ij"] = mi, mij
     q.update(N=np.int32(N_pad), L=np.int32(L_pad), F=np.int32(L_pad // n_matmul),
-             n_matmul=np.int32(n_matmul), period_eqlt=np.int32(n_matmul),
-             num_i=np.int32(num_i_pad), num_ij=np.int32(num_ij_pad))
+             n_matmul=np.int32(n_matmul), num_i=np.int32(num_i_pad), num_ij=np.int32(num_ij_pad))
     for k in _OPTIONAL_COUNTS:
         q[k] = np.int32(0)
     st = dict(sim["state"])

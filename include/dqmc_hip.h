@@ -132,7 +132,9 @@ typedef struct dqmc_tuning {
      * exp_lambda = 1, del = 0) and map to measurement bins the caller drops,
      * and slices >= chain_L[c] are identity slices (no update, no wrap, no
      * measurement; the chain's last group of n_matmul slices may be partial).
-     * Its random stream advances exactly as for a chain of its own size.
+     * Its random stream advances exactly as for a chain of its own size,
+     * and it measures at its own params' period_eqlt (which, unlike the
+     * other shape fields, may differ between the chains of a padded engine).
      * Unequal-time measurements and the adjoint need equal sizes.
      * No counterpart in the reference, which runs one chain per process. */
     const int *chain_L;
@@ -161,7 +163,10 @@ int dqmc_engine_get_meas(dqmc_engine *e, int chain, dqmc_meas *eq, dqmc_meas *ue
  * n_sweep).  The loop body is that of src/dqmc.c:222-459 without the
  * checkpoint, which the host performs between calls.  *stop_flag (may be NULL)
  * is polled before each sweep as sig_check_state is (src/dqmc.c:224-230);
- * when it is non-zero the call returns DQMC_INTERRUPTED. */
+ * when it is non-zero the call returns DQMC_INTERRUPTED.  The call queues a
+ * sweep only once the sweep two before it has finished on the device, so a
+ * flag that a signal handler or a timer sets while the call runs ends it
+ * within two sweeps. */
 int dqmc_engine_run(dqmc_engine *e, int n_sweeps, const volatile int *stop_flag);
 
 /* Oracle hooks (read-only):

@@ -43,3 +43,10 @@ def test_padding_refuses_what_needs_equal_sizes():
         buckets.pad_sim(_sim(meas_bond_corr=1), 25, 30, 10, 2, 17)
     with pytest.raises(ValueError):
         buckets.pad_sim(_sim(), 9, 30, 10, 2, 17)
+
+
+def test_padding_keeps_the_chains_equal_time_cadence():
+    s = gen.create_batch(Nfiles=1, seed=5, Nx=4, Ny=4, L=20, n_matmul=10, period_eqlt=5,
+                         U=4.0)[0]
+    q = buckets.pad_sim(s, 25, 30, 10, 2, 17)
+    assert int(q["params"]["period_eqlt"]) == 5 and int(q["params"]["n_matmul"]) == 10

@@ -1,9 +1,9 @@
 """GPU: chains of uneven size padded into one engine (dqmc_amd.buckets) run as
 they run alone.
 
-A chain whose L is a multiple of the engine's n_matmul (with period_eqlt
-equal to it) has the same random stream, recalculation points and
-measurement slices as in an engine of its own: its HS field and RNG state
+A chain whose L is a multiple of the engine's n_matmul has the same random
+stream, recalculation points and measurement slices (each chain at its own
+period_eqlt) as in an engine of its own: its HS field and RNG state
 must match bit for bit, G and the equal-time accumulators to 1e-10 relative.
 A chain whose last group is partial (L = 23 in groups of 10) is compared with
 its own engine at n_matmul = 1: the same Markov chain, stabilised at other
@@ -58,6 +58,28 @@ def test_padded_chains_match_their_own_engines():
         for k in ("density", "double_occ", "g00", "nn", "xx", "zz", "pair_sw"):
             assert m2[k].shape == meq[k].shape
             assert rel(m2[k], meq[k]) < 1e-10, k
+    pb.close()
+
+
+def test_padded_chains_keep_their_own_equal_time_cadence():
+    """Chains with different period_eqlt share one padded engine; each is
+    measured at its own cadence (src/dqmc.c:361), so its sample count and
+    accumulators are those of an engine of its own."""
+    sims = [gen.create_batch(Nfiles=1, seed=70 + k, Nx=side, Ny=side, L=L, n_matmul=10,
+                             period_eqlt=pe, U=4.0, mu=-0.4, tp=0.0, n_sweep_warm=1,
+                             n_sweep_meas=100)[0]
+            for k, (side, L, pe) in enumerate([(4, 20, 5), (5, 30, 10), (4, 30, 3), (6, 10, 2)])]
+    sweeps = 3
+    pb = buckets.PaddedBatch(sims, 10)
+    pb.run(sweeps)
+    for c, sim in enumerate(sims):
+        st, meq, _, _, _ = _alone(sim, sweeps)
+        m2 = pb.meas(c)
+        L, pe = int(sim["params"]["L"]), int(sim["params"]["period_eqlt"])
+        assert int(m2["n_sample"]) == int(meq["n_sample"]) == (sweeps - 1) * (L // pe), c
+        assert np.array_equal(pb.state(c)["hs"], st["hs"])
+        for k in ("density", "double_occ", "nn", "zz"):
+            assert rel(m2[k], meq[k]) < 1e-10, (c, k)
     pb.close()
 
 

@@ -104,6 +104,27 @@ def test_time_limit_interrupts_and_resume_matches_oracle(tmp_path):
     _check_against_oracle(fn, resumed)
 
 
+def test_time_limit_without_checkpoints_stops_within_a_few_sweeps(tmp_path):
+    """checkpoint_every = 0: the whole run is one dqmc_engine_run call.  The
+    time limit must still end it within a few sweeps (the engine queues a
+    sweep only after the one two before it has finished), not after every
+    sweep the host could queue.  10^5 sweeps of an 8x8 chain take minutes."""
+    import time
+    sim = gen.create_batch(seed=43, Nx=8, Ny=8, U=4.0, mu=-0.3, L=40, n_matmul=8, period_eqlt=8,
+                           n_sweep_warm=0, n_sweep_meas=100000, checkpoint_every=0)[0]
+    fn = str(tmp_path / "nockpt.h5")
+    h5.write_sim(fn, sim)
+    t0 = time.time()
+    r = _run([os.path.join(BIN, "dqmc_1"), "-t", "3", "-l", fn + ".log", fn], timeout=90)
+    wall = time.time() - t0
+    log = open(fn + ".log").read()
+    print("wall %.1f s with a 3 s limit" % wall, flush=True)
+    assert r.returncode == 1, log  # WRAP_INCOMPLETE
+    assert "dqmc() interrupted" in log and "time limit" in log
+    assert wall < 3 + 15, wall
+    assert int(h5.read_sim(fn)["state"]["sweep"]) == 0  # nothing saved after an interruption
+
+
 def test_dqmc_stack_pads_uneven_sizes(tmp_path):
     # files of uneven N and L with one n_matmul / period_eqlt share a padded
     # engine per element type; each must end as the oracle run of its own file:
