@@ -73,9 +73,24 @@ sim = bench.workload(%(nproc)d)[%(k)d]
 while time.time() < %(start)r:  # all chains start together
     time.sleep(0.01)
 t0 = time.time()
+oracle.run(sim, max_sweeps=0)  # the initial B/C/G build alone (setup)
+t1 = time.time()
 out = oracle.run(sim, max_sweeps=%(sweeps)d)
-print(json.dumps({"sweeps": out["sweeps_run"], "seconds": time.time() - t0}))
+t2 = time.time()
+# sweeps / (wall - setup), as the reference's own sweep rate is defined
+print(json.dumps({"sweeps": out["sweeps_run"], "seconds": (t2 - t1) - (t1 - t0)}))
 """
+
+
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def cpu_baseline(sweeps=4, cpu_share=16):
@@ -110,11 +125,14 @@ def cpu_baseline(sweeps=4, cpu_share=16):
     rates = [r["sweeps"] / r["seconds"] for r in res]
     single = sum(rates) / len(rates)
     return {"value": sum(rates), "unit": "sweeps/s", "cores": 2 * nproc, "kind": "port",
-            "sample": "%d chains x %d sweeps (every 2nd with unequal-time, incl. initial G) in %d "
-                      "concurrent processes of 2 OpenMP threads (stack mode), %.1f s each; "
-                      "per chain %.3f sweeps/s" % (nproc, sweeps, nproc,
-                                                   max(r["seconds"] for r in res), single),
-            "single_chain_value": round(single, 4)}
+            "sample": "%d chains x %d sweeps (every 2nd with unequal-time; the initial G build "
+                      "timed apart and subtracted) in %d concurrent processes of 2 OpenMP threads "
+                      "(stack mode), %.1f s each; per chain %.3f sweeps/s"
+                      % (nproc, sweeps, nproc, max(r["seconds"] for r in res), single),
+            "single_chain_value": round(single, 4),
+            "cpu_model": _cpu_model(), "host_cpus_visible": ncpu,
+            "blas": "Intel MKL 2021.4 LP64 sequential (conda libmkl_intel_lp64/sequential/core), "
+                    "the BLAS/LAPACK the reference's build/Makefile links"}
 
 
 # --------------------------------------------------------------------------
@@ -200,8 +218,8 @@ def bench_config3(args):
     n_matmul=10, delay 16, complex128, eq + unequal-time with the bond
     (current-current jj, jsjs, kk, ksks) correlators."""
     from dqmc_amd import engine, gen
-    kw = dict(Nx=12, Ny=12, tp=0.0, U=4.0, mu=-0.5, dt=0.1, L=60, nflux=1, n_matmul=10,
-              n_delay=16, period_eqlt=10, period_uneqlt=2, n_sweep_warm=0,
+    kw = dict(Nx=12, Ny=12, tp=0.0, U=4.0, mu=-0.5, dt=0.1, L=60, nflux=1, alpha=1.0,
+              n_matmul=10, n_delay=16, period_eqlt=10, period_uneqlt=2, n_sweep_warm=0,
               n_sweep_meas=100000, meas_bond_corr=1)
     if args.all_bond_sets:  # also the thermal and two-hop (2-bond) correlators
         kw.update(meas_thermal=1, meas_2bond_corr=1)
@@ -214,7 +232,8 @@ def bench_config3(args):
     engb.close()
     return _line("DQMC sweeps/s, complex Peierls 12x12 N=144 L=60 (batched chains, aggregate)",
                  nbs / dtb, "sweeps/s", args.steps, args.warmup, dtb,
-                 "12x12 square Hubbard, nflux=1 (Landau gauge Peierls phases), complex128, "
+                 "12x12 square Hubbard, nflux=1, Peierls phases in the Landau gauge (alpha=1; "
+                 "at t'=0 gauge-equivalent to the reference generator's alpha=1/2), complex128, "
                  "N=144 L=60 U=4 mu=-0.5 beta=6 n_matmul=10 delay=16, eq every 10 slices, "
                  "uneq + bond correlators every 2 sweeps"
                  + (" (+ thermal and 2-bond sets)" if args.all_bond_sets else ""),
@@ -322,8 +341,12 @@ def bench_config5(args):
     full hopping matrix K."""
     import numpy as np
     from dqmc_amd import adjoint
+    from dqmc_amd import gen
     nf = 64
-    fields = np.random.default_rng(2024).integers(0, 2, size=(nf, 50, 100)).astype(np.int32)
+    # the 64 fields as create_batch(seed=2024) draws them (the validated ones)
+    fields = np.array([s["state"]["hs"] for s in gen.create_batch(
+        Nfiles=nf, seed=2024, Nx=10, Ny=10, L=50, n_matmul=10, period_eqlt=10, n_sweep_warm=0,
+        n_sweep_meas=1)])
     prob = adjoint.AdjointProblem(10, 10, 50, 0.1, fields, n_matmul=10)
     K = prob.hopping(1.0, -0.2)
     lam, _ = adjoint.lambda_of_U(4.0, 0.1)
@@ -411,8 +434,13 @@ def main():
         return dt, acc, gstats, tile
 
     nch = args.chains
-    dt, acc, (g_ms, g_n, g_flops), tile = timed(nch, args.steps, args.warmup, True)
+    # the production path: every sweep one HIP-graph launch (eager with --no-graphs)
+    dt, acc, _, tile = timed(nch, args.steps, args.warmup, False)
     value = nch * args.steps / dt
+    # the same workload launched kernel by kernel with a HIP event pair around
+    # every N x N x N GEMM on the engine stream: the dominant kernel's average
+    # duration for the roofline (the event pairs cost a little throughput)
+    dte, _, (g_ms, g_n, g_flops), _ = timed(nch, args.steps, 0, True)
     N, L, F, n_matmul = 256, 80, 8, 10
     acc_per_sweep = acc / (nch * args.steps)
     fl = flop_model(N, L, F, n_matmul, acc_per_sweep, 32, L // 8, 0.5)
@@ -443,6 +471,8 @@ def main():
                                "n_matmul=10 delay rank 32 (A in registers, B in LDS) eq every 8 slices, "
                                "uneq every 2 sweeps",
                    "chains": nch, "parallelism": "1 GPU, chains batched"},
+        "graphs": not args.no_graphs,
+        "eager_timed_gemms_sweeps_per_s": round(nch * args.steps / dte, 4),
         "single_chain_sweeps_per_s": None if single is None else round(single, 4),
         "batched_over_single": None if single is None else round(value / single, 3),
         "model_gflop_per_sweep": round(fl / 1e9, 2),
@@ -454,7 +484,10 @@ def main():
                      "achieved": round(achieved, 3), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(achieved / PEAK_FP64_TFLOPS, 4),
                      "traffic": GEMM_HBM_BYTES_PER_MATRIX_N256 * 2 * nch,
+                     "traffic_source": "FETCH_SIZE + WRITE_SIZE counter passes of this kernel "
+                                       "(profiles/k_gemm_N256_L80_256chains.md), per launch",
                      "avg_launch_ms": round(g_ms, 5), "launches": g_n,
+                     "timing": "HIP events on the engine stream around every launch, eager pass",
                      "mfma_loop_tflops": loop,
                      "frac_of_loop_4x4x4": round(achieved / loop["4x4x4"], 4) if loop["4x4x4"] else None},
     }
