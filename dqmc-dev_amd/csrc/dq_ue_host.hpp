@@ -183,7 +183,11 @@ static int ue_expand(dqmc_engine* e, T* Gred, long sU2, bool half)
     T* Gt0 = dp<T>(e->Gt0);
     T* tmp = dp<T>(e->T1);
     const UeSteps<T> st{e, half, N, L, B, NN, LNN, LN, one, zero};
-    if (half) RC(ue_half_factors<T>(e));
+    if (half) {
+        ProfSec ps(e, PS_half_wrap);
+        RC(ue_half_factors<T>(e));
+    }
+    ProfSec ps(e, PS_expand_g);
     const int nm = (L / F) * 2;
     const int n_left = (nm - 1) / 2, n_right = nm / 2, n_up = n_left, n_down = n_right;
     const int rstop_last = ((E - 1) * nm + L) / 2;
@@ -278,6 +282,7 @@ static int ue_calc(dqmc_engine* e, bool half)
     auto BLK = [&](int i, int j) { return Gred + (long)N * i + (long)NL * N * j; };
 
     // calc_o (src/greens.c:247-264)
+    ProfSec ps_o(e, PS_calc_o);
     {
         const long nd = (long)B * sU2 * (long)(sizeof(T) / 8);
         k_zero<<<nblk(nd), 256, 0, e->st>>>(nd, (double*)Gred);
@@ -288,8 +293,11 @@ static int ue_calc(dqmc_engine* e, bool half)
     RC(mul_seq_C<T>(e, (E - 1) * 2, 0, one, BLK(0, E - 1), NE, sU2));
     k_set_identity<T><<<nblk((long)NE * NE * B), 256, 0, e->st>>>(NE, Gred, sU2, NE, 1, B);
     LCHK();
+    ps_o.stop();
 
     // bsofi (src/greens.c:266-337)
+    {
+    ProfSec ps_b(e, PS_bsofi);
     if (E == 1) {
         k_set_identity<T><<<nblk(NN * B), 256, 0, e->st>>>(N, Wa, sU2, N, 0, B);
         LCHK();
@@ -387,6 +395,7 @@ static int ue_calc(dqmc_engine* e, bool half)
             LCHK();
         }
     }
+    }
 
     return ue_expand<T>(e, Gred, sU2, half);
 }
@@ -395,6 +404,7 @@ template <typename T>
 static int ue_measure(dqmc_engine* e)
 {
     RC(ue_calc<T>(e, true));
+    ProfSec ps(e, PS_meas_uneq);
     MeasArgs<T> ma = meas_args<T>(e, 1);
     ma.G0t = dp<T>(e->G0t);
     ma.Gtt = dp<T>(e->Gtt);
@@ -415,6 +425,7 @@ static int ue_measure(dqmc_engine* e)
     if ((e->thermal || e->twobond) && (e->num_bb2 > 0 || e->num_b2b > 0 || e->num_b2b2 > 0) &&
         (!e->btab || !e->b2tab))
         return set_err(DQMC_ERR_UNSUPPORTED, "two-hop bond measurements need bonds and two-hop bonds");
+    ProfSec ps_sub(e, PS_meas_uneq_sub);
     if (e->bond || e->thermal || e->pbbonly || e->twobond || e->energy) {
         // G(0,t)^T for the bond-pair kernels, into T2 (the expansion's
         // half-wrapped B_l, free until the next measurement rebuilds it)

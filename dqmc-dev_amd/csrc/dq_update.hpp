@@ -45,6 +45,7 @@ struct UpdArgs {
     T* phase;                       // [c]
     double* escale; double* iscale; // [b][L*N]
     unsigned long long* n_acc;      // [c]
+    unsigned long long* n_acc_sl;   // [c][L] accepted flips per slice (or null)
     const int* active;              // [c]
     const int* Lc;                  // [c] own slice counts of padded chains (or null)
 };
@@ -173,6 +174,7 @@ __global__ __launch_bounds__(UPD_THREADS) void k_update(UpdArgs<T> a)
     if (tid == 0) {
         a.phase[c] = phase;
         a.n_acc[c] += nacc;
+        if (a.n_acc_sl) a.n_acc_sl[(long)c * a.L + a.l] += nacc;
     }
 }
 

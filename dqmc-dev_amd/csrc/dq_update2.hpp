@@ -31,6 +31,7 @@ struct Upd2Args {
     T* phase;                       // [c]
     double* escale; double* iscale; // [b][L*N]
     unsigned long long* n_acc;      // [c]
+    unsigned long long* n_acc_sl;   // [c][L] accepted flips per slice (or null)
     const int* active;              // [c]
     const int* Lc;                  // [c] own slice counts of padded chains (or null)
 };
@@ -280,5 +281,6 @@ __global__ __launch_bounds__(UPD2_THREADS) void k_update_reg(Upd2Args<T> a)
     if (tid == 0) {
         a.phase[c] = phase;
         a.n_acc[c] += nacc;
+        if (a.n_acc_sl) a.n_acc_sl[(long)c * a.L + a.l] += nacc;
     }
 }

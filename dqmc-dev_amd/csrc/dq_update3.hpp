@@ -338,5 +338,6 @@ __global__ __launch_bounds__(NT) void k_update_lds(Upd2Args<T> a)
     if (tid == 0) {
         a.phase[c] = phase;
         a.n_acc[c] += nacc;
+        if (a.n_acc_sl) a.n_acc_sl[(long)c * a.L + a.l] += nacc;
     }
 }

@@ -16,6 +16,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <chrono>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -140,6 +141,16 @@ static bool ue_supported(int k, int cplx)
 // --------------------------------------------------------------------------
 // engine state
 // --------------------------------------------------------------------------
+enum {
+    PS_wall, PS_updates, PS_calcb, PS_multb, PS_recalc, PS_wrap, PS_half_wrap, PS_meas_eq,
+    PS_calc_o, PS_bsofi, PS_expand_g, PS_meas_uneq, PS_meas_uneq_sub, PS_checkpoint, PS_COUNT
+};
+
+struct ProfPending {
+    int s;
+    hipEvent_t a, b;
+};
+
 struct dqmc_engine {
     int nc = 0, N = 0, L = 0, F = 0, n_matmul = 0, cplx = 0, kd = 16, device = 0;
     int period_eqlt = 0, period_uneqlt = 0, energy = 0;
@@ -162,6 +173,7 @@ struct dqmc_engine {
     double *dvec = 0, *vvec = 0;                            // [B][N]
     void *phb = 0, *phb2 = 0, *phase = 0;                   // [B], [B], [nc] T
     unsigned long long* nacc = 0;                           // [nc]
+    unsigned long long* nacc_sl = 0;                        // [nc][L]
     uint64_t* raw = 0;                                      // [nc][2 L N] draws of a sweep
     int *order = 0, *draws = 0;                             // [nc][L][N], [nc]
     double* unif = 0;                                       // [nc][L][N]
@@ -230,10 +242,76 @@ struct dqmc_engine {
     unsigned long long* d_drift = 0;  // [nc] bits of a non-negative double
     double gemm_ms = 0.0, gemm_flops = 0.0;
     long long gemm_launches = 0;
+    // per-section profile (dqmc_tuning.flags bit 3)
+    bool prof = false;
+    std::vector<hipEvent_t> ev_pool;
+    std::vector<ProfPending> prof_pending;
+    double prof_ms[PS_COUNT] = {};
+    long long prof_calls[PS_COUNT] = {};
 };
 
 template <typename T>
 static T* dp(void* p) { return (T*)p; }
+
+// Profile sections: the reference's table (src/prof.h:8-22) in its order.
+static const char* const g_prof_names[PS_COUNT] = {
+    "wall", "updates", "calcb", "multb", "recalc", "wrap", "half_wrap", "meas_eq",
+    "calc_o", "bsofi", "expand_g", "meas_uneq", "meas_uneq_sub", "checkpoint"};
+
+// A section of device work on the engine stream, timed by a HIP event pair
+// when the engine profiles (eager launches only); a no-op otherwise.
+static hipEvent_t prof_event(dqmc_engine* e)
+{
+    hipEvent_t ev = nullptr;
+    if (!e->ev_pool.empty()) {
+        ev = e->ev_pool.back();
+        e->ev_pool.pop_back();
+    } else if (hipEventCreate(&ev) != hipSuccess) {
+        ev = nullptr;
+    }
+    return ev;
+}
+
+struct ProfSec {
+    dqmc_engine* e;
+    int s;
+    hipEvent_t a = nullptr;
+    ProfSec(dqmc_engine* e_, int s_) : e(e_), s(s_)
+    {
+        if (e->prof && (a = prof_event(e))) (void)hipEventRecord(a, e->st);
+    }
+    // ends the section (at the latest when it goes out of scope)
+    void stop()
+    {
+        if (!a) return;
+        hipEvent_t b = prof_event(e);
+        if (b) {
+            (void)hipEventRecord(b, e->st);
+            e->prof_pending.push_back({s, a, b});
+        } else {
+            e->ev_pool.push_back(a);
+        }
+        a = nullptr;
+    }
+    ~ProfSec() { stop(); }
+};
+
+// adds the pending sections' durations (ms) to the totals
+static void prof_collect(dqmc_engine* e)
+{
+    if (e->prof_pending.empty()) return;
+    (void)hipStreamSynchronize(e->st);
+    for (auto& p : e->prof_pending) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+            e->prof_ms[p.s] += ms;
+            e->prof_calls[p.s]++;
+        }
+        e->ev_pool.push_back(p.a);
+        e->ev_pool.push_back(p.b);
+    }
+    e->prof_pending.clear();
+}
 
 static int dalloc(dqmc_engine* e, void** p, size_t bytes)
 {
@@ -333,11 +411,15 @@ static int build_C(dqmc_engine* e, int f0, int nf)
         const int l0 = f * n;
         T* bufs[2] = {C + f * NN, W + f * NN};
         int cur = ((n - 1) % 2 == 0) ? 0 : 1;  // the last product lands in C_f
-        k_colscale<T><<<nblk(NN * B), 256, 0, e->st>>>(N, dp<T>(e->expK), NN, 1,
-                                                      e->escale + (long)N * l0, LN, bufs[cur],
-                                                      (long)F * NN, B, e->ragged ? e->d_Lc : nullptr,
-                                                      l0);
-        LCHK();
+        {
+            ProfSec ps(e, PS_calcb);
+            k_colscale<T><<<nblk(NN * B), 256, 0, e->st>>>(N, dp<T>(e->expK), NN, 1,
+                                                          e->escale + (long)N * l0, LN, bufs[cur],
+                                                          (long)F * NN, B,
+                                                          e->ragged ? e->d_Lc : nullptr, l0);
+            LCHK();
+        }
+        ProfSec ps(e, PS_multb);
         for (int s = 1; s < n; s++) {
             RC(gemm<T>(e, OP_N, OP_N, N, N, N, one, dp<T>(e->expK), N, NN, bufs[cur], N,
                        (long)F * NN, zero, bufs[1 - cur], N, (long)F * NN, B, nullptr, 0,
@@ -863,7 +945,7 @@ static int launch_update(dqmc_engine* e, int l)
         a.del = e->del; a.exp_lambda = e->exp_lambda; a.hs = e->hs;
         a.order = e->order; a.unif = e->unif;
         a.phase = dp<T>(e->phase); a.escale = e->escale; a.iscale = e->iscale;
-        a.n_acc = e->nacc; a.active = e->active;
+        a.n_acc = e->nacc; a.n_acc_sl = e->nacc_sl; a.active = e->active;
         a.Lc = e->ragged ? e->d_Lc : nullptr;
         constexpr bool Z = is_cplx<T>::value;
         const int kd = e->kd;
@@ -893,7 +975,7 @@ static int launch_update(dqmc_engine* e, int l)
     ua.del = e->del; ua.exp_lambda = e->exp_lambda; ua.hs = e->hs;
     ua.order = e->order; ua.unif = e->unif;
     ua.phase = dp<T>(e->phase); ua.escale = e->escale; ua.iscale = e->iscale;
-    ua.n_acc = e->nacc; ua.active = e->active;
+    ua.n_acc = e->nacc; ua.n_acc_sl = e->nacc_sl; ua.active = e->active;
     ua.Lc = e->ragged ? e->d_Lc : nullptr;
     k_update<T><<<nc, UPD_THREADS, 0, e->st>>>(ua);
     LCHK();
@@ -933,15 +1015,21 @@ static int sweep_body(dqmc_engine* e, bool any_eq, bool any_ue)
                                                         e->d_nsweep, e->d_warm, e->active,
                                                         e->flag_eq, e->flag_ue);
     LCHK();
-    // the sweep's random stream: site orders and uniforms (dq_rng.hpp)
-    k_rng_raw<<<nc, 64, 0, e->st>>>(N, L, e->draws, e->rng, e->raw, 2L * L * N, e->active,
-                                    e->d_Lc, e->d_Nc);
-    LCHK();
-    k_rng_orders<<<nc, 256, 0, e->st>>>(N, L, e->raw, 2L * L * N, e->del, e->draws, e->order,
-                                        e->unif, e->active, e->d_Lc, e->d_Nc);
-    LCHK();
+    {
+        // the sweep's random stream: site orders and uniforms (dq_rng.hpp)
+        ProfSec ps(e, PS_updates);
+        k_rng_raw<<<nc, 64, 0, e->st>>>(N, L, e->draws, e->rng, e->raw, 2L * L * N, e->active,
+                                        e->d_Lc, e->d_Nc);
+        LCHK();
+        k_rng_orders<<<nc, 256, 0, e->st>>>(N, L, e->raw, 2L * L * N, e->del, e->draws, e->order,
+                                            e->unif, e->active, e->d_Lc, e->d_Nc);
+        LCHK();
+    }
     for (int l = 0; l < L; l++) {
-        RC(launch_update<T>(e, l));
+        {
+            ProfSec ps(e, PS_updates);
+            RC(launch_update<T>(e, l));
+        }
         const int f = l / n;
         const bool recalc = ((l + 1) % n == 0);
         if (recalc) {
@@ -953,6 +1041,7 @@ static int sweep_body(dqmc_engine* e, bool any_eq, bool any_ue)
                            0, l, 1));
             }
             RC(build_C<T>(e, f, 1));
+            ProfSec ps(e, PS_recalc);
             RC(calc_eq_g<T>(e, (f + 1) % F, G, dp<T>(e->phb)));
             if (e->check_drift) {
                 k_drift<T><<<dim3(nblk(NN, 256 * 16), B), 256, 0, e->st>>>(N, dp<T>(e->Gw), G, NN,
@@ -964,17 +1053,22 @@ static int sweep_body(dqmc_engine* e, bool any_eq, bool any_ue)
             LCHK();
         } else {
             // G <- B_l G B_l^-1 = expK diag(e_l) [G diag(e'_l) invexpK]  (src/dqmc.c:294-295)
+            ProfSec ps(e, PS_wrap);
             RC(gemm<T>(e, OP_N, OP_N, N, N, N, one, G, N, NN, dp<T>(e->iexpK), N, NN, zero, T1,
                        N, NN, B, nullptr, 0, e->iscale + (long)N * l, LN, nullptr, 0, l, 2));
             RC(gemm<T>(e, OP_N, OP_N, N, N, N, one, dp<T>(e->expK), N, NN, T1, N, NN, zero, G,
                        N, NN, B, nullptr, 0, e->escale + (long)N * l, LN, nullptr, 0, l, 1));
         }
         if (any_eq && e->meas_at[l]) {
-            // half-wrap e^{dtK/2} G e^{-dtK/2} (src/dqmc.c:364-380), into Y2
-            RC(gemm<T>(e, OP_N, OP_N, N, N, N, one, G, N, NN, dp<T>(e->ehalf), N, NN, zero, T1,
-                       N, NN, B));
-            RC(gemm<T>(e, OP_N, OP_N, N, N, N, one, dp<T>(e->iehalf), N, NN, T1, N, NN, zero,
-                       dp<T>(e->Y2), N, NN, B));
+            {
+                // half-wrap e^{dtK/2} G e^{-dtK/2} (src/dqmc.c:364-380), into Y2
+                ProfSec ps(e, PS_half_wrap);
+                RC(gemm<T>(e, OP_N, OP_N, N, N, N, one, G, N, NN, dp<T>(e->ehalf), N, NN, zero,
+                           T1, N, NN, B));
+                RC(gemm<T>(e, OP_N, OP_N, N, N, N, one, dp<T>(e->iehalf), N, NN, T1, N, NN, zero,
+                           dp<T>(e->Y2), N, NN, B));
+            }
+            ProfSec ps(e, PS_meas_eq);
             MeasArgs<T> ma = meas_args<T>(e, 0);
             ma.G = dp<T>(e->Y2);
             ma.sG = NN;
@@ -1002,6 +1096,22 @@ static int sweep_body(dqmc_engine* e, bool any_eq, bool any_ue)
 }
 
 template <typename T>
+static int run_t(dqmc_engine* e, int n_sweeps, const volatile int* stop);
+
+template <typename T>
+static int run_prof(dqmc_engine* e, int n_sweeps, const volatile int* stop)
+{
+    if (!e->prof) return run_t<T>(e, n_sweeps, stop);
+    const auto t0 = std::chrono::steady_clock::now();
+    const int rc = run_t<T>(e, n_sweeps, stop);
+    prof_collect(e);
+    e->prof_ms[PS_wall] +=
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    e->prof_calls[PS_wall]++;
+    return rc;
+}
+
+template <typename T>
 static int run_t(dqmc_engine* e, int n_sweeps, const volatile int* stop)
 {
     const int nc = e->nc;
@@ -1022,7 +1132,7 @@ static int run_t(dqmc_engine* e, int n_sweeps, const volatile int* stop)
         }
         if (!any_active) break;
         const int key = (any_eq ? 1 : 0) | (any_ue ? 2 : 0);
-        if (e->use_graph && !e->time_gemm) {
+        if (e->use_graph && !e->time_gemm && !e->prof) {
             if (!e->gexec[key]) {
                 HIPCHK(hipStreamBeginCapture(e->st, hipStreamCaptureModeRelaxed));
                 const int rc = sweep_body<T>(e, any_eq, any_ue);
@@ -1170,6 +1280,7 @@ static int engine_setup(dqmc_engine* e, const dqmc_params* const* P)
     DALLOC(e->phb2, B * es);
     DALLOC(e->phase, nc * es);
     DALLOC(e->nacc, nc * 8);
+    DALLOC(e->nacc_sl, (long)nc * L * 8);
     if (e->check_drift) {
         DALLOC(e->Gw, B * NN * es);
         DALLOC(e->d_drift, nc * 8);
@@ -1503,6 +1614,7 @@ extern "C" int dqmc_engine_create(dqmc_engine** out, const dqmc_params* const* P
     e->time_gemm = tune && (tune->flags & 1);
     e->use_graph = !(tune && (tune->flags & 2));
     e->check_drift = tune && (tune->flags & 4);
+    e->prof = tune && (tune->flags & 8);
     if (tune && (tune->chain_L || tune->chain_N)) {
         // chains of uneven size, padded by the caller to N and L
         e->ragged = true;
@@ -1575,6 +1687,8 @@ extern "C" void dqmc_engine_destroy(dqmc_engine* e)
     }
     for (auto& ev : e->done)
         if (ev) (void)hipEventDestroy(ev);
+    prof_collect(e);
+    for (auto ev : e->ev_pool) (void)hipEventDestroy(ev);
     for (void* p : e->allocs) (void)hipFree(p);
     if (e->st) (void)hipStreamDestroy(e->st);
     delete e;
@@ -1659,7 +1773,7 @@ extern "C" int dqmc_engine_run(dqmc_engine* e, int n_sweeps, const volatile int*
 {
     if (!e || n_sweeps < 0) return set_err(DQMC_ERR_ARG, "bad arguments");
     HIPCHK(hipSetDevice(e->device));
-    return e->cplx ? run_t<zd>(e, n_sweeps, stop_flag) : run_t<double>(e, n_sweeps, stop_flag);
+    return e->cplx ? run_prof<zd>(e, n_sweeps, stop_flag) : run_prof<double>(e, n_sweeps, stop_flag);
 }
 
 extern "C" int dqmc_engine_green(dqmc_engine* e, int c, int spin, double* G, double* phase)
@@ -1839,6 +1953,17 @@ extern "C" int64_t dqmc_engine_accepts(dqmc_engine* e)
     return s;
 }
 
+extern "C" int dqmc_engine_slice_accepts(dqmc_engine* e, int c, int64_t* counts)
+{
+    if (!e || c < 0 || c >= e->nc || !counts) return set_err(DQMC_ERR_ARG, "bad arguments");
+    HIPCHK(hipSetDevice(e->device));
+    HIPCHK(hipStreamSynchronize(e->st));
+    std::vector<unsigned long long> v(e->L);
+    HIPCHK(hipMemcpy(v.data(), e->nacc_sl + (long)c * e->L, e->L * 8, hipMemcpyDeviceToHost));
+    for (int l = 0; l < e->L; l++) counts[l] = (int64_t)v[l];
+    return 0;
+}
+
 extern "C" int dqmc_engine_drift(dqmc_engine* e, double* max_abs)
 {
     if (!e || !max_abs) return set_err(DQMC_ERR_ARG, "bad arguments");
@@ -1853,10 +1978,18 @@ extern "C" int dqmc_engine_drift(dqmc_engine* e, double* max_abs)
     return 0;
 }
 
-extern "C" int dqmc_engine_profile(dqmc_engine* e, const char** names, double* ms, int max)
+extern "C" int dqmc_engine_profile(dqmc_engine* e, const char** names, double* ms,
+                                   int64_t* calls, int max)
 {
-    (void)e; (void)names; (void)ms; (void)max;
-    return 0;
+    if (!e) return set_err(DQMC_ERR_ARG, "null engine");
+    if (!e->prof) return set_err(DQMC_ERR_ARG, "engine created without profiling (flags bit 3)");
+    prof_collect(e);
+    for (int k = 0; k < PS_COUNT && k < max; k++) {
+        if (names) names[k] = g_prof_names[k];
+        if (ms) ms[k] = e->prof_ms[k];
+        if (calls) calls[k] = e->prof_calls[k];
+    }
+    return PS_COUNT;
 }
 
 extern "C" int dqmc_engine_gemm_stats(dqmc_engine* e, double* avg_ms, int64_t* launches,

@@ -71,6 +71,11 @@ def lib():
         L.dqmc_engine_drift.restype = C.c_int
         L.dqmc_engine_set_params.argtypes = [C.c_void_p, C.c_int, P(capi.DqmcParams)]
         L.dqmc_engine_set_params.restype = C.c_int
+        L.dqmc_engine_profile.argtypes = [C.c_void_p, P(C.c_char_p), P(C.c_double), P(C.c_int64),
+                                          C.c_int]
+        L.dqmc_engine_profile.restype = C.c_int
+        L.dqmc_engine_slice_accepts.argtypes = [C.c_void_p, C.c_int, P(C.c_int64)]
+        L.dqmc_engine_slice_accepts.restype = C.c_int
         for f in ("set_state", "get_state", "set_meas", "get_meas", "run", "green", "calc_eq_g",
                   "calc_ue_g", "gemm_stats"):
             getattr(L, "dqmc_engine_" + f).restype = C.c_int
@@ -109,7 +114,7 @@ class Engine:
     with :meth:`drift`."""
 
     def __init__(self, sims, device=0, delay_rank=0, time_gemm=False, gemm_tile=0, chain_L=None,
-                 chain_N=None, graphs=True, check_drift=False):
+                 chain_N=None, graphs=True, check_drift=False, profile=False):
         self.L = lib()
         self.sims = sims
         self.cplx = capi.is_complex(sims[0])
@@ -126,7 +131,8 @@ class Engine:
         # own sizes of chains padded to the common N, L (dqmc_amd.buckets)
         self._cl = None if chain_L is None else (C.c_int * len(sims))(*chain_L)
         self._cn = None if chain_N is None else (C.c_int * len(sims))(*chain_N)
-        flags = (1 if time_gemm else 0) | (0 if graphs else 2) | (4 if check_drift else 0)
+        flags = ((1 if time_gemm else 0) | (0 if graphs else 2) | (4 if check_drift else 0) |
+                 (8 if profile else 0))
         tune = DqmcTuning(delay_rank, flags, gemm_tile,
                           C.cast(self._cl, C.POINTER(C.c_int)) if self._cl is not None else None,
                           C.cast(self._cn, C.POINTER(C.c_int)) if self._cn is not None else None)
@@ -241,6 +247,23 @@ class Engine:
 
     def accepts(self):
         return int(self.L.dqmc_engine_accepts(self.h))
+
+    def slice_accepts(self, c):
+        """Accepted flips of chain c per slice, summed over all sweeps run."""
+        out = np.zeros(self.Lt, dtype=np.int64)
+        _check(self.L.dqmc_engine_slice_accepts(self.h, c, out.ctypes.data_as(C.POINTER(C.c_int64))),
+               "dqmc_engine_slice_accepts")
+        return out
+
+    def profile(self):
+        """{section: (ms, calls)} in the reference's profile order (needs profile=True)."""
+        names = (C.c_char_p * 16)()
+        ms = np.zeros(16)
+        calls = np.zeros(16, dtype=np.int64)
+        n = _check(self.L.dqmc_engine_profile(self.h, names, ms.ctypes.data_as(C.POINTER(C.c_double)),
+                                              calls.ctypes.data_as(C.POINTER(C.c_int64)), 16),
+                   "dqmc_engine_profile")
+        return {names[k].decode(): (float(ms[k]), int(calls[k])) for k in range(n)}
 
     def gemm_tile(self):
         """GEMM C-tile shape in use (1 = 64x64, 2 = 128x64 with K steps of 16,

@@ -59,6 +59,8 @@ int64_t dq_time_ns(void)
     return (int64_t)t.tv_sec * 1000000000LL + t.tv_nsec;
 }
 
+#define N_PROF 14
+
 typedef struct job {
     const char *file;
     FILE *log;
@@ -66,7 +68,19 @@ typedef struct job {
     int loaded, failed, interrupted;
     int last_ckpt;
     dq_sim sim;
+    /* the engine's profile table (src/prof.h:8-22) and this file's checkpoints */
+    int prof_chains;
+    const char *prof_name[N_PROF];
+    double prof_ms[N_PROF];
+    int64_t prof_calls[N_PROF];
+    int64_t ckpt_ns, ckpt_calls;
 } job;
+
+static bool g_profile = false;
+
+void dq_set_profile(bool on) { g_profile = on; }
+
+
 
 typedef struct group {
     job **j;
@@ -87,6 +101,33 @@ static void logf_(job *j, const char *fmt, ...)
     fflush(j->log);
     pthread_mutex_unlock(&g_log_mu);
     va_end(ap);
+}
+
+/* the table of src/prof.c:12-50 (profile_print): sections sorted by time,
+ * share of the wall time, total, time per call and calls; the device
+ * sections are the engine's, shared by the chains that ran in it */
+static void print_profile(job *j, double wall_s)
+{
+    if (!j->prof_chains) return;
+    int idx[N_PROF];
+    for (int i = 0; i < N_PROF; i++) {
+        int k = i;
+        while (k > 0 && j->prof_ms[idx[k - 1]] < j->prof_ms[i]) {
+            idx[k] = idx[k - 1];
+            k--;
+        }
+        idx[k] = i;
+    }
+    logf_(j, "gpu_engine_(%d_chain%s)|_%% of all_|___total (s)_|___us per call_|___# calls\n",
+          j->prof_chains, j->prof_chains == 1 ? "" : "s");
+    for (int q = 0; q < N_PROF; q++) {
+        const int i = idx[q];
+        if (j->prof_calls[i] == 0) continue;
+        logf_(j, "%16s |%9.3f |%12.3f |%14.3f |%10lld\n", j->prof_name[i],
+              100.0 * j->prof_ms[i] * 1e-3 / wall_s, j->prof_ms[i] * 1e-3,
+              1e3 * j->prof_ms[i] / (double)j->prof_calls[i], (long long)j->prof_calls[i]);
+    }
+    logf_(j, "---------------------------------------------------------------------\n");
 }
 
 static void now_str(char *buf, size_t n)
@@ -157,7 +198,7 @@ static void *run_group(void *arg)
     dqmc_engine *e = NULL;
     /* HIP stream capture fails while other host threads make HIP calls, so
      * graphs are used only when this group is the only one */
-    dqmc_tuning tune = {0, g->alone ? 0 : 2, 0, NULL, NULL};
+    dqmc_tuning tune = {0, (g->alone ? 0 : 2) | (g_profile ? 8 : 0), 0, NULL, NULL};
     int rc = 0;
     for (int c = 0; c < n && !rc; c++) {
         job *j = g->j[c];
@@ -228,7 +269,12 @@ static void *run_group(void *arg)
             any = 1;
             if (ce > 0 && sw % ce == 0 && j->last_ckpt != sw) {
                 j->last_ckpt = sw;
-                if (fetch(e, c, j, uneven ? &pads[c] : NULL) != 0 || dq_sim_save(&j->sim) < 0) {
+                const int64_t t0 = dq_time_ns();
+                const int bad = fetch(e, c, j, uneven ? &pads[c] : NULL) != 0 ||
+                                dq_sim_save(&j->sim) < 0;
+                j->ckpt_ns += dq_time_ns() - t0;
+                j->ckpt_calls++;
+                if (bad) {
                     logf_(j, "dqmc() checkpoint failed\n");
                     j->failed = 1;
                     j->code = WRAP_FAIL;
@@ -279,6 +325,26 @@ static void *run_group(void *arg)
         job *j = g->j[c];
         if (j->failed || j->interrupted || j->code == WRAP_FAIL) continue;
         logf_(j, "[%s] %d/%d sweeps completed\n", ts, j->sim.sweep, j->sim.p.n_sweep);
+    }
+    if (g_profile) {
+        const char *names[N_PROF];
+        double ms[N_PROF];
+        int64_t calls[N_PROF];
+        if (dqmc_engine_profile(e, names, ms, calls, N_PROF) == N_PROF) {
+            for (int c = 0; c < n; c++) {
+                job *j = g->j[c];
+                j->prof_chains = n;
+                for (int i = 0; i < N_PROF; i++) {
+                    j->prof_name[i] = names[i];
+                    j->prof_ms[i] = ms[i];
+                    j->prof_calls[i] = calls[i];
+                    if (!strcmp(names[i], "checkpoint")) {  /* the host's own */
+                        j->prof_ms[i] = (double)j->ckpt_ns * 1e-6;
+                        j->prof_calls[i] = j->ckpt_calls;
+                    }
+                }
+            }
+        }
     }
     dqmc_engine_destroy(e);
 done:
@@ -421,6 +487,7 @@ void dqmc_wrapper_batch(int n, const char *const *files, const char *const *logs
         if (j->loaded) dq_sim_free(&j->sim);
         if (!j->log) continue;
         logf_(j, "wall time: %.3f\n", wall);
+        print_profile(j, wall);
         if (j->log != stdout) fclose(j->log);
         else fflush(stdout);
     }

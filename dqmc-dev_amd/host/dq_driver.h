@@ -30,6 +30,13 @@ int dqmc_wrapper(const char *sim_file, const char *log_file, int64_t max_time_ns
 void dqmc_wrapper_batch(int n, const char *const *files, const char *const *logs,
                         int64_t max_time_ns, bool dry, bool bench, int *codes);
 
+/* Print the reference's profile table (src/prof.h:8-22, src/prof.c:12-50:
+ * time per section, the engine's, and the host's checkpoints) into every log
+ * at the end of the run; the engines then time each section with HIP events
+ * and launch kernel by kernel.  The reference selects this at compile time
+ * (PROFILE_ENABLE). */
+void dq_set_profile(bool on);
+
 /* install the stop-signal handlers (SIGUSR1, SIGINT, SIGTERM, SIGHUP), as
  * sig_init does (src/sig.c:43-59) */
 void dq_sig_init(void);

@@ -2,7 +2,7 @@
  * dqmc_1 -- run one simulation file on the MI355X engine.
  *
  * Command line and exit status of src/main_1.c:20-113:
- *   dqmc_1 [-t TIME] [-n|--dry-run] [-l|--log LOGFILE] [-b|--benchmark] sim_file.h5
+ *   dqmc_1 [-t TIME] [-n|--dry-run] [-l|--log LOGFILE] [-b|--benchmark] [-p|--profile] sim_file.h5
  * The exit status is dqmc_wrapper's code (0 done, 1 incomplete, -1 -> 255 failed).
  * TIME is taken as seconds (the reference passes the seconds value where its
  * wrapper expects clock ticks, src/main_1.c:103-104 vs src/sig.c:75).
@@ -22,6 +22,7 @@ static struct argp_option options[] = {
     {"dry-run", 'n', 0, 0, "Report the memory requirement only", 0},
     {"log", 'l', "LOGFILE", 0, "Redirect stdout to LOGFILE", 0},
     {"benchmark", 'b', 0, 0, "Don't save sim_data to disk", 0},
+    {"profile", 'p', 0, 0, "Print the per-section profile table into the log", 0},
     {0}};
 
 struct arguments {
@@ -39,6 +40,7 @@ static error_t parse_opt(int key, char *arg, struct argp_state *state)
     case 'l': a->log_file = arg; break;
     case 'n': a->dry_run = true; break;
     case 'b': a->bench = true; break;
+    case 'p': dq_set_profile(true); break;
     case ARGP_KEY_ARG:
         if (state->arg_num >= 1) argp_usage(state);
         a->file = arg;

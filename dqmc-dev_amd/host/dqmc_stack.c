@@ -2,7 +2,7 @@
  * dqmc_stack -- work through a stack file of simulation paths on the MI355X
  * engine (src/main_stack.c:309-479).
  *
- *   dqmc_stack [-t TIME] [-n|--dry-run] [-b|--benchmark] [-B|--batch N] stack_file
+ *   dqmc_stack [-t TIME] [-n|--dry-run] [-b|--benchmark] [-B|--batch N] [-p|--profile] stack_file
  *
  * The stack file holds one .h5 path per line and is used LIFO: paths are
  * popped from the end and a run that did not finish is pushed back.  Access
@@ -179,6 +179,7 @@ static struct argp_option options[] = {
     {0, 't', "TIME", 0, "Max allowed wall time in integer seconds", 0},
     {"dry-run", 'n', 0, 0, "Run through stack, report mem requirement only", 0},
     {"benchmark", 'b', 0, 0, "Don't save sim_data to disk", 0},
+    {"profile", 'p', 0, 0, "Print the per-section profile table into the log", 0},
     {"batch", 'B', "N", 0, "Run up to N files together on the GPU (default 64)", 0},
     {0}};
 
@@ -196,6 +197,7 @@ static error_t parse_opt(int key, char *arg, struct argp_state *state)
     case 'B': a->batch = atol(arg); break;
     case 'n': a->dry_run = true; break;
     case 'b': a->bench = true; break;
+    case 'p': dq_set_profile(true); break;
     case ARGP_KEY_ARG:
         if (state->arg_num >= 1) argp_usage(state);
         a->stack = arg;

@@ -121,7 +121,9 @@ typedef struct dqmc_tuning {
                        the sweeps into HIP graphs (needed when other host
                        threads make HIP calls while this engine runs: HIP's
                        stream capture fails then); bit 2: the wrap drift check
-                       (dqmc_engine_drift) */
+                       (dqmc_engine_drift); bit 3: the per-section profile
+                       (dqmc_engine_profile; the sweeps then run launch by
+                       launch) */
     int gemm_tile;  /* C tile of the MFMA GEMM: 0 = time the shapes at creation
                        and keep the fastest, 1 = 64x64, 2 = 128x64 with K
                        steps of 16, 3 = 128x64 with K steps of 8 */
@@ -201,9 +203,23 @@ int dqmc_engine_adjoint(dqmc_engine *e, double *logdet, double *W, double *trG, 
 /* Accepted flips so far, summed over chains (for the flop model). */
 int64_t dqmc_engine_accepts(dqmc_engine *e);
 
-/* Per-kernel-class timing of the last run (ms); names are the reference's
- * profile sections (src/prof.h:8-22).  Returns the number of entries. */
-int dqmc_engine_profile(dqmc_engine *e, const char **names, double *ms, int max);
+/* Oracle hook: accepted flips of chain c per slice, counts[l] for l < L,
+ * summed over every sweep since the engine was created (the count of
+ * acceptances of update_delayed at slice l, src/updates.c:26-50).  In one
+ * sweep it equals the number of sites whose field changed at slice l. */
+int dqmc_engine_slice_accepts(dqmc_engine *e, int chain, int64_t *counts);
+
+/* Time per section of the reference's profile table (src/prof.h:8-22, printed
+ * by src/dqmc.c:661-663), accumulated over every run of an engine created with
+ * dqmc_tuning.flags bit 3: names[k], ms[k] and calls[k] for the k-th section
+ * in the reference's order (wall, updates, calcb, multb, recalc, wrap,
+ * half_wrap, meas_eq, calc_o, bsofi, expand_g, meas_uneq, meas_uneq_sub,
+ * checkpoint).  Device sections are timed by HIP event pairs on the engine
+ * stream (the sweeps then run launch by launch); wall is the host time of the
+ * dqmc_engine_run calls; meas_uneq_sub lies inside meas_uneq, as in the
+ * reference; checkpoint is the host's (0 here).  Any pointer may be NULL.
+ * Returns the number of sections (14), or a negative code. */
+int dqmc_engine_profile(dqmc_engine *e, const char **names, double *ms, int64_t *calls, int max);
 
 /* Average duration (ms) of the dominant GEMM kernel in the last run, measured
  * with HIP events on the engine stream, and its launch count. */

@@ -157,3 +157,21 @@ def test_dqmc_stack_pads_uneven_sizes(tmp_path):
         print("file", fn, "N", int(s["params"]["N"]), "L", int(s["params"]["L"]), flush=True)
         _check_against_oracle(fn, s, eq_keys=("density", "double_occ", "g00", "nn", "xx", "zz",
                                               "pair_sw"), eq_tol=1e-8)
+
+
+def test_dqmc_1_profile_table(tmp_path):
+    """-p prints the reference's profile table (src/prof.c:12-50) into the log."""
+    sim = gen.create_batch(seed=44, Nx=4, Ny=4, U=4.0, mu=-0.3, L=20, n_matmul=5, period_eqlt=5,
+                           period_uneqlt=2, n_sweep_warm=2, n_sweep_meas=4, checkpoint_every=3)[0]
+    fn = str(tmp_path / "prof.h5")
+    h5.write_sim(fn, sim)
+    r = _run([os.path.join(BIN, "dqmc_1"), "-p", "-l", fn + ".log", fn])
+    log = open(fn + ".log").read()
+    assert r.returncode == 0, log
+    assert "_% of all_|___total (s)_|___us per call_|___# calls" in log
+    rows = {ln.split("|")[0].strip(): ln for ln in log.splitlines() if ln.count("|") == 4}
+    for k in ("wall", "updates", "recalc", "wrap", "half_wrap", "meas_eq", "calc_o", "bsofi",
+              "expand_g", "meas_uneq", "checkpoint"):
+        assert k in rows, (k, log[-2000:])
+    assert int(rows["checkpoint"].split("|")[4]) == 2  # saves before sweeps 0 and 3
+    _check_against_oracle(fn, sim, ue_keys=("gt0", "nn", "zz"))

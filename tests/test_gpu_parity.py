@@ -410,3 +410,67 @@ def test_set_params_refreshes_two_hop_phases():
     finally:
         e1.close()
         e2.close()
+
+
+def test_profile_sections_and_unchanged_chain():
+    """The per-section profile (src/prof.h:8-22): every device section of a
+    sweep with equal- and unequal-time measurements is timed, the sections
+    stay within the wall time, and the chain is the one an engine without
+    profiling walks (bit for bit)."""
+    sims = gen.create_batch(Nfiles=2, seed=12, Nx=6, Ny=6, U=4.0, mu=-0.3, L=40, n_matmul=8,
+                            period_eqlt=8, period_uneqlt=1, n_sweep_warm=1, n_sweep_meas=3,
+                            meas_bond_corr=1)
+    ep = engine.Engine(sims, profile=True)
+    eq = engine.Engine(sims)
+    try:
+        ep.run(4)
+        eq.run(4)
+        for c in range(2):
+            assert np.array_equal(ep.get_state(c)["hs"], eq.get_state(c)["hs"])
+            assert np.array_equal(ep.green(c, 0)[0], eq.green(c, 0)[0])
+        prof = ep.profile()
+        names = ["wall", "updates", "calcb", "multb", "recalc", "wrap", "half_wrap", "meas_eq",
+                 "calc_o", "bsofi", "expand_g", "meas_uneq", "meas_uneq_sub", "checkpoint"]
+        assert list(prof) == names
+        print({k: (round(v[0], 2), v[1]) for k, v in prof.items()}, flush=True)
+        for k in names[1:-1]:
+            assert prof[k][1] > 0 and prof[k][0] > 0, k
+        assert prof["updates"][1] == 4 * (40 + 1)  # one per slice plus the sweep's draws
+        assert prof["recalc"][1] == 4 * 5 and prof["wrap"][1] == 4 * 35
+        assert prof["meas_uneq"][1] == 3 and prof["meas_uneq_sub"][1] == 3
+        device = sum(prof[k][0] for k in names[1:-1] if k != "meas_uneq_sub")
+        assert device <= prof["wall"][0] * 1.02
+        assert prof["checkpoint"][1] == 0
+    finally:
+        ep.close()
+        eq.close()
+    with pytest.raises(engine.DqmcError):
+        e2 = engine.Engine(sims)
+        try:
+            e2.profile()
+        finally:
+            e2.close()
+
+
+@pytest.mark.parametrize("nflux", [0, 1])
+def test_slice_acceptance_counts(nflux):
+    """dqmc_engine_slice_accepts: in one sweep every site of a slice is visited
+    once, so the accepted flips at slice l are the sites whose field changed
+    there -- on the GPU and in the oracle's run of the same chain."""
+    sim = gen.create_batch(seed=14, Nx=6, Ny=6, nflux=nflux, U=5.0, mu=-0.2, L=20, n_matmul=5,
+                           period_eqlt=5, n_sweep_warm=1, n_sweep_meas=2)[0]
+    eng = engine.Engine([sim])
+    try:
+        h0 = sim["state"]["hs"].copy()
+        eng.run(1)
+        h1 = eng.get_state(0)["hs"]
+        cnt = eng.slice_accepts(0)
+        assert np.array_equal(cnt, (h0 != h1).sum(axis=1))
+        out = oracle.run(sim, max_sweeps=1)
+        assert np.array_equal(cnt, (h0 != out["state"]["hs"]).sum(axis=1))
+        eng.run(1)
+        h2 = eng.get_state(0)["hs"]
+        assert np.array_equal(eng.slice_accepts(0) - cnt, (h1 != h2).sum(axis=1))
+        assert eng.slice_accepts(0).sum() == eng.accepts()
+    finally:
+        eng.close()
