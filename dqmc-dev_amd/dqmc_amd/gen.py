@@ -97,9 +97,11 @@ def init_hs_field(rng, L, N):
 # ----------------------------------------------------------------------------
 # Hopping matrices (util/tight_binding.py)
 # ----------------------------------------------------------------------------
-def peierls_matrix(Nx, Ny, shape, nflux, alpha=0.5):
+def peierls_matrix(Nx, Ny, shape, nflux, alpha=0.5, jr_orbit=(0.0, 0.0), ir_orbit=(0.0, 0.0)):
     """Peierls phase matrix exp(i*2*pi*nflux/(A*Nx*Ny) * phi); phi[j, i] for the
-    hop i -> j with the symmetric-gauge wrap correction (tight_binding.py:4-78)."""
+    hop i -> j with the symmetric-gauge wrap correction (tight_binding.py:4-78).
+    ir_orbit / jr_orbit: positions of the start / end orbital in the unit cell
+    (the honeycomb and kagome sublattices)."""
     beta = 1.0 - alpha
     if shape == "triangular":
         a1 = (0.5, math.sqrt(3) / 2)
@@ -122,9 +124,10 @@ def peierls_matrix(Nx, Ny, shape, nflux, alpha=0.5):
                     jjx = jx % Nx
                     o1 = jx - jjx
                     o2 = jy - jjy
-                    ir = (ix * a1[0] + iy * a2[0], ix * a1[1] + iy * a2[1])
-                    jr = (jx * a1[0] + jy * a2[0], jx * a1[1] + jy * a2[1])
-                    jjr = (jjx * a1[0] + jjy * a2[0], jjx * a1[1] + jjy * a2[1])
+                    ir = (ix * a1[0] + iy * a2[0] + ir_orbit[0], ix * a1[1] + iy * a2[1] + ir_orbit[1])
+                    jr = (jx * a1[0] + jy * a2[0] + jr_orbit[0], jx * a1[1] + jy * a2[1] + jr_orbit[1])
+                    jjr = (jjx * a1[0] + jjy * a2[0] + jr_orbit[0],
+                           jjx * a1[1] + jjy * a2[1] + jr_orbit[1])
                     dr = (jr[0] - ir[0], jr[1] - ir[1])
                     mr = ((ir[0] + jr[0]) / 2, (ir[1] + jr[1]) / 2)
                     A_off = (-alpha * (o1 * a1[1] + o2 * a2[1]),
@@ -182,6 +185,172 @@ def hopping_triangular(Nx, Ny, t=1.0, nflux=0, alpha=0.5):
             kij[ix + Nx * iy1, ix1 + Nx * iy] += -t
     peierls = peierls_matrix(Nx, Ny, "triangular", nflux, alpha)
     return kij * peierls, peierls
+
+
+def hopping_honeycomb(Nx, Ny, t=1.0, nflux=0, alpha=0.5):
+    """Periodic honeycomb lattice, orbitals A (0) and B (1) per triangular-lattice
+    cell, site index x + Nx y + Nx Ny o (tight_binding.py:227-324).  Each A
+    site hops to the B sites of its own cell and of the cells one step along
+    either primitive vector.  Returns (K, P): the A-A and B-B blocks of P are
+    the triangular-lattice phases, the B<-A block the phases of hops to a site
+    displaced by the B offset (0, -1/sqrt 3), the A<-B block its conjugate
+    transpose; the off-diagonal blocks of K carry them."""
+    Nc = Nx * Ny
+    N = 2 * Nc
+    tij = np.zeros((N, N), dtype=np.complex128)
+    for iy in range(Ny):
+        for ix in range(Nx):
+            a = ix + Nx * iy
+            for bx, by in ((ix, (iy + 1) % Ny), ((ix + 1) % Nx, iy), (ix, iy)):
+                b = bx + Nx * by + Nc
+                tij[b, a] += -t
+                tij[a, b] += -t
+    pd = peierls_matrix(Nx, Ny, "triangular", nflux, alpha)
+    po = peierls_matrix(Nx, Ny, "triangular", nflux, alpha, jr_orbit=(0.0, -1.0 / math.sqrt(3)))
+    P = np.empty((N, N), dtype=np.complex128)
+    P[:Nc, :Nc] = pd
+    P[Nc:, Nc:] = pd
+    P[:Nc, Nc:] = po.T.conj()
+    P[Nc:, :Nc] = po
+    K = tij.copy()
+    K[:Nc, :Nc] *= pd
+    K[Nc:, Nc:] *= pd
+    K[:Nc, Nc:] *= po.T.conj()
+    K[Nc:, :Nc] *= po
+    if nflux != 0:
+        assert np.linalg.norm(K - K.T.conj()) < 1e-10
+    return K, P
+
+
+def hopping_kagome(Nx, Ny, t=1.0, nflux=0, alpha=0.5):
+    """Periodic kagome lattice, orbitals A (0), B (1), C (2) per triangular-
+    lattice cell, site index x + Nx y + Nx Ny o (tight_binding.py:330-452).
+    Bonds: A-B within the cell and A(x, y) - B(x, y+1); A-C within the cell and
+    C(x, y) - A(x-1, y); B-C within the cell and B(x, y) - C(x+1, y-1).  The
+    sublattice offsets B (1/4, -sqrt3/4) and C (-1/4, -sqrt3/4) enter the
+    phases of the inter-orbital blocks as in the reference."""
+    Nc = Nx * Ny
+    N = 3 * Nc
+    tij = np.zeros((N, N), dtype=np.complex128)
+
+    def bond(i, j):
+        tij[i, j] += -t
+        tij[j, i] += -t
+
+    for iy in range(Ny):
+        for ix in range(Nx):
+            c = ix + Nx * iy
+            bond(ix + Nx * ((iy + 1) % Ny) + Nc, c)                    # B(x, y+1) - A(x, y)
+            bond(c + Nc, c)                                             # B - A, same cell
+            bond(((ix - 1) % Nx) + Nx * iy, c + 2 * Nc)                 # A(x-1, y) - C(x, y)
+            bond(c, c + 2 * Nc)                                         # A - C, same cell
+            bond(((ix + 1) % Nx) + Nx * ((iy - 1) % Ny) + 2 * Nc, c + Nc)  # C(x+1, y-1) - B
+            bond(c + 2 * Nc, c + Nc)                                    # C - B, same cell
+    d4 = (0.25, -math.sqrt(3) / 4)   # B relative to A
+    d3 = (-0.25, -math.sqrt(3) / 4)  # C relative to A
+    p01 = peierls_matrix(Nx, Ny, "triangular", nflux, alpha, jr_orbit=d4)
+    p20 = peierls_matrix(Nx, Ny, "triangular", nflux, alpha, ir_orbit=d3)
+    p12 = peierls_matrix(Nx, Ny, "triangular", nflux, alpha, jr_orbit=d3, ir_orbit=d4)
+    pd = peierls_matrix(Nx, Ny, "triangular", nflux, alpha)
+    A, B, C = slice(0, Nc), slice(Nc, 2 * Nc), slice(2 * Nc, 3 * Nc)
+    P = np.empty((N, N), dtype=np.complex128)
+    for o in (A, B, C):
+        P[o, o] = pd
+    P[A, B], P[B, A] = p01.T.conj(), p01
+    P[C, A], P[A, C] = p20.T.conj(), p20
+    P[C, B], P[B, C] = p12, p12.T.conj()
+    K = tij.copy()
+    K[B, A] *= p01
+    K[A, B] *= p01.T.conj()
+    K[C, A] *= p20.T.conj()
+    K[A, C] *= p20
+    K[C, B] *= p12
+    K[B, C] *= p12.T.conj()
+    if nflux != 0:
+        assert np.linalg.norm(K - K.T.conj()) < 1e-10
+    return K, P
+
+
+NORB = {"square": 1, "triangular": 1, "honeycomb": 2, "kagome": 3}
+
+
+def _orbital_maps(geometry, Nx, Ny, trans_sym):
+    """Maps of the triangular, honeycomb and kagome lattices
+    (gen_1band_unified_hub.py:40-263, 697-903): sites by orbital, site pairs by
+    (displacement of the cells, orbital of each site), plaquettes (the square
+    lattice's two triangles per cell on the triangular lattice; the A-C-B
+    triangles on kagome; none on honeycomb).  The reference defines no bonds,
+    two-hop bonds or bond maps for these lattices (placeholders of zeros); the
+    same placeholders are written here, and the measurements that need them
+    are refused by create_1."""
+    Norb = NORB[geometry]
+    Nc = Nx * Ny
+    N = Norb * Nc
+    x = np.arange(N) % Nx
+    y = (np.arange(N) // Nx) % Ny
+    o = np.arange(N) // Nc
+    m = {}
+    if trans_sym:
+        map_i = o.astype(np.int32)
+        degen_i = np.full(Norb, Nc, dtype=np.int32)
+        # bin of the pair (j, i): cell displacement i - j and both orbitals
+        k = ((x[None, :] - x[:, None]) % Nx + Nx * ((y[None, :] - y[:, None]) % Ny)
+             + Nc * o[None, :] + N * o[:, None])
+        num_ij = Norb * Norb * Nc
+    else:
+        map_i = np.arange(N, dtype=np.int32)
+        degen_i = np.ones(N, dtype=np.int32)
+        k = np.arange(N)[None, :] + N * np.arange(N)[:, None]
+        num_ij = N * N
+    map_ij = k.astype(np.int32)  # map_ij[j, i]
+    degen_ij = np.bincount(map_ij.reshape(-1), minlength=num_ij).astype(np.int32)
+    assert np.all(degen_ij == degen_ij[0])
+    m.update(map_i=map_i, degen_i=degen_i, num_i=int(map_i.max()) + 1, map_ij=map_ij,
+             degen_ij=degen_ij, num_ij=num_ij)
+    ppc = {"triangular": 2, "honeycomb": 1, "kagome": 2}[geometry]
+    num_plaq = ppc * Nc
+    if trans_sym:
+        map_plaq = np.repeat(np.arange(ppc, dtype=np.int32), Nc)
+        degen_plaq = np.full(ppc, Nc, dtype=np.int32)
+    else:
+        map_plaq = np.arange(num_plaq, dtype=np.int32)
+        degen_plaq = np.ones(num_plaq, dtype=np.int32)
+    plaqs = np.zeros((3, num_plaq), dtype=np.int32)
+    c = np.arange(Nc)
+    cx, cy = c % Nx, c // Nx
+    cell = lambda a, b: (a % Nx) + Nx * (b % Ny)
+    if geometry == "triangular":
+        plaqs[:, :Nc] = (c, cell(cx + 1, cy), cell(cx, cy + 1))
+        plaqs[:, Nc:] = (cell(cx + 1, cy), cell(cx + 1, cy + 1), cell(cx, cy + 1))
+    elif geometry == "kagome":
+        plaqs[:, :Nc] = (c, c + 2 * Nc, c + Nc)
+        plaqs[:, Nc:] = (c, cell(cx + 1, cy) + 2 * Nc, cell(cx, cy + 1) + Nc)
+    m.update(plaqs=plaqs, map_plaq=map_plaq, degen_plaq=degen_plaq, num_plaq=num_plaq,
+             num_plaq_accum=int(map_plaq.max()) + 1, plaq_per_cell=ppc)
+    bps = {"triangular": 3, "honeycomb": 3, "kagome": 4}[geometry]
+    b2ps = 4
+    num_b, num_b2 = bps * N, b2ps * N
+    for name, per, n in (("b", bps, num_b), ("b2", b2ps, num_b2)):
+        if trans_sym:
+            mp = np.repeat(np.arange(per, dtype=np.int32), N)
+            dg = np.full(per, N, dtype=np.int32)
+        else:
+            mp = np.arange(n, dtype=np.int32)
+            dg = np.ones(n, dtype=np.int32)
+        m.update({"map_" + name: mp, "degen_" + name: dg, "num_%s_accum" % name: int(mp.max()) + 1})
+    cnt = {"bs": (bps * N if trans_sym else num_b * N, (N, num_b)),
+           "bb": (bps * bps * N if trans_sym else num_b * num_b, (num_b, num_b)),
+           "b2b2": (b2ps * b2ps * N if trans_sym else num_b2 * num_b2, (num_b2, num_b2)),
+           "bb2": (bps * b2ps * N if trans_sym else num_b * num_b2, (num_b, num_b2)),
+           "b2b": (b2ps * bps * N if trans_sym else num_b2 * num_b, (num_b2, num_b))}
+    for name, (n, shape) in cnt.items():
+        m["num_" + name] = n
+        m["map_" + name] = np.zeros(shape, dtype=np.int32)
+        m["degen_" + name] = np.zeros(max(n, 1), dtype=np.int32)
+    m.update(bps=bps, b2ps=b2ps, num_b=num_b, num_b2=num_b2,
+             bonds=np.zeros((2, num_b), dtype=np.int32),
+             bond2s=np.zeros((2, num_b2), dtype=np.int32))
+    return m
 
 
 # ----------------------------------------------------------------------------
@@ -386,7 +555,9 @@ def create_1(init_rng=None, geometry="square", bc=1, Nx=4, Ny=4, mu=0.0, tp=0.0,
         raise ValueError("init_rng is required (use splitmix64_seed(seed))")
     init_rng = [int(v) for v in init_rng]
     rng = XorShift1024Star(init_rng)
-    N = Nx * Ny
+    if geometry not in NORB:
+        raise NotImplementedError("geometry %r" % geometry)
+    N = NORB[geometry] * Nx * Ny
     hs = init_hs_field(rng, L, N)
 
     if tog["meas_gen_suscept"] and not trans_sym:
@@ -396,12 +567,24 @@ def create_1(init_rng=None, geometry="square", bc=1, Nx=4, Ny=4, mu=0.0, tp=0.0,
         kij, peierls = hopping_square(Nx, Ny, t=1.0, tp=tp, tpp=tpp, nflux=nflux, alpha=alpha,
                                       twistx=twistx, twisty=twisty)
         thermal = _thermal_phases(maps, peierls, tp, N)
-    elif geometry == "triangular":
-        maps = _square_maps(Nx, Ny, 0.0, trans_sym)  # placeholder maps (trajectory tests only)
-        kij, peierls = hopping_triangular(Nx, Ny, t=1.0, nflux=nflux, alpha=alpha)
-        thermal = np.ones((maps["b2ps"], N), dtype=np.complex128)
     else:
-        raise NotImplementedError("geometry %r" % geometry)
+        # the reference builds no bonds for these lattices (placeholders,
+        # util/gen_1band_unified_hub.py:137-160, 697-903) and its tight-binding
+        # models have no t' (util/tight_binding.py:195-196)
+        if tp != 0.0:
+            raise NotImplementedError("t' on the %s lattice" % geometry)
+        if twistx != 0.0 or twisty != 0.0:
+            raise NotImplementedError("twisted boundaries on the %s lattice" % geometry)
+        for k in ("meas_bond_corr", "meas_thermal", "meas_2bond_corr", "meas_energy_corr",
+                  "meas_nematic_corr", "meas_local_JQ", "meas_pair_bb_only"):
+            if tog[k]:
+                raise NotImplementedError("%s needs bonds, which the %s lattice does not define"
+                                          % (k, geometry))
+        maps = _orbital_maps(geometry, Nx, Ny, trans_sym)
+        hop = {"triangular": hopping_triangular, "honeycomb": hopping_honeycomb,
+               "kagome": hopping_kagome}[geometry]
+        kij, peierls = hop(Nx, Ny, t=1.0, nflux=nflux, alpha=alpha)
+        thermal = np.ones((maps["b2ps"], N), dtype=np.complex128)
 
     if not cplx:
         thermal = thermal.real
@@ -523,8 +706,9 @@ def create_1(init_rng=None, geometry="square", bc=1, Nx=4, Ny=4, mu=0.0, tp=0.0,
         if tog["meas_nematic_corr"]:
             mue["nem_nnnn"] = z(nbb * L)
             mue["nem_ssss"] = z(nbb * L)
-    meta = {"model": "Hubbard (complex)" if cplx else "Hubbard", "Nx": Nx, "Ny": Ny, "Norb": 1,
-            "bps": maps["bps"], "b2ps": maps["b2ps"], "plaq_per_cell": 2, "U": U, "t'": tp,
+    meta = {"model": "Hubbard (complex)" if cplx else "Hubbard", "Nx": Nx, "Ny": Ny,
+            "Norb": NORB[geometry], "bps": maps["bps"], "b2ps": maps["b2ps"],
+            "plaq_per_cell": maps["plaq_per_cell"], "U": U, "t'": tp,
             "t''": tpp, "nflux": nflux, "twistx": twistx, "twisty": twisty, "mu": mu, "h": h,
             "beta": L * dt, "trans_sym": trans_sym, "geometry": geometry, "bc": bc,
             "gauge_alpha": alpha,

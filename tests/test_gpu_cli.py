@@ -61,6 +61,20 @@ def test_dqmc_1_golden_file_with_checkpoints(tmp_path, golden):
     assert np.array_equal(out["state"]["hs"], np.array(golden["hs_end"]["square"]))
 
 
+@pytest.mark.parametrize("geometry", ["honeycomb", "kagome"])
+def test_dqmc_1_other_lattices_with_unequal_time(tmp_path, geometry):
+    """A two- and a three-orbital lattice through the command-line driver, with
+    flux (complex path) and unequal-time measurements, against the oracle."""
+    sim = gen.create_batch(seed=71, geometry=geometry, Nx=4, Ny=4, U=4.0, mu=-0.2, L=20,
+                           n_matmul=5, period_eqlt=5, period_uneqlt=2, nflux=1,
+                           n_sweep_warm=4, n_sweep_meas=6)[0]
+    fn = str(tmp_path / (geometry + ".h5"))
+    h5.write_sim(fn, sim)
+    r = _run([os.path.join(BIN, "dqmc_1"), "-l", fn + ".log", fn])
+    assert r.returncode == 0, open(fn + ".log").read()
+    _check_against_oracle(fn, sim, ue_keys=("gt0", "nn", "xx", "zz", "pair_sw"))
+
+
 def test_dqmc_stack_batches_two_shapes(tmp_path):
     a = gen.create_batch(Nfiles=3, seed=31, Nx=4, Ny=4, U=5.0, mu=-0.4, L=20, n_matmul=5,
                          period_eqlt=5, period_uneqlt=2, n_sweep_warm=2, n_sweep_meas=6)

@@ -23,10 +23,13 @@ def _same(a, b):
     return a.dtype == b.dtype and a.shape == b.shape and np.array_equal(a, b)
 
 
+@pytest.mark.parametrize("geometry", ["square", "honeycomb", "kagome"])
 @pytest.mark.parametrize("nflux", [0, 1])
-def test_h5_roundtrip_batch_layout(tmp_path, nflux):
-    sims = gen.create_batch(Nfiles=3, seed=5, Nx=4, Ny=4, L=20, n_matmul=5, period_eqlt=5,
-                            period_uneqlt=1, nflux=nflux, meas_bond_corr=1, meas_energy_corr=1)
+def test_h5_roundtrip_batch_layout(tmp_path, nflux, geometry):
+    # bond measurements are defined on the square lattice only
+    extra = dict(meas_bond_corr=1, meas_energy_corr=1) if geometry == "square" else {}
+    sims = gen.create_batch(Nfiles=3, seed=5, geometry=geometry, Nx=4, Ny=4, L=20, n_matmul=5,
+                            period_eqlt=5, period_uneqlt=1, nflux=nflux, **extra)
     files = h5.create_batch_files(str(tmp_path / "sim"), sims)
     assert os.path.exists(str(tmp_path / "sim.h5.params"))
     for fn, s in zip(files, sims):

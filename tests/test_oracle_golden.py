@@ -66,15 +66,26 @@ def test_jump_commutes_with_next():
     assert a.s == b.s
 
 
-@pytest.mark.parametrize("geometry", ["square", "triangular"])
+GEOMETRIES = ["square", "triangular", "honeycomb", "kagome"]
+
+
+@pytest.mark.parametrize("geometry", GEOMETRIES)
+def test_initial_fields_of_every_lattice(golden, geometry):
+    # hs_init of test/hs_ref.py: N = Norb Nx Ny sites (16, 16, 32, 48) per slice
+    s = gen.create_batch(seed=1234, geometry=geometry, n_sweep_warm=20, n_sweep_meas=20)[0]
+    assert np.array_equal(s["state"]["hs"], np.array(golden["hs_init"][geometry]))
+
+
+@pytest.mark.parametrize("geometry", GEOMETRIES)
 def test_end_to_end_hs_trajectory(golden, geometry):
     s = gen.create_batch(seed=1234, geometry=geometry, n_sweep_warm=20, n_sweep_meas=20)[0]
     out = oracle.run(s)
     assert out["sweeps_run"] == 40
     assert np.array_equal(out["state"]["hs"], np.array(golden["hs_end"][geometry]))
     # RNG invariant: 2*N*L draws per sweep at U > 0
+    N = int(s["params"]["N"])
     r = gen.XorShift1024Star([int(v) for v in s["state"]["rng"]])
-    for _ in range(2 * 16 * 40 * 40):
+    for _ in range(2 * N * 40 * 40):
         r.next()
     assert [int(v) for v in out["state"]["rng"]] == r.s
     if geometry == "square":
