@@ -16,13 +16,26 @@
 // accumulating W, sum tr G_l and the lambda term; log|det M| comes from the
 // factorisation of the first calc_eq_g:
 //   I + Q D T = Q diag(v)^-1 M',  M' = Q_M R_M  =>  log|det M| = sum log|R_ii| - sum log|v_i|.
+//
+// The same walk gives derivatives of a G-based observable O = tr(X G), G the
+// equal-time Green's function at slice 0 and X fixed: dG = -G dM G and
+// dM = sum_l B_{L-1}..B_{l+1} dB_l B_{l-1}..B_0 give
+//   dO = sum_l tr(Z_l B_l^-1 dB_l),   Z_l = G(t_l,0) X G^(0,t_l),
+// with the unequal-time Green's functions G(t_l,0) = B_{l-1}..B_0 G and
+// G^(0,t_l) = -(I - G) (B_{l-1}..B_0)^-1 (at l = 0: G - I, the limit from
+// above, where calc_ue_g stores G itself).  B_l^-1 dB_l is e^{-V_l} E^-1 dE
+// e^{V_l} for the hopping and dV_l for the HS coupling, so O has the
+// log-determinant's chain rule with (I - G_l) replaced by Z_l:
+//   W_X = sum_l e^{V_l} Z_l e^{-V_l},  dO/dmu = dt sum_l tr Z_l,
+//   dO/dlambda = sum_l sum_i sigma s_li (Z_l)_ii.
 #pragma once
 #include "dq_common.hpp"
 
-// W[b] += e^{V_l} (I - G) e^{-V_l} (elementwise with the slice's scales)
+// W[b] += e^{V_l} (a I + c G) e^{-V_l} (elementwise with the slice's scales);
+// a = 1, c = -1 for the log-determinant, a = 0, c = 1 for Z_l
 template <typename T>
 __global__ void k_adj_accum_W(int N, const T* G, long sG, const double* esc, const double* isc,
-                              long ss, int l, T* W, long sW, int B)
+                              long ss, int l, T* W, long sW, int B, double a, double c)
 {
     const long NN = (long)N * N;
     for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < NN * B;
@@ -31,26 +44,27 @@ __global__ void k_adj_accum_W(int N, const T* G, long sG, const double* esc, con
         const long r = e % NN;
         const int i = (int)(r % N), j = (int)(r / N);
         const T g = G[b * sG + r];
-        const T ig = (i == j ? from_re<T>(1.0) : from_re<T>(0.0)) - g;
+        const T ig = from_re<T>(i == j ? a : 0.0) + c * g;
         W[b * sW + r] += (esc[b * ss + (long)N * l + i] * isc[b * ss + (long)N * l + j]) * ig;
     }
 }
 
-// trG[b] += tr G;  sG[b] += sigma_b sum_i s_li (1 - G_ii), s = 2 hs - 1,
+// trG[b] += tr G;  sG[b] += sigma_b sum_i s_li (a + c G_ii), s = 2 hs - 1,
 // sigma = +1 (up, b even) / -1 (down)
 template <typename T>
 __global__ __launch_bounds__(64) void k_adj_accum_diag(int N, int L, const T* G, long sG,
-                                                       const int* hs, int l, T* trG, T* sGl)
+                                                       const int* hs, int l, T* trG, T* sGl,
+                                                       double a, double c)
 {
     const int b = blockIdx.x, lane = threadIdx.x;
-    const int c = b >> 1;
+    const int cc = b >> 1;
     const double sigma = (b & 1) ? -1.0 : 1.0;
-    const int* h = hs + (long)c * L * N + (long)N * l;
+    const int* h = hs + (long)cc * L * N + (long)N * l;
     T t = from_re<T>(0.0), s = t;
     for (int i = lane; i < N; i += 64) {
         const T g = G[b * sG + i + (long)N * i];
         t += g;
-        s += (sigma * (2 * h[i] - 1)) * (from_re<T>(1.0) - g);
+        s += (sigma * (2 * h[i] - 1)) * (a + c * g);
     }
     t = wave_sum(t);
     s = wave_sum(s);
@@ -72,4 +86,30 @@ __global__ __launch_bounds__(64) void k_eqg_logdet(int n, const double* v_, long
     for (int i = lane; i < n; i += 64) s += log(absv(R[i + (long)n * i])) - log(fabs(v[i]));
     s = wave_sum(s);
     if (lane == 0) out[b] = s;
+}
+
+// out[b] = tr(X_b G_b) = sum_ij X_ij G_ji (column-major N x N)
+template <typename T>
+__global__ __launch_bounds__(64) void k_adj_trace_prod(int N, const T* X, long sX, const T* G,
+                                                       long sG, T* out)
+{
+    const int b = blockIdx.x, lane = threadIdx.x;
+    const long NN = (long)N * N;
+    T s = from_re<T>(0.0);
+    for (long r = lane; r < NN; r += 64) {
+        const int i = (int)(r % N), j = (int)(r / N);
+        s += X[b * sX + r] * G[b * sG + j + (long)N * i];
+    }
+    s = wave_sum(s);
+    if (lane == 0) out[b] = s;
+}
+
+// X[b] += v I
+template <typename T>
+__global__ void k_adj_add_diag(int N, T* X, long sX, double v, int B)
+{
+    const long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (e >= (long)N * B) return;
+    const int b = (int)(e / N), i = (int)(e % N);
+    X[b * sX + i + (long)N * i] += from_re<T>(v);
 }

@@ -206,6 +206,7 @@ struct dqmc_engine {
     // adjoint outputs (allocated on first use)
     void *adjW = 0, *adjT = 0;
     double* adjL = 0;
+    void *adjX = 0, *adjZ = 0, *adjO = 0;  // observable adjoint: X per (chain, spin), Z_l, tr(X G)
     int* invij = 0;      // [c][j][num_ij] row of column j in bin r (when ij_percol)
     bool ij_percol = false;
     // chains of uneven size (padded to N, L): own slice / site counts
@@ -822,9 +823,9 @@ static int adjoint_t(dqmc_engine* e, double* logdet, double* Wout, double* trG, 
     LCHK();
     for (int l = 0; l < L; l++) {
         k_adj_accum_W<T><<<nblk(B * NN), 256, 0, e->st>>>(N, G, NN, e->escale, e->iscale, LN, l, W,
-                                                           NN, B);
+                                                           NN, B, 1.0, -1.0);
         LCHK();
-        k_adj_accum_diag<T><<<B, 64, 0, e->st>>>(N, L, G, NN, e->hs, l, tr, sg);
+        k_adj_accum_diag<T><<<B, 64, 0, e->st>>>(N, L, G, NN, e->hs, l, tr, sg, 1.0, -1.0);
         LCHK();
         if ((l + 1) % n == 0) {
             RC(calc_eq_g<T>(e, ((l / n) + 1) % F, G, dp<T>(e->phb)));
@@ -1557,6 +1558,121 @@ extern "C" int dqmc_engine_adjoint(dqmc_engine* e, double* logdet, double* W, do
     if (e->ragged) return set_err(DQMC_ERR_UNSUPPORTED, "the adjoint needs chains of equal size");
     HIPCHK(hipSetDevice(e->device));
     return e->cplx ? adjoint_t<zd>(e, logdet, W, trG, sG) : adjoint_t<double>(e, logdet, W, trG, sG);
+}
+
+// The unequal-time buffers of an engine whose files measure no unequal-time
+// quantities (allocated on first use by the observable adjoint): the G blocks,
+// bsofi's work space, and Householder vectors for its 2N-row panels (4 N^2 per
+// matrix where calc_eq_g alone needs N^2).  The sweep graphs captured with the
+// smaller vector block are dropped; that block stays allocated until destroy.
+static int ue_ensure(dqmc_engine* e)
+{
+    if (e->G0t) return 0;
+    HIPCHK(hipStreamSynchronize(e->st));  // no graph of this engine still running
+    const long NN = (long)e->N * e->N, LNN = (long)e->L * NN, B = 2L * e->nc;
+    DALLOC(e->G0t, B * LNN * e->esz);
+    DALLOC(e->Gtt, B * LNN * e->esz);
+    DALLOC(e->Gt0, B * LNN * e->esz);
+    RC(ue_alloc(e));
+    if (e->sVb < 4 * NN) {
+        for (int k = 0; k < 4; k++) {
+            if (e->gexec[k]) (void)hipGraphExecDestroy(e->gexec[k]);
+            if (e->graph[k]) (void)hipGraphDestroy(e->graph[k]);
+            e->gexec[k] = nullptr;
+            e->graph[k] = nullptr;
+        }
+        e->sVb = 4 * NN;
+        DALLOC(e->Vb, B * e->sVb * e->esz);
+    }
+    return 0;
+}
+
+// Observable adjoint: O[b] = tr(X_s G) at slice 0 and, with the unequal-time
+// Green's functions of calc_ue_g in the plain frame, the sums
+// W_X = sum_l e^{V_l} Z_l e^{-V_l}, sum_l tr Z_l and sum_l sum_i sigma s_li
+// (Z_l)_ii with Z_l = G(t_l,0) X G^(0,t_l) (dq_adjoint.hpp).  Xh: the two
+// spins' X, column-major, shared by every chain.
+template <typename T>
+static int adjoint_green_t(dqmc_engine* e, const double* Xh, double* obs, double* Wout,
+                           double* trZ, double* sZ)
+{
+    const int N = e->N, L = e->L, nc = e->nc, B = 2 * nc;
+    const long NN = (long)N * N, LN = (long)L * N, LNN = (long)L * NN;
+    const T one = from_re<T>(1.0), zero = from_re<T>(0.0);
+    RC(ue_ensure(e));
+    if (!e->adjW) {
+        DALLOC(e->adjW, B * NN * e->esz);
+        DALLOC(e->adjT, 2L * B * e->esz);
+        DALLOC(e->adjL, B * 8);
+    }
+    if (!e->adjX) {
+        DALLOC(e->adjX, B * NN * e->esz);
+        DALLOC(e->adjZ, B * NN * e->esz);
+        DALLOC(e->adjO, B * e->esz);
+    }
+    {
+        std::vector<T> xb((size_t)B * NN);
+        for (int b = 0; b < B; b++)
+            memcpy(xb.data() + (size_t)b * NN, (const char*)Xh + (size_t)(b & 1) * NN * sizeof(T),
+                   NN * sizeof(T));
+        HIPCHK(hipMemcpyAsync(e->adjX, xb.data(), xb.size() * sizeof(T), hipMemcpyHostToDevice,
+                              e->st));
+        HIPCHK(hipStreamSynchronize(e->st));  // xb leaves scope
+    }
+    T* X = dp<T>(e->adjX);
+    T* Z = dp<T>(e->adjZ);
+    T* W = dp<T>(e->adjW);
+    T* tr = dp<T>(e->adjT);
+    T* sg = tr + B;
+    T* T1 = dp<T>(e->T1);
+    T* G0t = dp<T>(e->G0t);
+    T* Gt0 = dp<T>(e->Gt0);
+    k_scales<<<nblk((long)nc * L * N), 256, 0, e->st>>>(N, L, nc, e->hs, e->exp_lambda, e->escale,
+                                                        e->iscale);
+    LCHK();
+    RC(build_C<T>(e, 0, e->F));
+    RC(ue_calc<T>(e, false));
+    k_adj_trace_prod<T><<<B, 64, 0, e->st>>>(N, X, NN, dp<T>(e->Gtt), LNN, dp<T>(e->adjO));
+    LCHK();
+    // G^(0, 0) = G - I
+    k_adj_add_diag<T><<<nblk((long)N * B), 256, 0, e->st>>>(N, G0t, LNN, -1.0, B);
+    LCHK();
+    {
+        const long w = (long)(sizeof(T) / 8);  // doubles per element
+        k_zero<<<nblk(B * NN * w), 256, 0, e->st>>>(B * NN * w, (double*)W);
+        LCHK();
+        k_zero<<<1, 256, 0, e->st>>>(2L * B * w, (double*)tr);
+        LCHK();
+    }
+    for (int l = 0; l < L; l++) {
+        // T1 = X G^(0, t_l), Z = G(t_l, 0) T1
+        RC(gemm<T>(e, OP_N, OP_N, N, N, N, one, X, N, NN, G0t + (long)l * NN, N, LNN, zero, T1, N,
+                   NN, B));
+        RC(gemm<T>(e, OP_N, OP_N, N, N, N, one, Gt0 + (long)l * NN, N, LNN, T1, N, NN, zero, Z, N,
+                   NN, B));
+        k_adj_accum_W<T><<<nblk(B * NN), 256, 0, e->st>>>(N, Z, NN, e->escale, e->iscale, LN, l, W,
+                                                           NN, B, 0.0, 1.0);
+        LCHK();
+        k_adj_accum_diag<T><<<B, 64, 0, e->st>>>(N, L, Z, NN, e->hs, l, tr, sg, 0.0, 1.0);
+        LCHK();
+    }
+    HIPCHK(hipStreamSynchronize(e->st));
+    if (obs) HIPCHK(hipMemcpy(obs, e->adjO, B * sizeof(T), hipMemcpyDeviceToHost));
+    if (Wout) HIPCHK(hipMemcpy(Wout, W, B * NN * sizeof(T), hipMemcpyDeviceToHost));
+    if (trZ) HIPCHK(hipMemcpy(trZ, tr, B * sizeof(T), hipMemcpyDeviceToHost));
+    if (sZ) HIPCHK(hipMemcpy(sZ, sg, B * sizeof(T), hipMemcpyDeviceToHost));
+    e->stale = true;
+    return 0;
+}
+
+extern "C" int dqmc_engine_adjoint_green(dqmc_engine* e, const double* X, double* obs, double* W,
+                                         double* trZ, double* sZ)
+{
+    if (!e || !X) return set_err(DQMC_ERR_ARG, "null engine or X");
+    if (e->ragged) return set_err(DQMC_ERR_UNSUPPORTED, "the adjoint needs chains of equal size");
+    HIPCHK(hipSetDevice(e->device));
+    return e->cplx ? adjoint_green_t<zd>(e, X, obs, W, trZ, sZ)
+                   : adjoint_green_t<double>(e, X, obs, W, trZ, sZ);
 }
 
 extern "C" int dqmc_engine_gemm_tile(dqmc_engine* e)

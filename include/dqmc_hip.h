@@ -197,8 +197,25 @@ int dqmc_engine_calc_ue_g(dqmc_engine *e, int chain, int spin, double *G0t, doub
  *   sG[2c+s]     = sum_l sum_i sigma_s s_li (1 - G_l,ii) (num), s = 2 hs - 1,
  *                  sigma = +1 up / -1 down;  d logdet / d lambda = sG
  * G is advanced by wraps and recomputed from the stored group products every
- * n_matmul slices.  Any pointer may be NULL.  The next run recomputes G. */
+ * n_matmul slices.  Any pointer may be NULL.  The next run recomputes G.
+ * "num" is one double per value for real engines and a (re, im) pair for
+ * complex ones (W: N*N of them per (chain, spin)). */
 int dqmc_engine_adjoint(dqmc_engine *e, double *logdet, double *W, double *trG, double *sG);
+
+/* Adjoint of a G-based observable.  No counterpart in the reference.  For the
+ * current HS field of every chain and X[s] (N x N column-major, num; X[0] for
+ * spin up, X[1] for spin down, shared by the chains), with G the equal-time
+ * Green's function at slice 0 and Z_l = G(t_l,0) X G^(0,t_l) built from the
+ * unequal-time Green's functions of calc_ue_g (G^(0,0) = G - I):
+ *   obs[2c+s] = tr(X_s G)                                              (num)
+ *   W[2c+s]   = sum_l e^{V_l} Z_l e^{-V_l}  (num, N x N);  d obs = Re tr(W E^-1 dE)
+ *   trZ[2c+s] = sum_l tr Z_l  (num);  d obs / d mu = dt trZ
+ *   sZ[2c+s]  = sum_l sum_i sigma_s s_li (Z_l)_ii  (num);  d obs / d lambda = sZ
+ * The unequal-time buffers are allocated on first use when the files measure
+ * no unequal-time quantities.  Any output pointer may be NULL.  The next run
+ * recomputes G. */
+int dqmc_engine_adjoint_green(dqmc_engine *e, const double *X, double *obs, double *W,
+                              double *trZ, double *sZ);
 
 /* Accepted flips so far, summed over chains (for the flop model). */
 int64_t dqmc_engine_accepts(dqmc_engine *e);
