@@ -184,3 +184,21 @@ __global__ void k_zero(long n, double* p)
     for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x)
         p[e] = 0.0;
 }
+
+// X[b][l] = Y[b][l] = I for the slices l >= Lc[b / 2] of every (chain, spin) b
+template <typename T>
+__global__ void k_ue_identity_slices(int N, int L, T* X, T* Y, const int* Lc, int B)
+{
+    const long NN = (long)N * N, LNN = (long)L * NN;
+    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < LNN * B;
+         e += (long)gridDim.x * blockDim.x) {
+        const int b = (int)(e / LNN);
+        const long r = e % LNN;
+        const int l = (int)(r / NN);
+        if (l < Lc[b >> 1]) continue;
+        const long q = r % NN;
+        const T v = from_re<T>((q % N) == (q / N) ? 1.0 : 0.0);
+        X[b * LNN + r] = v;
+        Y[b * LNN + r] = v;
+    }
+}

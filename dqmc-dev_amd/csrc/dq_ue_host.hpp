@@ -148,6 +148,14 @@ static int ue_half_factors(dqmc_engine* e)
     e->gemm_bdiv = L;
     RC(gemm<T>(e, OP_N, OP_N, N, N, N, one, dp<T>(e->iehalf), N, NN, M2, N, NN, zero,
                dp<T>(e->Bthi), N, NN, B * L, nullptr, 0, e->iscale, N));
+    if (e->ragged) {
+        // slices past a chain's own L are identity slices: B~_l = B~_l^-1 = I.
+        // They sit after the chain's last slice, so G(t,0), G(0,t), G(t,t) of
+        // its own slices t < L_c are those of the chain alone.
+        k_ue_identity_slices<T><<<nblk((long)B * L * NN), 256, 0, e->st>>>(
+            N, L, dp<T>(e->T2), dp<T>(e->Bthi), e->d_Lc, B);
+        LCHK();
+    }
     return 0;
 }
 

@@ -1746,11 +1746,6 @@ extern "C" int dqmc_engine_create(dqmc_engine** out, const dqmc_params* const* P
                                lc, ncs, p0->L, p0->N);
             }
         }
-        if (e->period_uneqlt > 0) {
-            delete e;
-            return set_err(DQMC_ERR_UNSUPPORTED,
-                           "unequal-time measurements need chains of equal size (chain_L/chain_N)");
-        }
     }
     e->pe.resize(n_chains);
     e->meas_at.assign(e->L, 0);
@@ -2052,6 +2047,7 @@ extern "C" int dqmc_engine_calc_ue_g(dqmc_engine* e, int c, int spin, double* G0
     if (!e || c < 0 || c >= e->nc || spin < 0 || spin > 1) return set_err(DQMC_ERR_ARG, "bad args");
     if (e->period_uneqlt <= 0)
         return set_err(DQMC_ERR_ARG, "engine created with period_uneqlt == 0");
+    if (e->ragged) return set_err(DQMC_ERR_UNSUPPORTED, "padded chains: measured blocks only");
     HIPCHK(hipSetDevice(e->device));
     return e->cplx ? ue_hook<zd>(e, c, spin, G0t, Gtt, Gt0)
                    : ue_hook<double>(e, c, spin, G0t, Gtt, Gt0);

@@ -13,7 +13,11 @@ engine of the bucket's N and L:
   ``n_matmul`` slices follow the engine's, so its last group may be partial;
 * the padding sites and pairs map to no measurement bin (-1), and the bins
   beyond a chain's own count (up to the largest chain's) stay empty; both
-  are dropped again by :meth:`PaddedBatch.meas`.
+  are dropped again by :meth:`PaddedBatch.meas`;
+* unequal-time measurements (the default set, one period_uneqlt for the
+  bucket): the identity slices follow the chain's last slice, so its G(t,0),
+  G(0,t), G(t,t) for t < L_c are its own; the accumulator slices t >= L_c
+  and the padding bins are dropped by :meth:`PaddedBatch.meas_uneqlt`.
 
 Every chain keeps its own ``period_eqlt`` (the engine measures each padded
 chain at its own cadence), so its accumulators collect the samples a run of
@@ -21,8 +25,8 @@ its own file collects.  A chain whose L is a multiple of the engine's
 ``n_matmul`` runs exactly as it would alone (same random stream,
 recalculation points and measurement slices); at another ``n_matmul`` only
 the recalculation points move, which changes G by rounding.  Supported: the
-default equal-time measurements; the optional groups and unequal-time
-measurements need equal sizes.
+default equal- and unequal-time measurements; the optional groups need equal
+sizes.
 """
 import numpy as np
 
@@ -30,6 +34,7 @@ from . import engine as _eng
 
 _NUM_I = ("density", "density_u", "density_d", "double_occ")
 _NUM_IJ = ("g00", "g00_u", "g00_d", "nn", "xx", "zz", "pair_sw")
+_UE_IJ = ("gt0", "gt0_u", "gt0_d", "nn", "xx", "zz", "pair_sw")  # [L][num_ij] each
 _EXPS = ("exp_Ku", "exp_Kd", "inv_exp_Ku", "inv_exp_Kd", "exp_halfKu", "exp_halfKd",
          "inv_exp_halfKu", "inv_exp_halfKd")
 _TOGGLES = ("meas_bond_corr", "meas_thermal", "meas_2bond_corr", "meas_energy_corr",
@@ -54,12 +59,10 @@ def _block(a, n_pad, fill_diag):
 
 
 def pad_sim(sim, N_pad, L_pad, n_matmul, num_i_pad, num_ij_pad):
-    """Padded copy of a simulation dict (params, state, meas_eqlt); the
-    chain's own period_eqlt is kept."""
+    """Padded copy of a simulation dict (params, state, meas_eqlt,
+    meas_uneqlt); the chain's own period_eqlt is kept."""
     p = sim["params"]
     N, L = _int(p["N"]), _int(p["L"])
-    if _int(p.get("period_uneqlt", 0)) > 0:
-        raise ValueError("unequal-time measurements need chains of equal size")
     for t in _TOGGLES:
         if _int(p.get(t, 0)):
             raise ValueError("%s needs chains of equal size" % t)
@@ -99,7 +102,18 @@ def pad_sim(sim, N_pad, L_pad, n_matmul, num_i_pad, num_ij_pad):
             meq[k] = out
         else:
             meq[k] = v.copy()
-    return {"params": q, "state": st, "meas_eqlt": meq, "meas_uneqlt": {},
+    mue = {}
+    if _int(p.get("period_uneqlt", 0)) > 0:
+        nij = _int(p["num_ij"])
+        for k, v in sim["meas_uneqlt"].items():
+            v = np.asarray(v)
+            if k in _UE_IJ:
+                out = np.zeros((L_pad, num_ij_pad), dtype=v.dtype)
+                out[:L, :nij] = v.reshape(L, nij)
+                mue[k] = out.reshape(-1)
+            else:
+                mue[k] = v.copy()
+    return {"params": q, "state": st, "meas_eqlt": meq, "meas_uneqlt": mue,
             "metadata": sim.get("metadata", {})}
 
 
@@ -120,6 +134,8 @@ class PaddedBatch:
         self.n_matmul = n_matmul
         self.num_i = [_int(s["params"]["num_i"]) for s in sims]
         self.num_ij = [_int(s["params"]["num_ij"]) for s in sims]
+        if len({_int(s["params"].get("period_uneqlt", 0)) for s in sims}) != 1:
+            raise ValueError("the chains of a bucket share period_uneqlt")
         ni, nij = max(self.num_i), max(self.num_ij)
         self.padded = [pad_sim(s, self.N, self.L, n_matmul, ni, nij) for s in sims]
         self.engine = _eng.Engine(self.padded, device=device, gemm_tile=gemm_tile, graphs=graphs,
@@ -152,6 +168,18 @@ class PaddedBatch:
                 out[k] = v[:self.num_i[c]].copy()
             elif k in _NUM_IJ:
                 out[k] = v[:self.num_ij[c]].copy()
+            else:
+                out[k] = v
+        return out
+
+    def meas_uneqlt(self, c):
+        """Chain c's unequal-time accumulators in its own slices and bins."""
+        _, ue = self.engine.get_meas(c)
+        out = {}
+        nij_pad = max(self.num_ij)
+        for k, v in ue.items():
+            if k in _UE_IJ:
+                out[k] = v.reshape(self.L, nij_pad)[:self.Lc[c], :self.num_ij[c]].reshape(-1).copy()
             else:
                 out[k] = v
         return out

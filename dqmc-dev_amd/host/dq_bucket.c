@@ -10,7 +10,6 @@ static const int BUCKETS[] = {64, 121, 196, 289, 400, 512};
 
 int dq_bucket_of(const dqmc_params *p)
 {
-    if (p->period_uneqlt > 0) return 0;
     if (p->meas_bond_corr || p->meas_thermal || p->meas_2bond_corr || p->meas_energy_corr ||
         p->meas_nematic_corr || p->meas_pair_bb_only || p->meas_chiral || p->meas_local_JQ ||
         p->meas_gen_suscept)
@@ -24,7 +23,8 @@ int dq_bucket_compatible(const dqmc_params *a, const dqmc_params *b)
 {
     const int ka = dq_bucket_of(a);
     return ka != 0 && ka == dq_bucket_of(b) && a->n_matmul == b->n_matmul &&
-           a->period_eqlt == b->period_eqlt && a->is_complex == b->is_complex &&
+           a->period_eqlt == b->period_eqlt && a->period_uneqlt == b->period_uneqlt &&
+           a->is_complex == b->is_complex &&
            a->L % a->n_matmul == 0 && b->L % b->n_matmul == 0;
 }
 
@@ -115,8 +115,9 @@ int dq_pad_build(const dq_sim *s, int N_pad, int L_pad, int num_i_pad, int num_i
     q->del = dl;
     q->map_i = mi;
     q->map_ij = mij;
-    /* equal-time accumulators of the padded bin counts */
+    /* accumulators of the padded bin (and, unequal-time, slice) counts */
     d->meq = s->meq;
+    d->mue = s->mue;
     for (int k = 0; k < DQMC_MAX_ACC; k++) {
         const long len = k < DQ_EQ_COUNT ? dqmc_acc_len(q, 0, k) : 0;
         d->meq.acc[k] = NULL;
@@ -124,8 +125,32 @@ int dq_pad_build(const dq_sim *s, int N_pad, int L_pad, int num_i_pad, int num_i
             d->meq.acc[k] = own(d, (size_t)len * es * sizeof(double));
             if (!d->meq.acc[k]) return -2;
         }
+        const long lue = (p->period_uneqlt > 0 && k < DQ_UE_COUNT) ? dqmc_acc_len(q, 1, k) : 0;
+        d->mue.acc[k] = NULL;
+        if (lue > 0 && s->mue.acc[k]) {
+            d->mue.acc[k] = own(d, (size_t)lue * es * sizeof(double));
+            if (!d->mue.acc[k]) return -2;
+        }
     }
     return 0;
+}
+
+/* unequal-time accumulator k: [L][n] slices of the file <-> [L_pad][n_pad] of
+ * the padded engine (the file's slices and bins first, zeros elsewhere) */
+static void ue_copy(const dq_pad *d, const dqmc_params *p0, int k, double *pad, double *file,
+                    int to_pad)
+{
+    const int es = d->p.is_complex ? 2 : 1;
+    const long lp = dqmc_acc_len(&d->p, 1, k), l0 = dqmc_acc_len(p0, 1, k);
+    const long np = lp / d->p.L, n0 = l0 / d->L0;
+    if (to_pad) memset(pad, 0, (size_t)lp * es * sizeof(double));
+    for (int t = 0; t < d->L0; t++) {
+        double *a = pad + (size_t)np * t * es, *b = file + (size_t)n0 * t * es;
+        if (to_pad)
+            memcpy(a, b, (size_t)n0 * es * sizeof(double));
+        else
+            memcpy(b, a, (size_t)n0 * es * sizeof(double));
+    }
 }
 
 void dq_pad_in(const dq_sim *s, dq_pad *d)
@@ -143,6 +168,11 @@ void dq_pad_in(const dq_sim *s, dq_pad *d)
         memset(d->meq.acc[k], 0, (size_t)lp * es * sizeof(double));
         memcpy(d->meq.acc[k], s->meq.acc[k], (size_t)l0 * es * sizeof(double));
     }
+    d->mue.n_sample = s->mue.n_sample;
+    d->mue.sign[0] = s->mue.sign[0];
+    d->mue.sign[1] = s->mue.sign[1];
+    for (int k = 0; k < DQ_UE_COUNT; k++)
+        if (d->mue.acc[k]) ue_copy(d, &s->p, k, d->mue.acc[k], s->mue.acc[k], 1);
 }
 
 void dq_pad_out(const dq_pad *d, dq_sim *s)
@@ -159,6 +189,12 @@ void dq_pad_out(const dq_pad *d, dq_sim *s)
         const long l0 = dqmc_acc_len(&s->p, 0, k);
         memcpy(s->meq.acc[k], d->meq.acc[k], (size_t)l0 * es * sizeof(double));
     }
+    if (s->p.period_uneqlt <= 0) return;
+    s->mue.n_sample = d->mue.n_sample;
+    s->mue.sign[0] = d->mue.sign[0];
+    s->mue.sign[1] = d->mue.sign[1];
+    for (int k = 0; k < DQ_UE_COUNT; k++)
+        if (d->mue.acc[k] && s->mue.acc[k]) ue_copy(d, &s->p, k, d->mue.acc[k], s->mue.acc[k], 0);
 }
 
 void dq_pad_free(dq_pad *d)

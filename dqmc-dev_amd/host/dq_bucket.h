@@ -2,15 +2,17 @@
  *
  * No counterpart in the reference, which runs one file per process
  * (src/main_stack.c).  Files whose N falls in one bucket (N <= 64, 121, 196,
- * 289, 400, 512) and that share n_matmul, period_eqlt, the element type and
- * the default measurement set (no unequal-time, no optional groups) run as
+ * 289, 400, 512) and that share n_matmul, period_eqlt, period_uneqlt, the
+ * element type and the default measurement sets (no optional groups) run as
  * one engine: each file is padded to the group's largest N and L.  The
  * padding sites are inert (identity blocks in every exponential,
  * exp_lambda = 1, del = 0, map entries -1: no measurement bin) and the
  * padding slices are whole identity groups (every file's L is a multiple of
  * n_matmul); the engine skips both (dqmc_tuning.chain_L / chain_N).  Every
  * file keeps its own random stream, recalculation slices and measurement
- * slices, so it runs as it would alone. */
+ * slices, so it runs as it would alone.  Unequal-time accumulators are laid
+ * out [L][bins]: the file's slices and bins are the first ones of the padded
+ * [L_pad][bins_pad] arrays. */
 #ifndef DQ_BUCKET_H
 #define DQ_BUCKET_H
 
@@ -27,8 +29,9 @@ typedef struct dq_pad {
     dqmc_params p;     /* padded parameters; arrays in own storage */
     int32_t *hs;       /* padded HS field [L][N] */
     dqmc_meas meq;     /* padded equal-time accumulators */
+    dqmc_meas mue;     /* padded unequal-time accumulators (period_uneqlt > 0) */
     int N0, L0;        /* the file's own sizes */
-    void *own[48];
+    void *own[96];
     int n_own;
 } dq_pad;
 

@@ -159,7 +159,9 @@ static int fetch(dqmc_engine *e, int c, job *j, dq_pad *pad)
 {
     if (pad) {
         int rc = dqmc_engine_get_state(e, c, j->sim.rng, &j->sim.sweep, pad->hs);
-        if (!rc) rc = dqmc_engine_get_meas(e, c, &pad->meq, NULL);
+        if (!rc)
+            rc = dqmc_engine_get_meas(e, c, &pad->meq,
+                                      j->sim.p.period_uneqlt > 0 ? &pad->mue : NULL);
         if (!rc) dq_pad_out(pad, &j->sim);
         return rc;
     }
@@ -232,7 +234,9 @@ static void *run_group(void *arg)
         if (uneven) {
             dq_pad_in(&j->sim, &pads[c]);
             rc = dqmc_engine_set_state(e, c, j->sim.rng, j->sim.sweep, pads[c].hs);
-            if (!rc) rc = dqmc_engine_set_meas(e, c, &pads[c].meq, NULL);
+            if (!rc)
+                rc = dqmc_engine_set_meas(e, c, &pads[c].meq,
+                                          j->sim.p.period_uneqlt > 0 ? &pads[c].mue : NULL);
             continue;
         }
         rc = dqmc_engine_set_state(e, c, j->sim.rng, j->sim.sweep, j->sim.hs);

@@ -137,7 +137,11 @@ typedef struct dqmc_tuning {
      * Its random stream advances exactly as for a chain of its own size,
      * and it measures at its own params' period_eqlt (which, unlike the
      * other shape fields, may differ between the chains of a padded engine).
-     * Unequal-time measurements and the adjoint need equal sizes.
+     * Unequal-time measurements run on padded chains (period_uneqlt shared
+     * by the chains): the identity slices follow a chain's last slice, so
+     * its blocks G(t,0), G(0,t), G(t,t) for t < chain_L[c] are its own, and
+     * the accumulator slices t >= chain_L[c] hold nothing the caller keeps.
+     * The adjoint and the calc_ue_g hook need equal sizes.
      * No counterpart in the reference, which runs one chain per process. */
     const int *chain_L;
     const int *chain_N;

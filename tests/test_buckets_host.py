@@ -38,8 +38,6 @@ def test_padding_keeps_the_chain_block_and_adds_identity():
 
 def test_padding_refuses_what_needs_equal_sizes():
     with pytest.raises(ValueError):
-        buckets.pad_sim(_sim(period_uneqlt=2), 25, 30, 10, 2, 17)
-    with pytest.raises(ValueError):
         buckets.pad_sim(_sim(meas_bond_corr=1), 25, 30, 10, 2, 17)
     with pytest.raises(ValueError):
         buckets.pad_sim(_sim(), 9, 30, 10, 2, 17)
@@ -50,3 +48,16 @@ def test_padding_keeps_the_chains_equal_time_cadence():
                          U=4.0)[0]
     q = buckets.pad_sim(s, 25, 30, 10, 2, 17)
     assert int(q["params"]["period_eqlt"]) == 5 and int(q["params"]["n_matmul"]) == 10
+
+
+def test_padding_places_unequal_time_accumulators_by_slice_and_bin():
+    s = _sim(period_uneqlt=2)
+    rs = np.random.RandomState(0)
+    for k in buckets._UE_IJ:
+        s["meas_uneqlt"][k] = rs.randn(*np.shape(s["meas_uneqlt"][k]))
+    q = buckets.pad_sim(s, 25, 30, 10, 2, 17)
+    for k in buckets._UE_IJ:
+        a = q["meas_uneqlt"][k].reshape(30, 17)
+        assert np.array_equal(a[:20, :16], s["meas_uneqlt"][k].reshape(20, 16))
+        assert not a[20:, :].any() and not a[:, 16:].any()
+    assert int(q["params"]["period_uneqlt"]) == 2

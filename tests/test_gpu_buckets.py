@@ -98,11 +98,48 @@ def test_partial_last_group():
     pb.close()
 
 
-def test_uneven_sizes_refuse_unequal_time():
+@pytest.mark.parametrize("nflux", [0, 1])
+def test_padded_chains_measure_unequal_time_as_alone(nflux):
+    """Unequal-time measurements on chains of uneven N and L in one engine:
+    fields and RNG bit for bit, the unequal-time accumulators of the chain's
+    own slices and bins to 1e-8 (calc_ue_g's bar; the padded engine groups
+    the slices into blocks of another count, which moves only rounding)."""
+    shapes = [(4, 20, 81), (5, 30, 82), (3, 10, 83)]
+    sims = [gen.create_batch(Nfiles=1, seed=seed, Nx=side, Ny=side, L=L, n_matmul=10,
+                             period_eqlt=5, period_uneqlt=1, U=4.0, mu=-0.3, nflux=nflux,
+                             n_sweep_warm=1, n_sweep_meas=100)[0]
+            for side, L, seed in shapes]
+    sweeps = 3
+    pb = buckets.PaddedBatch(sims, 10)
+    try:
+        pb.run(sweeps)
+        for c, sim in enumerate(sims):
+            print("chain", c, "shape", shapes[c][:2], flush=True)
+            e = engine.Engine([sim])
+            try:
+                e.run(sweeps)
+                st = e.get_state(0)
+                _, ue = e.get_meas(0)
+            finally:
+                e.close()
+            s2 = pb.state(c)
+            assert np.array_equal(s2["rng"], st["rng"]) and np.array_equal(s2["hs"], st["hs"])
+            u2 = pb.meas_uneqlt(c)
+            assert int(u2["n_sample"]) == int(ue["n_sample"]) == sweeps - 1
+            for k in ("gt0", "gt0_u", "gt0_d", "nn", "xx", "zz", "pair_sw"):
+                assert u2[k].shape == ue[k].shape, k
+                assert rel(u2[k], ue[k]) < 1e-8, (c, k)
+    finally:
+        pb.close()
+
+
+def test_padded_chains_share_period_uneqlt():
     a = gen.create_batch(Nfiles=1, seed=3, Nx=4, Ny=4, L=20, n_matmul=10, period_eqlt=10,
                          period_uneqlt=1)[0]
+    b = gen.create_batch(Nfiles=1, seed=4, Nx=3, Ny=3, L=20, n_matmul=10, period_eqlt=10,
+                         period_uneqlt=2)[0]
     with pytest.raises(ValueError):
-        buckets.PaddedBatch([a], 10)
+        buckets.PaddedBatch([a, b], 10)
 
 
 def test_wrap_drift_check_leaves_the_chain_unchanged():

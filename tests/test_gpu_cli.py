@@ -173,6 +173,37 @@ def test_dqmc_stack_pads_uneven_sizes(tmp_path):
                                               "pair_sw"), eq_tol=1e-8)
 
 
+def test_dqmc_stack_pads_uneven_sizes_with_unequal_time(tmp_path):
+    # as above with unequal-time measurements every sweep after warm-up: the
+    # padded engine's identity slices follow each file's last slice, so the
+    # file's G(t,0), G(0,t), G(t,t) are its own; the accumulators of its
+    # slices and bins match the oracle run of the file (calc_ue_g's 1e-8)
+    real = [gen.create_batch(seed=90 + k, Nx=nx, Ny=ny, U=4.0, mu=-0.3, L=L, n_matmul=10,
+                             period_eqlt=5, period_uneqlt=1, n_sweep_warm=1, n_sweep_meas=3)[0]
+            for k, (nx, ny, L) in enumerate([(4, 4, 20), (5, 5, 30), (3, 4, 10)])]
+    cplx = [gen.create_batch(seed=95 + k, Nx=nx, Ny=ny, U=4.0, mu=-0.2, L=L, n_matmul=10,
+                             period_eqlt=5, period_uneqlt=1, nflux=1, n_sweep_warm=1,
+                             n_sweep_meas=3)[0]
+            for k, (nx, ny, L) in enumerate([(4, 4, 20), (5, 3, 30)])]
+    files = []
+    for k, s in enumerate(real + cplx):
+        fn = str(tmp_path / ("w%d.h5" % k))
+        h5.write_sim(fn, s)
+        files.append(fn)
+    stack = str(tmp_path / "stack")
+    with open(stack, "w") as f:
+        f.write("\n".join(files) + "\n")
+    r = _run([os.path.join(BIN, "dqmc_stack"), "-B", "8", stack])
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert r.stdout.count("completed:") == len(files), r.stdout + _logs(files)
+    for fn in files:
+        assert "padded from" in open(fn + ".log").read(), fn
+    for fn, s in zip(files, real + cplx):
+        print("file", fn, "N", int(s["params"]["N"]), "L", int(s["params"]["L"]), flush=True)
+        _check_against_oracle(fn, s, eq_keys=("density", "double_occ", "nn", "zz"), eq_tol=1e-8,
+                              ue_keys=("gt0", "gt0_u", "nn", "xx", "zz", "pair_sw"))
+
+
 def test_dqmc_1_profile_table(tmp_path):
     """-p prints the reference's profile table (src/prof.c:12-50) into the log."""
     sim = gen.create_batch(seed=44, Nx=4, Ny=4, U=4.0, mu=-0.3, L=20, n_matmul=5, period_eqlt=5,
