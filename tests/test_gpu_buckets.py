@@ -186,6 +186,36 @@ def test_n_matmul_autotuner_picks_a_stable_interval():
     assert pick == (min(ok, key=lambda x: x[1])[0] if ok else 5)
 
 
+def test_regrouped_interval_matches_the_oracle_at_that_interval():
+    """A bucket run at an n_matmul other than the files' (the autotuner's
+    choice): each chain is the oracle run of its own file with n_matmul (and
+    period_eqlt, the cadence the reference ties to it) set to the bucket's --
+    the field and RNG bit for bit, G and the accumulators to rounding."""
+    import oracle
+    sims = [_sim(4, 20, 10, 51), _sim(5, 40, 10, 52), _sim(6, 30, 10, 53)]
+    sweeps = 4
+    for nm in (5, 10):
+        pb = buckets.PaddedBatch(sims, nm)
+        try:
+            pb.run(sweeps)
+            for c, sim in enumerate(sims):
+                ref = gen.create_batch(Nfiles=1, seed=51 + c, Nx=[4, 5, 6][c], Ny=[4, 5, 6][c],
+                                       L=[20, 40, 30][c], n_matmul=nm, period_eqlt=10, U=4.0,
+                                       mu=-0.5, tp=0.0, n_sweep_warm=1, n_sweep_meas=sweeps - 1)[0]
+                assert np.array_equal(ref["state"]["hs"], sim["state"]["hs"])
+                out = oracle.run(ref)
+                st = pb.state(c)
+                print("n_matmul", nm, "chain", c, flush=True)
+                assert np.array_equal(st["hs"], out["state"]["hs"])
+                assert [int(v) for v in st["rng"]] == [int(v) for v in out["state"]["rng"]]
+                m = pb.meas(c)
+                assert int(m["n_sample"]) == int(out["meas_eqlt"]["n_sample"])
+                for k in ("density", "double_occ", "nn", "xx", "zz", "pair_sw"):
+                    assert rel(m[k], out["meas_eqlt"][k]) < 1e-9, (nm, c, k)
+        finally:
+            pb.close()
+
+
 def test_delay_rank_autotuner_times_each_compiled_rank():
     sims = [_sim(9, 20, 10, 44), _sim(10, 40, 10, 45)]  # N = 81, 100: ranks 8, 16, 32
     hs0 = [np.array(s["state"]["hs"]) for s in sims]
