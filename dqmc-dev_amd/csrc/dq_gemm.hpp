@@ -55,7 +55,9 @@ struct GemmArgs {
 
 // Tile shapes (rows x columns of C per 256-thread workgroup).  The engine's
 // autotuner times the shapes on its own product sizes and keeps the fastest.
-enum { GT_AUTO = 0, GT_64 = 1, GT_128 = 2, GT_128x64 = 3 };
+// GT_W: the real op-N product on the 16x16x4 form with 128-bit LDS traffic
+// (dq_gemm16.hpp), for the shapes it takes; other products fall back to 64x64.
+enum { GT_AUTO = 0, GT_64 = 1, GT_128 = 2, GT_128x64 = 3, GT_W = 4 };
 
 __device__ inline dbl4 mfma(double a, double b, dbl4 c)
 {
@@ -353,6 +355,8 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgs<T> g, int tiles_m, int ti
             }
 }
 
+#include "dq_gemm16.hpp"
+
 template <typename T, int TM, int TN, int BK, int MF>
 static hipError_t gemm_launch_tile(int opa, int opb, const GemmArgs<T>& g, int batch, hipStream_t s)
 {
@@ -403,6 +407,9 @@ static hipError_t gemm_launch(int opa, int opb, const GemmArgs<T>& g, int batch,
                               int tile)
 {
     constexpr bool Z = is_cplx<T>::value;
+    if constexpr (!Z) {
+        if (tile == GT_W && gemm_w_fits(opa, opb, g)) return gemm_w_launch(g, batch, s);
+    }
     if (tile == GT_128 && g.M > 64)  // 128 x 64 with K steps of 16 (8 complex)
         return gemm_launch_tile<T, 4, 2, Z ? 8 : 16, 1>(opa, opb, g, batch, s);
     if (tile == GT_128x64 && g.M > 64)

@@ -1,0 +1,191 @@
+// dq_gemm16.hpp -- the real N x N product on v_mfma_f64_16x16x4f64 with
+// 128-bit LDS traffic (tile GT_W of dq_gemm.hpp's launcher).
+//
+//   C[b] = alpha * diag(r) * A * diag(u) * B * diag(w) + beta * C,  A, B op N,
+//   M % 128 == 0, N % 64 == 0, K % 16 == 0, 16-byte aligned columns.
+//
+// 256 threads per 128 x 64 tile of C, four waves in a 2 x 2 arrangement, each
+// wave 64 x 32 = 4 x 2 blocks of 16 x 16.  K advances in steps of 16 through
+// two LDS buffers (the next step's global loads in flight during the current
+// step's MFMAs, one barrier per step).
+//
+// K order inside a step: MFMA sub-step s (0..3) gives the lanes of quarter q
+// (lane / 16) the k index 4q + s, for both operands, so a lane needs
+// k = 4q .. 4q + 3 of its row of A and of its column of B.  LDS holds the
+// tiles in that fragment order: per 16-row (16-column) block, two 1 KiB
+// regions, region j holding at slot l the pair k = 4(l/16) + 2j, +1 of row
+// (column) l % 16.  Every LDS access of the kernel is then one 16-byte slot
+// per lane at slot = lane: a wave moves 1 KiB of contiguous LDS, the form that
+// reads at the full 256 bytes per clock (rows 144 bytes apart, conflict-free
+// by banks, took twice as long per read on gfx950).
+//
+// Global -> LDS: B's columns are k-contiguous in memory, so a thread moves
+// k = 4q .. 4q + 3 of one column (two 16-byte loads; u is applied here) into
+// its slot of both regions.  A's columns are m-contiguous: a thread loads its
+// row's four k columns (four 8-byte loads, the 16 lanes of a quarter reading
+// 128 contiguous bytes of each) for two 16-row blocks.
+#pragma once
+#include "dq_common.hpp"
+// included by dq_gemm.hpp after GemmArgs
+
+#define GW_BM 128
+#define GW_BN 64
+#define GW_BK 16
+
+template <bool BETA>
+__global__ __launch_bounds__(256, 3) void k_gemm_w(GemmArgs<double> g, int tiles_m, int tiles_n,
+                                                   int nwg)
+{
+    __shared__ __align__(16) double As[2][GW_BM * GW_BK];
+    __shared__ __align__(16) double Bs[2][GW_BN * GW_BK];
+
+    // Workgroup ids are dealt round-robin over the 8 XCDs; renumber so that
+    // each XCD works through a contiguous range of tiles (one matrix's tiles
+    // share an L2).
+    const int id = blockIdx.x;
+    const int xcd = id & 7, per = nwg >> 3, rem = nwg & 7;
+    const int lid = xcd * per + (xcd < rem ? xcd : rem) + (id >> 3);
+    const int tm = lid % tiles_m;
+    const int rest = lid / tiles_m;
+    const int tn = rest % tiles_n;
+    const int b = rest / tiles_n;
+    const int m0 = tm * GW_BM, n0 = tn * GW_BN;
+    const double* A = g.A + (long)(b / g.adiv) * g.sA;
+    const double* B = g.B + (long)(b / g.bdiv) * g.sB;
+    double* C = g.C + (long)b * g.sC;
+    const double* u = g.u ? g.u + (long)b * g.su : nullptr;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    if (g.idLc && g.idl >= g.idLc[b >> 1]) {  // identity factor (uniform)
+        const double* X = (g.idside == 1) ? B : A;
+        const int ldx = (g.idside == 1) ? g.ldb : g.lda;
+        for (int e = tid; e < GW_BM * GW_BN; e += 256) {
+            const int row = m0 + e % GW_BM, col = n0 + e / GW_BM;
+            double v = g.alpha * X[row + (long)ldx * col];
+            if constexpr (BETA) v += g.beta * C[row + (long)g.ldc * col];
+            C[row + (long)g.ldc * col] = v;
+        }
+        return;
+    }
+    const int q = lane >> 4, l16 = lane & 15;
+
+    // A loads: rows 16 (2 wave + t) + l16, k = 4q .. 4q + 3
+    const double* Ag = A + (m0 + 32 * wave + l16) + (long)g.lda * (4 * q);
+    // B loads: column 16 wave + l16, k = 4q .. 4q + 3
+    const double* Bg = B + 4 * q + (long)g.ldb * (n0 + 16 * wave + l16);
+
+    double ra[2][4];
+    double2 rb[2];
+    double2 ub0 = make_double2(1.0, 1.0), ub1 = ub0;
+    auto gload = [&](int k0) {
+#pragma unroll
+        for (int t = 0; t < 2; t++)
+#pragma unroll
+            for (int c = 0; c < 4; c++) ra[t][c] = Ag[16 * t + (long)g.lda * (k0 + c)];
+        const double* bb = Bg + k0;
+        rb[0] = *(const double2*)(bb);
+        rb[1] = *(const double2*)(bb + 2);
+        if (u) {
+            ub0 = *(const double2*)(u + k0 + 4 * q);
+            ub1 = *(const double2*)(u + k0 + 4 * q + 2);
+        }
+    };
+    // slot `lane` of region j of block blk: double offset ((2 blk + j) 64 + lane) 2
+    auto slot = [&](int blk, int j) { return ((2 * blk + j) * 64 + lane) * 2; };
+    auto sstore = [&](int buf) {
+#pragma unroll
+        for (int t = 0; t < 2; t++) {
+            *(double2*)(&As[buf][slot(2 * wave + t, 0)]) = make_double2(ra[t][0], ra[t][1]);
+            *(double2*)(&As[buf][slot(2 * wave + t, 1)]) = make_double2(ra[t][2], ra[t][3]);
+        }
+        *(double2*)(&Bs[buf][slot(wave, 0)]) = make_double2(rb[0].x * ub0.x, rb[0].y * ub0.y);
+        *(double2*)(&Bs[buf][slot(wave, 1)]) = make_double2(rb[1].x * ub1.x, rb[1].y * ub1.y);
+    };
+
+    const int wr = wave >> 1, wc = wave & 1;
+    const int wm = 64 * wr, wn = 32 * wc;
+    dbl4 acc[4][2];
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++) acc[i][j] = (dbl4){0.0, 0.0, 0.0, 0.0};
+    auto compute = [&](int buf) {
+        double2 av[4][2], bv[2][2];
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            av[i][0] = *(const double2*)(&As[buf][slot(4 * wr + i, 0)]);
+            av[i][1] = *(const double2*)(&As[buf][slot(4 * wr + i, 1)]);
+        }
+#pragma unroll
+        for (int j = 0; j < 2; j++) {
+            bv[j][0] = *(const double2*)(&Bs[buf][slot(2 * wc + j, 0)]);
+            bv[j][1] = *(const double2*)(&Bs[buf][slot(2 * wc + j, 1)]);
+        }
+#pragma unroll
+        for (int s = 0; s < 4; s++) {
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const double a = (s & 1) ? av[i][s >> 1].y : av[i][s >> 1].x;
+#pragma unroll
+                for (int j = 0; j < 2; j++) {
+                    const double bq = (s & 1) ? bv[j][s >> 1].y : bv[j][s >> 1].x;
+                    acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(bq, a, acc[i][j], 0, 0, 0);
+                }
+            }
+        }
+    };
+
+    const int nk = g.K / GW_BK;
+    gload(0);
+    sstore(0);
+    __syncthreads();
+    for (int kt = 0; kt < nk; kt++) {
+        const int cur = kt & 1;
+        if (kt + 1 < nk) gload((kt + 1) * GW_BK);
+        compute(cur);
+        if (kt + 1 < nk) sstore(cur ^ 1);
+        __syncthreads();
+    }
+
+    const double* rr = g.r ? g.r + (long)b * g.sr : nullptr;
+    const double* ww = g.w ? g.w + (long)b * g.sw : nullptr;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        const int row = m0 + wm + 16 * i + l16;
+        const double rs = rr ? rr[row] : 1.0;
+#pragma unroll
+        for (int j = 0; j < 2; j++)
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                const int col = n0 + wn + 16 * j + 4 * e + q;
+                double v = g.alpha * acc[i][j][e];
+                if (rr) v = v * rs;
+                if (ww) v = v * ww[col];
+                const long o = row + (long)g.ldc * col;
+                if constexpr (BETA) v += g.beta * C[o];
+                C[o] = v;
+            }
+    }
+}
+
+// The shapes and alignments k_gemm_w takes
+static inline bool gemm_w_fits(int opa, int opb, const GemmArgs<double>& g)
+{
+    const auto al = [](const void* p) { return ((unsigned long)p & 15) == 0; };
+    return opa == 0 && opb == 0 && g.M % GW_BM == 0 && g.N % GW_BN == 0 && g.K % GW_BK == 0 &&
+           g.M > 0 && g.N > 0 && g.K > 0 && g.lda % 2 == 0 && g.ldb % 2 == 0 && g.sA % 2 == 0 &&
+           g.sB % 2 == 0 && al(g.A) && al(g.B) && (!g.u || (al(g.u) && g.su % 2 == 0));
+}
+
+static inline hipError_t gemm_w_launch(const GemmArgs<double>& g, int batch, hipStream_t s)
+{
+    const int tm = g.M / GW_BM, tn = g.N / GW_BN;
+    const long nwg = (long)tm * tn * batch;
+    if (nwg <= 0) return hipSuccess;
+    if (nwg > 0x7fffffffL) return hipErrorInvalidValue;
+    const int n = (int)nwg;
+    if (g.beta_zero)
+        k_gemm_w<false><<<n, 256, 0, s>>>(g, tm, tn, n);
+    else
+        k_gemm_w<true><<<n, 256, 0, s>>>(g, tm, tn, n);
+    return hipGetLastError();
+}

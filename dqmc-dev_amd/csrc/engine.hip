@@ -1470,8 +1470,8 @@ static int autotune_gemm(dqmc_engine* e)
 {
     const int N = e->N, B = 2 * e->nc;
     const long NN = (long)N * N;
-    const int cand[3] = {GT_64, GT_128x64, GT_128};
-    const int ncand = is_cplx<T>::value ? 2 : 3;
+    const int cand[4] = {GT_64, GT_128x64, GT_128, GT_W};
+    const int ncand = is_cplx<T>::value ? 2 : (N % 128 == 0 ? 4 : 3);
     hipEvent_t a0, a1;
     HIPCHK(hipEventCreate(&a0));
     HIPCHK(hipEventCreate(&a1));
@@ -1771,7 +1771,7 @@ extern "C" int dqmc_engine_create(dqmc_engine** out, const dqmc_params* const* P
     int rc = engine_setup(e, P);
     if (!rc) {
         const int t = tune ? tune->gemm_tile : 0;
-        if (t < 0 || t > GT_128x64) rc = set_err(DQMC_ERR_ARG, "bad gemm_tile %d", t);
+        if (t < 0 || t > GT_W) rc = set_err(DQMC_ERR_ARG, "bad gemm_tile %d", t);
         else if (t > 0) e->gemm_tile = t;
         else rc = e->cplx ? autotune_gemm<zd>(e) : autotune_gemm<double>(e);
     }
@@ -2168,7 +2168,7 @@ extern "C" int dqmc_test_gemm(int is_complex, int opa, int opb, int M, int N, in
                               const double* u, const double* w, int tile)
 {
     if (M <= 0 || N <= 0 || K <= 0) return set_err(DQMC_ERR_ARG, "bad GEMM shape");
-    if (tile < 0 || tile > GT_128x64) return set_err(DQMC_ERR_ARG, "bad tile %d", tile);
+    if (tile < 0 || tile > GT_W) return set_err(DQMC_ERR_ARG, "bad tile %d", tile);
     return is_complex
                ? test_gemm_t<zd>(opa, opb, M, N, K, alpha, A, lda, B, ldb, beta, Cm, ldc, r, u, w, tile)
                : test_gemm_t<double>(opa, opb, M, N, K, alpha, A, lda, B, ldb, beta, Cm, ldc, r, u, w,

@@ -126,7 +126,10 @@ typedef struct dqmc_tuning {
                        launch) */
     int gemm_tile;  /* C tile of the MFMA GEMM: 0 = time the shapes at creation
                        and keep the fastest, 1 = 64x64, 2 = 128x64 with K
-                       steps of 16, 3 = 128x64 with K steps of 8 */
+                       steps of 16, 3 = 128x64 with K steps of 8, 4 = 128x64
+                       on the 16x16x4 form with 128-bit LDS reads for the real
+                       products whose sizes are multiples of 128 x 64 x 16
+                       (others take 64x64) */
     /* Chains of uneven size in one engine (both NULL: every chain has the
      * params' N and L).  chain_L[c] <= L and chain_N[c] <= N are chain c's own
      * slice and site counts; its params are padded to the common N and L by
@@ -283,7 +286,7 @@ int dqmc_engine_set_params(dqmc_engine *e, int chain, const dqmc_params *params)
  * previous call (which resets it).  Used to pick the stabilization interval. */
 int dqmc_engine_drift(dqmc_engine *e, double *max_abs);
 
-/* The GEMM tile shape the engine uses (dqmc_tuning.gemm_tile values 1..3). */
+/* The GEMM tile shape the engine uses (dqmc_tuning.gemm_tile values 1..4). */
 int dqmc_engine_gemm_tile(dqmc_engine *e);
 
 /* The delay rank the engine's Metropolis kernel runs at (n_delay of
