@@ -26,10 +26,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "dqmc-dev_amd"))
 
 PEAK_FP64_TFLOPS = 78.6  # MI355X dense FP64 (matrix = vector)
-# HBM bytes per 256 x 256 x 256 matrix product of the dominant GEMM, from the
-# FETCH_SIZE / WRITE_SIZE counter passes scaled by a 512 MiB copy
-# (profiles/k_gemm_N256_L80_256chains.md): 1.0007 x the 3 x 512 KiB it must move.
-GEMM_HBM_BYTES_PER_MATRIX_N256 = 1573916
+# HBM bytes per 256 x 256 x 256 matrix product of the dominant GEMM (k_gemm_w),
+# from the FETCH_SIZE / WRITE_SIZE counter passes scaled by a 512 MiB copy
+# (profiles/k_gemm_N256_L80_256chains.md): 1.0010 x the 3 x 512 KiB it must move.
+GEMM_HBM_BYTES_PER_MATRIX_N256 = 1574365
 
 
 def workload(nchains, seed=7):

@@ -108,28 +108,24 @@ __global__ __launch_bounds__(256, 3) void k_gemm_w(GemmArgs<double> g, int tiles
     for (int i = 0; i < 4; i++)
 #pragma unroll
         for (int j = 0; j < 2; j++) acc[i][j] = (dbl4){0.0, 0.0, 0.0, 0.0};
-    auto compute = [&](int buf) {
-        double2 av[4][2], bv[2][2];
+    double2 av[4][2], bv[2][2];
+    auto reads = [&](int buf) {
+#pragma unroll
+        for (int jj = 0; jj < 2; jj++) {
+#pragma unroll
+            for (int i = 0; i < 4; i++) av[i][jj] = *(const double2*)(&As[buf][slot(4 * wr + i, jj)]);
+#pragma unroll
+            for (int j = 0; j < 2; j++) bv[j][jj] = *(const double2*)(&Bs[buf][slot(2 * wc + j, jj)]);
+        }
+    };
+    auto mf = [&](int s) {  // MFMA sub-step s: k = 4q + s
 #pragma unroll
         for (int i = 0; i < 4; i++) {
-            av[i][0] = *(const double2*)(&As[buf][slot(4 * wr + i, 0)]);
-            av[i][1] = *(const double2*)(&As[buf][slot(4 * wr + i, 1)]);
-        }
+            const double a = (s & 1) ? av[i][s >> 1].y : av[i][s >> 1].x;
 #pragma unroll
-        for (int j = 0; j < 2; j++) {
-            bv[j][0] = *(const double2*)(&Bs[buf][slot(2 * wc + j, 0)]);
-            bv[j][1] = *(const double2*)(&Bs[buf][slot(2 * wc + j, 1)]);
-        }
-#pragma unroll
-        for (int s = 0; s < 4; s++) {
-#pragma unroll
-            for (int i = 0; i < 4; i++) {
-                const double a = (s & 1) ? av[i][s >> 1].y : av[i][s >> 1].x;
-#pragma unroll
-                for (int j = 0; j < 2; j++) {
-                    const double bq = (s & 1) ? bv[j][s >> 1].y : bv[j][s >> 1].x;
-                    acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(bq, a, acc[i][j], 0, 0, 0);
-                }
+            for (int j = 0; j < 2; j++) {
+                const double bq = (s & 1) ? bv[j][s >> 1].y : bv[j][s >> 1].x;
+                acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(bq, a, acc[i][j], 0, 0, 0);
             }
         }
     };
@@ -140,9 +136,16 @@ __global__ __launch_bounds__(256, 3) void k_gemm_w(GemmArgs<double> g, int tiles
     __syncthreads();
     for (int kt = 0; kt < nk; kt++) {
         const int cur = kt & 1;
-        if (kt + 1 < nk) gload((kt + 1) * GW_BK);
-        compute(cur);
-        if (kt + 1 < nk) sstore(cur ^ 1);
+        const bool more = kt + 1 < nk;
+        if (more) gload((kt + 1) * GW_BK);
+        reads(cur);
+        mf(0);
+        mf(1);
+        // the next step's LDS writes between the MFMA halves (2% fewer cycles
+        // than after them: the loads have landed and the LDS is idle)
+        if (more) sstore(cur ^ 1);
+        mf(2);
+        mf(3);
         __syncthreads();
     }
 
