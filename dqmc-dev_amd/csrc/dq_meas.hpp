@@ -267,7 +267,30 @@ __global__ __launch_bounds__(256) void k_meas_ue_strip(MeasArgs<T> a)
         if (a.energy) { vv = acc[off[UE_vv] + o]; vn = acc[off[UE_vn] + o]; }
     }
     const double delta_t = (tt == 0) ? 1.0 : 0.0;
-    for (int j0 = 0; j0 < N; j0 += CH) {
+    // this thread's rows i of the strip's columns j and their G(t,0)[i, j],
+    // fetched one strip ahead into registers so that the loads of a strip
+    // are all in flight at once instead of one dependent chain per column
+    int iv[2][CH];
+    T cu[2][CH], cd[2][CH];
+    auto fetch = [&](int j0, auto hc) {
+        constexpr int h = decltype(hc)::value;
+#pragma unroll
+        for (int jj = 0; jj < CH; jj++) {
+            const int j = j0 + jj;
+            iv[h][jj] = (r < nij && j < N) ? inv[(long)j * nij + r] : 0;
+        }
+#pragma unroll
+        for (int jj = 0; jj < CH; jj++) {
+            const int j = j0 + jj;
+            const bool ok = r < nij && j < N;
+            cu[h][jj] = ok ? Gt0[0][iv[h][jj] + (long)N * j] : zero;
+            cd[h][jj] = ok ? Gt0[1][iv[h][jj] + (long)N * j] : zero;
+        }
+    };
+    fetch(0, std::integral_constant<int, 0>());
+    // strips in pairs, so that the register set of a strip is a constant
+    auto strip_step = [&](int j0, auto hc) {
+        constexpr int h = decltype(hc)::value;
         const int ch = (N - j0 < CH) ? N - j0 : CH;
         __syncthreads();
         for (int e = tid; e < CH * N; e += nt) {
@@ -280,14 +303,17 @@ __global__ __launch_bounds__(256) void k_meas_ue_strip(MeasArgs<T> a)
                 }
             }
         }
+        if (j0 + CH < N) fetch(j0 + CH, std::integral_constant<int, 1 - h>());
         __syncthreads();
         if (r < nij) {
-            for (int jj = 0; jj < ch; jj++) {
+#pragma unroll
+            for (int jj = 0; jj < CH; jj++) {
+                if (jj >= ch) continue;
                 const int j = j0 + jj;
-                const int i = inv[(long)j * nij + r];
+                const int i = iv[h][jj];
                 const double dtij = delta_t * ((i == j) ? 1.0 : 0.0);
                 const T guii = dtt[0][i], gdii = dtt[1][i], gujj = doo[0][j], gdjj = doo[1][j];
-                const T guij = Gt0[0][i + (long)N * j], gdij = Gt0[1][i + (long)N * j];
+                const T guij = cu[h][jj], gdij = cd[h][jj];
                 const T guji = strip[0][jj][i], gdji = strip[1][jj][i];
                 T pgu = guij, pgd = gdij;
                 if constexpr (Z) {   // g * p, the product order of peierls_mul
@@ -310,12 +336,161 @@ __global__ __launch_bounds__(256) void k_meas_ue_strip(MeasArgs<T> a)
                 }
             }
         }
+    };
+    for (int j0 = 0; j0 < N; j0 += 2 * CH) {
+        strip_step(j0, std::integral_constant<int, 0>());
+        if (j0 + CH < N) strip_step(j0 + CH, std::integral_constant<int, 1>());
     }
     if (r < nij) {
         acc[off[UE_gt0] + o] = gt0; acc[off[UE_gt0_u] + o] = gt0u; acc[off[UE_gt0_d] + o] = gt0d;
         acc[off[UE_nn] + o] = nn; acc[off[UE_xx] + o] = xx; acc[off[UE_zz] + o] = zz;
         acc[off[UE_pair_sw] + o] = psw;
         if (a.energy) { acc[off[UE_vv] + o] = vv; acc[off[UE_vn] + o] = vn; }
+    }
+}
+
+// Equal time (src/meas.c:66-120) for maps in which every column j adds exactly
+// one term to every bin (translation-symmetric maps, num_ij = N): the
+// unequal-time strip contraction at t = 0 (G(0,0) = G(t,t) = G(t,0) = G(0,t),
+// delta_t = 1).  One workgroup per chain, one thread per bin; rows of G are
+// staged through LDS in strips of CH with line-coalesced reads, where the CSR
+// form reads row j of the column-major G one cache line per element.  Each
+// bin adds its terms in increasing j, as the CSR form (and the reference)
+// does, so the sums are the same to the bit.
+template <typename T, int CH>
+__global__ __launch_bounds__(256) void k_meas_eq_strip(MeasArgs<T> a)
+{
+    constexpr bool Z = is_cplx<T>::value;
+    __shared__ T strip[2][CH][UES_NMAX];                            // G_s[j0 + jj, i]
+    __shared__ T pstrip[Z ? 2 : 1][Z ? CH : 1][Z ? UES_NMAX : 1];  // P_s[j0 + jj + N i]
+    __shared__ T dg[2][UES_NMAX];
+    const int c = blockIdx.x;
+    if (!a.flag[c]) return;
+    const int N = a.N, nij = a.num_ij;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    const T phase = a.phase[c];
+    if (tid == 0) {
+        a.n_sample[c] += 1;
+        a.sign[c] += phase;
+    }
+    const T* G[2] = {a.G + (2 * c) * a.sG, a.G + (2 * c + 1) * a.sG};
+    const T* P[2] = {a.peierlsu + c * a.sP, a.peierlsd + c * a.sP};
+    for (int i = tid; i < N; i += nt) {
+#pragma unroll
+        for (int s = 0; s < 2; s++) dg[s][i] = G[s][i + (long)N * i];
+    }
+    __syncthreads();
+    T* acc = a.acc + c * a.sacc;
+    const long* off = a.off;
+    const T zero = from_re<T>(0.0), one = from_re<T>(1.0), two = from_re<T>(2.0);
+    const int r = tid;
+    // one-site bins (their CSR lists, diagonals from LDS)
+    if (r < a.num_i) {
+        const int* rp = a.rp_i + c * a.srp_i;
+        const int* ix = a.ix_i + c * a.six_i;
+        const T pre = phase * (1.0 / a.degen[4 * c]);
+        T s0 = acc[off[EQ_density] + r], s1 = acc[off[EQ_density_u] + r];
+        T s2 = acc[off[EQ_density_d] + r], s3 = acc[off[EQ_double_occ] + r];
+        for (int q = rp[r]; q < rp[r + 1]; q++) {
+            const int i = ix[q];
+            const T guii = dg[0][i], gdii = dg[1][i];
+            s0 += pre * (two - guii - gdii);
+            s1 += pre * (one - guii);
+            s2 += pre * (one - gdii);
+            s3 += pre * (one - guii) * (one - gdii);
+        }
+        acc[off[EQ_density] + r] = s0;
+        acc[off[EQ_density_u] + r] = s1;
+        acc[off[EQ_density_d] + r] = s2;
+        acc[off[EQ_double_occ] + r] = s3;
+    }
+    const int* inv = a.invij + c * a.sinv;
+    const T pre = phase * (1.0 / a.degen[4 * c + 1]);
+    T g00 = zero, g00u = zero, g00d = zero, nn = zero, xx = zero, zz = zero, psw = zero;
+    T vv = zero, vn = zero;
+    if (r < nij) {
+        g00 = acc[off[EQ_g00] + r]; g00u = acc[off[EQ_g00_u] + r]; g00d = acc[off[EQ_g00_d] + r];
+        nn = acc[off[EQ_nn] + r]; xx = acc[off[EQ_xx] + r]; zz = acc[off[EQ_zz] + r];
+        psw = acc[off[EQ_pair_sw] + r];
+        if (a.energy) { vv = acc[off[EQ_vv] + r]; vn = acc[off[EQ_vn] + r]; }
+    }
+    // as in k_meas_ue_strip: the strip's columns fetched one strip ahead
+    int iv[2][CH];
+    T cu[2][CH], cd[2][CH];
+    auto fetch = [&](int j0, auto hc) {
+        constexpr int h = decltype(hc)::value;
+#pragma unroll
+        for (int jj = 0; jj < CH; jj++) {
+            const int j = j0 + jj;
+            iv[h][jj] = (r < nij && j < N) ? inv[(long)j * nij + r] : 0;
+        }
+#pragma unroll
+        for (int jj = 0; jj < CH; jj++) {
+            const int j = j0 + jj;
+            const bool ok = r < nij && j < N;
+            cu[h][jj] = ok ? G[0][iv[h][jj] + (long)N * j] : zero;
+            cd[h][jj] = ok ? G[1][iv[h][jj] + (long)N * j] : zero;
+        }
+    };
+    fetch(0, std::integral_constant<int, 0>());
+    // strips in pairs, so that the register set of a strip is a constant
+    auto strip_step = [&](int j0, auto hc) {
+        constexpr int h = decltype(hc)::value;
+        const int ch = (N - j0 < CH) ? N - j0 : CH;
+        __syncthreads();
+        for (int e = tid; e < CH * N; e += nt) {
+            const int jj = e % CH, i = e / CH;
+            if (jj < ch) {
+#pragma unroll
+                for (int s = 0; s < 2; s++) {
+                    strip[s][jj][i] = G[s][(j0 + jj) + (long)N * i];
+                    if constexpr (Z) pstrip[s][jj][i] = P[s][(j0 + jj) + (long)N * i];
+                }
+            }
+        }
+        if (j0 + CH < N) fetch(j0 + CH, std::integral_constant<int, 1 - h>());
+        __syncthreads();
+        if (r < nij) {
+#pragma unroll
+            for (int jj = 0; jj < CH; jj++) {
+                if (jj >= ch) continue;
+                const int j = j0 + jj;
+                const int i = iv[h][jj];
+                const double delta = (i == j) ? 1.0 : 0.0;
+                const T guii = dg[0][i], gdii = dg[1][i], gujj = dg[0][j], gdjj = dg[1][j];
+                const T guij = cu[h][jj], gdij = cd[h][jj];
+                const T guji = strip[0][jj][i], gdji = strip[1][jj][i];
+                T pgu = guij, pgd = gdij;
+                if constexpr (Z) {   // g * p, the product order of peierls_mul
+                    pgu = guij * pstrip[0][jj][i];
+                    pgd = gdij * pstrip[1][jj][i];
+                }
+                g00 += 0.5 * (pre * (pgu + pgd));
+                g00u += pre * pgu;
+                g00d += pre * pgd;
+                const T x = delta * (guii + gdii) - (guji * guij + gdji * gdij);
+                nn += pre * ((two - guii - gdii) * (two - gujj - gdjj) + x);
+                xx += 0.25 * (pre * (delta * (guii + gdii) - (guji * gdij + gdji * guij)));
+                zz += 0.25 * (pre * ((gdii - guii) * (gdjj - gujj) + x));
+                psw += pre * guij * gdij;
+                if (a.energy) {
+                    const T nuinuj = (one - guii) * (one - gujj) + (delta - guji) * guij;
+                    const T ndindj = (one - gdii) * (one - gdjj) + (delta - gdji) * gdij;
+                    vv += pre * nuinuj * ndindj;
+                    vn += pre * (nuinuj * (one - gdii) + (one - guii) * ndindj);
+                }
+            }
+        }
+    };
+    for (int j0 = 0; j0 < N; j0 += 2 * CH) {
+        strip_step(j0, std::integral_constant<int, 0>());
+        if (j0 + CH < N) strip_step(j0 + CH, std::integral_constant<int, 1>());
+    }
+    if (r < nij) {
+        acc[off[EQ_g00] + r] = g00; acc[off[EQ_g00_u] + r] = g00u; acc[off[EQ_g00_d] + r] = g00d;
+        acc[off[EQ_nn] + r] = nn; acc[off[EQ_xx] + r] = xx; acc[off[EQ_zz] + r] = zz;
+        acc[off[EQ_pair_sw] + r] = psw;
+        if (a.energy) { acc[off[EQ_vv] + r] = vv; acc[off[EQ_vn] + r] = vn; }
     }
 }
 

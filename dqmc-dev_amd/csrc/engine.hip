@@ -1079,8 +1079,12 @@ static int sweep_body(dqmc_engine* e, bool any_eq, bool any_ue)
                 LCHK();
                 ma.flag = e->flag_sl;
             }
-            const int nb = (std::max(e->num_i, e->num_ij) + 255) / 256;
-            k_meas_eq<T><<<dim3(nb, nc), 256, 0, e->st>>>(ma);
+            if (e->ij_percol && e->num_i <= 256) {
+                k_meas_eq_strip<T, is_cplx<T>::value ? 8 : 16><<<nc, 256, 0, e->st>>>(ma);
+            } else {
+                const int nb = (std::max(e->num_i, e->num_ij) + 255) / 256;
+                k_meas_eq<T><<<dim3(nb, nc), 256, 0, e->st>>>(ma);
+            }
             LCHK();
             if (e->energy) {
                 const int nb2 = (std::max(e->num_bs, e->num_bb) + 255) / 256;
@@ -1351,7 +1355,7 @@ static int engine_setup(dqmc_engine* e, const dqmc_params* const* P)
     DALLOC(e->degen2, (long)nc * PM_COUNT * 4);
     // does every column j of map_ij feed every bin exactly once (translation
     // symmetry)?  Then the strip kernel can walk columns with an inverse table.
-    e->ij_percol = e->period_uneqlt > 0 && e->num_ij == N && N <= UES_NMAX;
+    e->ij_percol = e->num_ij == N && N <= UES_NMAX;
     for (int c = 0; c < nc && e->ij_percol; c++) {
         std::vector<char> seen(N);
         for (int j = 0; j < N && e->ij_percol; j++) {
