@@ -16,156 +16,190 @@
 #include "dq_common.hpp"
 #include "dq_factor.hpp"
 
-#define QRB_THREADS 1024
-#define QRB_WAVES 16
 
-// QM: row slots per lane (panel height <= 64*QM); NCW: columns per wave
-// (panel width nb = 16*NCW).
-template <typename T, int QM, int NCW>
-__global__ __launch_bounds__(QRB_THREADS) void k_qr_panel(int m, int k0, int nbp, T* A_, int lda,
+// QM: row slots per lane (panel height <= 64*QM); NW waves of NCW columns
+// (panel width nb = NW*NCW).  Panel column jc belongs to wave jc % NW,
+// register slot jc / NW.  8 waves where the registers allow it: two
+// workgroups then share a CU (at most 128 VGPRs each), and the latency of one
+// workgroup's steps hides behind the other's.
+//
+// One workgroup barrier per Householder step j.  Between two barriers:
+//   (b) every wave applies H_j^H to its columns right of j and takes the dots
+//       V(:,jc)^H v_j of its columns left of j (the compact-WY recurrence);
+//       v_j is zero outside rows j .. mp-1, so the row loops run unmasked;
+//   (a) the owner of column j + 1, right after its own update of that column,
+//       builds reflector j + 1 into the other v buffer;
+//   (c) the owner of column j - 1 finishes T(0:j-1, j-1) from the dots of
+//       step j - 1 (double-buffered), off the critical path.
+// f(integral_constant<int, i>) for i = 0 .. K-1, unrolled at compile time
+template <int K, int I = 0, typename F>
+__device__ __forceinline__ void static_for(F&& f)
+{
+    if constexpr (I < K) {
+        f(std::integral_constant<int, I>());
+        static_for<K, I + 1>(f);
+    }
+}
+
+template <typename T, int QM, int NCW, int NW>
+__global__ __launch_bounds__(64 * NW, 4) void k_qr_panel(int m, int k0, int nbp, T* A_, int lda,
                                                           long sA, T* tau_, long stau, T* V_,
                                                           int ldv, long sV, T* Tw_, long sT,
                                                           T* det_, int det_init)
 {
-    constexpr int NB = 16 * NCW;
-    __shared__ T v[64 * QM];
+    constexpr int NB = NW * NCW, NT = 64 * NW;
+    constexpr int NP = val_parts<T>::n;
+    __shared__ T v[2][64 * QM];
     __shared__ T Ts[NB * NB];
-    __shared__ T z[NB];
-    __shared__ T s_tau;
+    __shared__ T z[2][NB];
     __shared__ T s_det;
     const int b = blockIdx.x;
     T* A = A_ + b * sA + k0 + (long)lda * k0;  // panel origin
     const int mp = m - k0;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const T zero = from_re<T>(0.0), one = from_re<T>(1.0);
     T x[QM][NCW];
 #pragma unroll
     for (int cs = 0; cs < NCW; cs++) {
-        const int jc = wave + 16 * cs;
+        const int jc = wave + NW * cs;
 #pragma unroll
         for (int q = 0; q < QM; q++) {
             const int r = lane + 64 * q;
-            x[q][cs] = (jc < nbp && r < mp) ? A[r + (long)lda * jc] : from_re<T>(0.0);
+            x[q][cs] = (jc < nbp && r < mp) ? A[r + (long)lda * jc] : zero;
         }
     }
-    for (int e = tid; e < NB * NB; e += QRB_THREADS) Ts[e] = from_re<T>(0.0);
-    if (tid == 0) s_det = from_re<T>(1.0);
+    for (int e = tid; e < NB * NB; e += NT) Ts[e] = zero;
+    if (tid == 0) s_det = one;
     __syncthreads();
 
+    // (a) reflector of panel column jn (register slot cs of its owner wave):
+    // v into buffer jn & 1 (zero outside rows jn .. mp-1), tau on the
+    // diagonal of T, det(H) = 1 - tau v^H v with v^H v = 1 + |scal|^2 |x1|^2
+    auto reflector = [&](auto csc, int jn) __attribute__((always_inline)) {
+        constexpr int cs = decltype(csc)::value;
+        T alpha = zero;
+        double xn2 = 0.0;
 #pragma unroll
-    for (int cs = 0; cs < NCW; cs++) {
-        for (int w0 = 0; w0 < 16; w0++) {
-            const int j = w0 + 16 * cs;
-            if (j >= nbp) break;
-            // (a) reflector of panel column j (owner wave w0, register column cs)
-            if (wave == w0) {
-                T alpha = from_re<T>(0.0);
-                double xn2 = 0.0;
+        for (int q = 0; q < QM; q++) {
+            const int r = lane + 64 * q;
+            if (r == jn) alpha = x[q][cs];
+            if (r > jn && r < mp) xn2 += abs2(x[q][cs]);
+        }
+        xn2 = wave_sum_r(xn2, lane);
+        alpha = shfl(alpha, jn & 63);
+        T beta, t, scal;
+        larfg(alpha, xn2, &beta, &t, &scal);
+        T* vb = v[jn & 1];
 #pragma unroll
-                for (int q = 0; q < QM; q++) {
-                    const int r = lane + 64 * q;
-                    if (r == j) alpha = x[q][cs];
-                    if (r > j && r < mp) xn2 += abs2(x[q][cs]);
-                }
-                xn2 = wave_sum_r(xn2, lane);
-                alpha = shfl(alpha, j & 63);
-                T beta, t, scal;
-                larfg(alpha, xn2, &beta, &t, &scal);
+        for (int q = 0; q < QM; q++) {
+            const int r = lane + 64 * q;
+            const bool below = r > jn && r < mp;
+            // component-wise selects: a conditional over whole complex
+            // values would be lowered through scratch memory
+            x[q][cs] = selv(below, x[q][cs] * scal, selv(r == jn, beta, x[q][cs]));
+            vb[r] = selv(below, x[q][cs], selv(r == jn, one, zero));
+        }
+        if (lane == 0) {
+            Ts[jn + NB * jn] = t;
+            const double vv = 1.0 + abs2(scal) * xn2;
+            s_det = s_det * (one - t * vv);
+        }
+    };
+    // (c) T(0:jp, jp) = -tau_jp T(0:jp, 0:jp) z(0:jp), z = V(:,0:jp)^H v_jp.
+    // T is upper triangular (zeros below the diagonal), so the sum may start
+    // at 0; groups of 8 independent LDS reads are issued together.
+    auto t_column = [&](int jp) __attribute__((always_inline)) {
+        if (wave == jp % NW && lane < jp) {
+            const T* zz = z[jp & 1];
+            T s = zero;
+            for (int a0 = 0; a0 < jp; a0 += 8) {
 #pragma unroll
-                for (int q = 0; q < QM; q++) {
-                    const int r = lane + 64 * q;
-                    if (r > j && r < mp) x[q][cs] = x[q][cs] * scal;
-                    if (r == j) x[q][cs] = beta;
-                    if (r < mp) v[r] = (r > j) ? x[q][cs] : from_re<T>(r == j ? 1.0 : 0.0);
-                }
-                if (lane == 0) {
-                    s_tau = t;
-                    Ts[j + NB * j] = t;
-                    // det(H_j) = 1 - tau v^H v, v^H v = 1 + |scal|^2 |x1|^2
-                    const double vv = 1.0 + abs2(scal) * xn2;
-                    s_det = s_det * (from_re<T>(1.0) - t * vv);
-                }
+                for (int u = 0; u < 8; u++)
+                    if (a0 + u < jp) s += Ts[lane + NB * (a0 + u)] * zz[a0 + u];
             }
-            __syncthreads();
-            // (b) apply H_j^H to the panel columns right of j; dots with the
-            //     reflectors left of j for the T recurrence.  The NCW dots of
-            //     a wave are summed over the wave in one register reduction.
-            const T tj = s_tau;
-            const T ct = cj(tj);
-            constexpr int NP = val_parts<T>::n;
+            Ts[lane + NB * jp] = -(Ts[jp + NB * jp] * s);
+        }
+    };
+
+    if (nbp > 0 && wave == 0) reflector(std::integral_constant<int, 0>(), 0);
+    __syncthreads();
+    // the register slot of the step's columns is a compile-time constant
+    // (a run-time index into x would move the panel to scratch memory)
+    static_for<NCW>([&](auto csc) __attribute__((always_inline)) {
+        constexpr int cs = decltype(csc)::value;
+        for (int w0 = 0; w0 < NW; w0++) {
+            const int j = w0 + NW * cs;
+            if (j >= nbp) break;
+            // the owner of column j + 1 is the step's critical path: its wave
+            // issues ahead of the others until its reflector is out
+            const int nxt = (j + 1) % NW;
+            const bool crit = j + 1 < nbp && wave == nxt;
+            if (crit) __builtin_amdgcn_s_setprio(2);
+            if (j > 0) t_column(j - 1);
+            // (b) d_jc = v_j^H x_jc for every column of the wave, summed over
+            //     the wave in one register reduction
+            const T* vb = v[j & 1];
+            const T ct = cj(Ts[j + NB * j]);
             double dr[NCW * NP];
 #pragma unroll
             for (int c2 = 0; c2 < NCW; c2++) {
-                const int jc = wave + 16 * c2;
-                T d = from_re<T>(0.0);
-                if (jc < nbp && jc != j) {
+                T d = zero;
 #pragma unroll
-                    for (int q = 0; q < QM; q++) {
-                        const int r = lane + 64 * q;
-                        if (r >= j && r < mp) {
-                            if (jc > j) d += cj(v[r]) * x[q][c2];
-                            else d += cj(x[q][c2]) * v[r];
-                        }
-                    }
-                }
+                for (int q = 0; q < QM; q++) d += cj(vb[lane + 64 * q]) * x[q][c2];
                 put_part(&dr[c2 * NP], d);
             }
             wave_sum_many<NCW * NP>(dr, lane);
 #pragma unroll
             for (int c2 = 0; c2 < NCW; c2++) {
-                const int jc = wave + 16 * c2;
-                if (jc < nbp && jc != j) {
-                    T d;
-                    get_part(&dr[c2 * NP], d);
-                    if (jc > j) {
-                        const T f = ct * d;
+                const int jc = wave + NW * c2;
+                T d;
+                get_part(&dr[c2 * NP], d);
+                if (jc > j) {
+                    const T f = ct * d;
 #pragma unroll
-                        for (int q = 0; q < QM; q++) {
-                            const int r = lane + 64 * q;
-                            if (r >= j && r < mp) x[q][c2] -= v[r] * f;
-                        }
-                    } else if (lane == 0) {
-                        z[jc] = d;
-                    }
+                    for (int q = 0; q < QM; q++) x[q][c2] -= vb[lane + 64 * q] * f;
+                } else if (jc < j && lane == 0) {
+                    z[j & 1][jc] = cj(d);
                 }
             }
-            __syncthreads();
-            // (c) T(0:j, j) = -tau_j T(0:j, 0:j) z(0:j)   (j <= 63), by the wave
-            //     that built reflector j: the owner of reflector j + 1 is
-            //     another wave, so this overlaps the next step's (a)
-            if (wave == w0 && lane < j) {
-                // T is upper triangular (zeros below the diagonal), so the sum
-                // may start at 0; the bound j is uniform, and the unrolled
-                // loop issues its independent LDS reads together
-                T s = from_re<T>(0.0);
-#pragma unroll
-                for (int a = 0; a < NB; a++)
-                    if (a < j) s += Ts[lane + NB * a] * z[a];
-                Ts[lane + NB * j] = -(tj * s);
+            // (a) of step j + 1 by the owner of column j + 1
+            if (j + 1 < nbp) {
+                if (w0 < NW - 1) {
+                    if (wave == w0 + 1) reflector(csc, j + 1);
+                } else if constexpr (cs + 1 < NCW) {
+                    if (wave == 0) reflector(std::integral_constant<int, cs + 1>(), j + 1);
+                }
             }
+            if (crit) __builtin_amdgcn_s_setprio(0);
+            __syncthreads();
         }
-    }
+    });
+    if (nbp > 0) t_column(nbp - 1);
     __syncthreads();
     // write back: R (upper) and the reflectors (below) into A, explicit V, T, tau
     T* V = V_ + b * sV + k0 + (long)ldv * k0;
+    // the row index passes through an empty asm statement, so the write
+    // addresses are formed here rather than kept live from the loads
+    int lw = lane;
+    asm volatile("" : "+v"(lw));
 #pragma unroll
     for (int cs = 0; cs < NCW; cs++) {
-        const int jc = wave + 16 * cs;
+        const int jc = wave + NW * cs;
         if (jc < nbp) {
 #pragma unroll
             for (int q = 0; q < QM; q++) {
-                const int r = lane + 64 * q;
+                const int r = lw + 64 * q;
                 if (r < mp) {
                     A[r + (long)lda * jc] = x[q][cs];
-                    V[r + (long)ldv * jc] =
-                        (r > jc) ? x[q][cs] : from_re<T>(r == jc ? 1.0 : 0.0);
+                    V[r + (long)ldv * jc] = selv(r > jc, x[q][cs], from_re<T>(r == jc ? 1.0 : 0.0));
                 }
             }
         }
     }
     T* Tw = Tw_ + b * sT;
-    for (int e = tid; e < NB * NB; e += QRB_THREADS) Tw[e] = Ts[e];
+    for (int e = tid; e < NB * NB; e += NT) Tw[e] = Ts[e];
     T* tau = tau_ + b * stau + k0;
-    for (int jc = tid; jc < nbp; jc += QRB_THREADS) tau[jc] = Ts[jc + NB * jc];
+    for (int jc = tid; jc < nbp; jc += NT) tau[jc] = Ts[jc + NB * jc];
     if (tid == 0 && det_) det_[b] = det_init ? s_det : det_[b] * s_det;
 }
 

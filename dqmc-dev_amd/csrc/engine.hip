@@ -465,11 +465,11 @@ static int mul_seq_C(dqmc_engine* e, int min, int maxp1, T alpha, T* out, int ld
 // --------------------------------------------------------------------------
 // blocked Householder QR and blocked upper-triangular solve (dq_qr_blocked.hpp)
 // --------------------------------------------------------------------------
-template <typename T, int QM, int NCW>
+template <typename T, int QM, int NCW, int NW>
 static void launch_panel(dqmc_engine* e, int batch, int m, int k0, int nbp, T* A, int lda, long sA,
                          T* tau, long stau, T* V, int ldv, long sV, T* Tk, long sT, T* det, int init)
 {
-    k_qr_panel<T, QM, NCW><<<batch, QRB_THREADS, 0, e->st>>>(m, k0, nbp, A, lda, sA, tau, stau, V,
+    k_qr_panel<T, QM, NCW, NW><<<batch, 64 * NW, 0, e->st>>>(m, k0, nbp, A, lda, sA, tau, stau, V,
                                                              ldv, sV, Tk, sT, det, init);
 }
 
@@ -518,8 +518,11 @@ static int qr_blocked(dqmc_engine* e, int m, int n, T* A, int lda, long sA, T* t
     constexpr bool Z = is_cplx<T>::value;
     if (m > 1024) return set_err(DQMC_ERR_UNSUPPORTED, "QR of %d rows", m);
     const int qm = m <= 256 ? 4 : (m <= 512 ? 8 : 16);
-    const int ncw = Z ? (qm == 16 ? 1 : 2) : (qm == 16 ? 2 : 4);
-    const int NB = 16 * ncw;
+    // panel width 64 (real) / 32 (complex) up to 512 rows, half that beyond;
+    // real panels of up to 256 rows in 8 waves of 8 columns (two workgroups
+    // per CU), the others in 16 waves (a lane's panel share and the rest of
+    // the step within 128 VGPRs)
+    const int NB = qm == 16 ? (Z ? 16 : 32) : (Z ? 32 : 64);
     const int kmax = m < n ? m : n;
     T* V = dp<T>(e->Vb);
     const long sV = e->sVb;
@@ -535,13 +538,13 @@ static int qr_blocked(dqmc_engine* e, int m, int n, T* A, int lda, long sA, T* t
         const int nbp = (kmax - k0 < NB) ? kmax - k0 : NB;
         T* Tk = Tb + (long)np * NB * NB;
         if constexpr (Z) {
-            if (qm == 4) launch_panel<T, 4, 2>(e, batch, m, k0, nbp, A, lda, sA, tau, stau, V, ldv, sV, Tk, sT, det, k0 == 0);
-            else if (qm == 8) launch_panel<T, 8, 2>(e, batch, m, k0, nbp, A, lda, sA, tau, stau, V, ldv, sV, Tk, sT, det, k0 == 0);
-            else launch_panel<T, 16, 1>(e, batch, m, k0, nbp, A, lda, sA, tau, stau, V, ldv, sV, Tk, sT, det, k0 == 0);
+            if (qm == 4) launch_panel<T, 4, 2, 16>(e, batch, m, k0, nbp, A, lda, sA, tau, stau, V, ldv, sV, Tk, sT, det, k0 == 0);
+            else if (qm == 8) launch_panel<T, 8, 2, 16>(e, batch, m, k0, nbp, A, lda, sA, tau, stau, V, ldv, sV, Tk, sT, det, k0 == 0);
+            else launch_panel<T, 16, 1, 16>(e, batch, m, k0, nbp, A, lda, sA, tau, stau, V, ldv, sV, Tk, sT, det, k0 == 0);
         } else {
-            if (qm == 4) launch_panel<T, 4, 4>(e, batch, m, k0, nbp, A, lda, sA, tau, stau, V, ldv, sV, Tk, sT, det, k0 == 0);
-            else if (qm == 8) launch_panel<T, 8, 4>(e, batch, m, k0, nbp, A, lda, sA, tau, stau, V, ldv, sV, Tk, sT, det, k0 == 0);
-            else launch_panel<T, 16, 2>(e, batch, m, k0, nbp, A, lda, sA, tau, stau, V, ldv, sV, Tk, sT, det, k0 == 0);
+            if (qm == 4) launch_panel<T, 4, 8, 8>(e, batch, m, k0, nbp, A, lda, sA, tau, stau, V, ldv, sV, Tk, sT, det, k0 == 0);
+            else if (qm == 8) launch_panel<T, 8, 4, 16>(e, batch, m, k0, nbp, A, lda, sA, tau, stau, V, ldv, sV, Tk, sT, det, k0 == 0);
+            else launch_panel<T, 16, 2, 16>(e, batch, m, k0, nbp, A, lda, sA, tau, stau, V, ldv, sV, Tk, sT, det, k0 == 0);
         }
         LCHK();
         const int nt = n - k0 - nbp;
