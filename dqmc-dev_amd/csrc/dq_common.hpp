@@ -193,5 +193,9 @@ __device__ inline void get_part(const double* a, double& v) { v = a[0]; }
 __device__ inline void get_part(const double* a, zd& v) { v = mkz(a[0], a[1]); }
 
 __device__ inline double shfl(double v, int src) { return __shfl(v, src, 64); }
+// broadcast of lane l (wave-uniform) through the scalar unit: v_readlane, no
+// LDS round trip (a shuffle from a uniform source costs a ds_bpermute)
+__device__ inline double bcast(double v, int l) { return readlane_d(v, l); }
+__device__ inline zd bcast(zd v, int l) { return mkz(readlane_d(v.x, l), readlane_d(v.y, l)); }
 __device__ inline int shfl(int v, int src) { return __shfl(v, src, 64); }
 __device__ inline zd shfl(zd v, int src) { return mkz(__shfl(v.x, src, 64), __shfl(v.y, src, 64)); }

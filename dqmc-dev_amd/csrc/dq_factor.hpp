@@ -261,26 +261,35 @@ __global__ void k_extract_v(int n, const T* A_, long sA, T* V_, long sV)
         V[i + (long)n * c] = (i > c) ? A[i + (long)n * c] : from_re<T>(i == c ? 1.0 : 0.0);
 }
 
-// After the pivoted QR of the first product (src/greens.c:91-101):
-// d_i = R_ii (1 if 0), T[:, jpvt_j - 1] = diag(1/d) R[:, j] (upper part).
+// d_i = R_ii of an upper-triangular R (1 if 0; R's diagonal is real, the
+// Householder betas), read once per matrix: the kernels below take d from
+// here instead of each gathering the whole diagonal again.
+template <typename T>
+__global__ __launch_bounds__(256) void k_eqg_diag(int n, const T* R_, long sR, double* d_, long sd)
+{
+    const int b = blockIdx.y;
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    double di = re(R_[b * sR + i + (long)n * i]);
+    d_[b * sd + i] = (di == 0.0) ? 1.0 : di;
+}
+
+// After the pivoted QR of the first product (src/greens.c:91-101), with d
+// from k_eqg_diag: T[:, jpvt_j - 1] = diag(1/d) R[:, j] (upper part).
 template <typename T>
 __global__ void k_eqg_init_T(int n, const T* R_, long sR, const int* jpvt_, long spiv,
-                             double* d_, long sd, T* Tm_, long sT)
+                             const double* d_, long sd, T* Tm_, long sT)
 {
     const int b = blockIdx.y;
     const T* R = R_ + b * sR;
     const int* jp = jpvt_ + b * spiv;
-    double* d = d_ + b * sd;
+    const double* d = d_ + b * sd;
     T* Tm = Tm_ + b * sT;
     const int j = blockIdx.x;  // one block per column
     const int c = jp[j] - 1;
     for (int i = threadIdx.x; i < n; i += blockDim.x) {
-        // R's diagonal is real (Householder beta), as is d throughout
-        double di = re(R[i + (long)n * i]);
-        if (di == 0.0) di = 1.0;
-        const double vi = 1.0 / di;
+        const double vi = 1.0 / d[i];
         Tm[i + (long)n * c] = (i <= j) ? vi * R[i + (long)n * j] : from_re<T>(0.0);
-        if (j == 0) d[i] = di;
     }
 }
 
@@ -323,52 +332,65 @@ __global__ __launch_bounds__(1024) void k_prepivot(int n, const T* Y_, long sY, 
     }
 }
 
-// After the QR of a pre-pivoted product (src/greens.c:137-154):
-// d_i = R_ii (1 if 0); Rs = diag(1/d) * triu(R); Tp[i,:] = T[perm_i, :].
+// After the QR of a pre-pivoted product (src/greens.c:137-154), with d from
+// k_eqg_diag: Rs = diag(1/d) * triu(R); Tp[i,:] = T[perm_i, :].
 template <typename T>
 __global__ void k_eqg_post_qr(int n, const T* R_, long sR, const int* perm_, long sp, const T* Tm_,
-                              long sT, double* d_, long sd, T* Rs_, T* Tp_, long sW)
+                              long sT, const double* d_, long sd, T* Rs_, T* Tp_, long sW)
 {
     const int b = blockIdx.y;
     const T* R = R_ + b * sR;
     const int* perm = perm_ + b * sp;
     const T* Tm = Tm_ + b * sT;
-    double* d = d_ + b * sd;
+    const double* d = d_ + b * sd;
     T* Rs = Rs_ + b * sW;
     T* Tp = Tp_ + b * sW;
     const int c = blockIdx.x;
     for (int i = threadIdx.x; i < n; i += blockDim.x) {
-        double di = re(R[i + (long)n * i]);
-        if (di == 0.0) di = 1.0;
-        const double vi = 1.0 / di;
+        const double vi = 1.0 / d[i];
         Rs[i + (long)n * c] = (i <= c) ? vi * R[i + (long)n * c] : from_re<T>(0.0);
         Tp[i + (long)n * c] = Tm[perm[i] + (long)n * c];
-        if (c == 0) d[i] = di;
     }
 }
 
 // Eq. 2.12 set-up (src/greens.c:158-175): split d into Db/Ds, then
 //   RHS = diag(v) Q^H,  M = diag(d) T + RHS.
+// 32 x 32 tiles (blockDim 32 x 8): Q^H is read along Q's columns and written
+// along RHS's columns through an LDS tile.
 template <typename T>
-__global__ void k_eqg_final_prep(int n, const T* Q_, long sQ, const double* d_, long sd,
-                                 const T* Tm_, long sT, T* M_, T* RHS_, long sW, double* v_, long sv)
+__global__ __launch_bounds__(256) void k_eqg_final_prep(int n, const T* Q_, long sQ,
+                                                        const double* d_, long sd, const T* Tm_,
+                                                        long sT, T* M_, T* RHS_, long sW, double* v_,
+                                                        long sv)
 {
-    const int b = blockIdx.y;
+    __shared__ T tile[32][33];  // tile[k][x] = Q[c0 + x, i0 + k]
+    const int b = blockIdx.z;
+    const int i0 = 32 * blockIdx.x, c0 = 32 * blockIdx.y;
+    const int tx = threadIdx.x, ty = threadIdx.y;
     const T* Q = Q_ + b * sQ;
+    for (int k = ty; k < 32; k += 8) {
+        const int i = i0 + k, c = c0 + tx;
+        if (i < n && c < n) tile[k][tx] = Q[c + (long)n * i];
+    }
+    __syncthreads();
     const double* d = d_ + b * sd;
     const T* Tm = Tm_ + b * sT;
     T* M = M_ + b * sW;
     T* RHS = RHS_ + b * sW;
-    const int c = blockIdx.x;
-    for (int i = threadIdx.x; i < n; i += blockDim.x) {
-        const double di = d[i];
-        double vi, ds;
-        if (fabs(di) > 1.0) { vi = 1.0 / di; ds = 1.0; }
-        else { vi = 1.0; ds = di; }
-        const T r = vi * cj(Q[c + (long)n * i]);
-        RHS[i + (long)n * c] = r;
-        M[i + (long)n * c] = ds * Tm[i + (long)n * c] + r;
-        if (c == 0) v_[b * sv + i] = vi;
+    const int i = i0 + tx;
+    if (i >= n) return;
+    const double di = d[i];
+    double vi, ds;
+    if (fabs(di) > 1.0) { vi = 1.0 / di; ds = 1.0; }
+    else { vi = 1.0; ds = di; }
+    if (c0 == 0 && ty == 0) v_[b * sv + i] = vi;
+    for (int k = ty; k < 32; k += 8) {
+        const int c = c0 + k;
+        if (c < n) {
+            const T r = vi * cj(tile[tx][k]);
+            RHS[i + (long)n * c] = r;
+            M[i + (long)n * c] = ds * Tm[i + (long)n * c] + r;
+        }
     }
 }
 
@@ -462,7 +484,7 @@ __global__ __launch_bounds__(256) void k_getrs_phase(int n, const T* M_, T* RHS_
 #pragma unroll
                 for (int r = 0; r < RPL; r++)
                     if (r == rk) val = x[q][r];
-                xk[q] = shfl(val, owner);
+                xk[q] = bcast(val, owner);
             }
 #pragma unroll
             for (int r = 0; r < RPL; r++) {
@@ -485,7 +507,7 @@ __global__ __launch_bounds__(256) void k_getrs_phase(int n, const T* M_, T* RHS_
                 for (int r = 0; r < RPL; r++)
                     if (r == rk) val = x[q][r];
                 val = rdiv(val, ukk);
-                xk[q] = shfl(val, owner);
+                xk[q] = bcast(val, owner);
 #pragma unroll
                 for (int r = 0; r < RPL; r++)
                     if (r == rk && lane == owner) x[q][r] = xk[q];

@@ -86,7 +86,7 @@ __global__ __launch_bounds__(64 * NW, 4) void k_qr_panel(int m, int k0, int nbp,
             if (r > jn && r < mp) xn2 += abs2(x[q][cs]);
         }
         xn2 = wave_sum_r(xn2, lane);
-        alpha = shfl(alpha, jn & 63);
+        alpha = bcast(alpha, jn & 63);
         T beta, t, scal;
         larfg(alpha, xn2, &beta, &t, &scal);
         T* vb = v[jn & 1];
@@ -233,7 +233,7 @@ __global__ __launch_bounds__(256) void k_trsm_diag64(int nk, int ncols, const T*
         const T uik = Us[lane + 64 * k];
 #pragma unroll
         for (int q = 0; q < LU_NC; q++) {
-            T xk = shfl(x[q], k);
+            T xk = bcast(x[q], k);
             xk = xk * rkk;
             if (lane == k) x[q] = xk;
             if (lane < k) x[q] -= uik * xk;

@@ -117,12 +117,13 @@ __device__ __forceinline__ void qrp_apply_batch(T (&xb)[CB][RPL], unsigned lv, i
 // m x n matrix A[b] (lda), n <= m <= 64 RPL, n <= 8 (SR + SL), SR a multiple
 // of CB.  Outputs: A holds R (upper) and the reflectors (below the diagonal)
 // in pivoted column order, tau[b][0..n), jpvt[b][j] = 1-based original column
-// of pivot j, and the explicit unit-lower V[b] (ldv, m x n).  CB: register
-// columns per batched reduction (LDS columns go in pairs).
+// of pivot j, the explicit unit-lower V[b] (ldv, m x n) and, if det_ is
+// non-null, det_[b] = prod_j det(H_j) = prod_j (1 - tau_j v_j^H v_j).  CB:
+// register columns per batched reduction (LDS columns go in pairs).
 template <typename T, int RPL, int SR, int SL, int CB>
 __global__ __launch_bounds__(QRP_THREADS) void k_qrp_res(int m, int n, T* A_, int lda, long sA,
                                                          T* tau_, long stau, int* jpvt_, long spiv,
-                                                         T* V_, int ldv, long sV)
+                                                         T* V_, int ldv, long sV, T* det_)
 {
     static_assert(SR % CB == 0, "register columns come in whole batches");
     constexpr int MR = 64 * RPL;
@@ -201,6 +202,7 @@ __global__ __launch_bounds__(QRP_THREADS) void k_qrp_res(int m, int n, T* A_, in
     }
     __syncthreads();
 
+    T det = from_re<T>(1.0);  // product of the reflectors' determinants (uniform)
     for (int j = 0; j < n; j++) {
         const int cb = j & 1;
         // (1) pivot: the largest candidate norm, first column on ties
@@ -228,6 +230,8 @@ __global__ __launch_bounds__(QRP_THREADS) void k_qrp_res(int m, int n, T* A_, in
         const T alpha = Cw[bp][j];
         T beta, tj, scal;
         larfg_sel(alpha, xn2, beta, tj, scal);
+        // det(H_j) = 1 - tau v^H v, v^H v = 1 + |scal|^2 |x1|^2
+        det = det * (from_re<T>(1.0) - tj * (1.0 + abs2(scal) * xn2));
         T v[RPL];
 #pragma unroll
         for (int t = 0; t < RPL; t++) {
@@ -290,6 +294,7 @@ __global__ __launch_bounds__(QRP_THREADS) void k_qrp_res(int m, int n, T* A_, in
         }
         __syncthreads();
     }
+    if (det_ && tid == 0) det_[b] = det;
 }
 
 // Compact-WY factor of each NB-column panel of reflectors from its Gram

@@ -130,7 +130,7 @@ __global__ __launch_bounds__(256) void k_trsm_upper(int n, int ncols, const T* U
             for (int r = 0; r < RPL; r++)
                 if (r == rk) val = x[q][r];
             val = rdiv(val, ukk);
-            xk[q] = shfl(val, owner);
+            xk[q] = bcast(val, owner);
 #pragma unroll
             for (int r = 0; r < RPL; r++)
                 if (r == rk && lane == owner) x[q][r] = xk[q];

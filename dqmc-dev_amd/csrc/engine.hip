@@ -577,14 +577,14 @@ static bool qrp_resident_fits(int n)
 
 template <typename T>
 static int qrp_resident(dqmc_engine* e, int n, T* A, long sA, T* tau, int* jpvt, T* Q, long sQ,
-                        int batch)
+                        T* det, int batch)
 {
     constexpr bool Z = is_cplx<T>::value;
     T* V = dp<T>(e->Vb);
     const long sV = e->sVb;
 #define QRP(R, SR, SL, CB)                                                                  \
     k_qrp_res<T, R, SR, SL, CB><<<batch, QRP_THREADS, 0, e->st>>>(n, n, A, n, sA, tau, n, jpvt, \
-                                                                  n, V, n, sV)
+                                                                  n, V, n, sV, det)
     if constexpr (Z) {
         if (n <= 64) QRP(1, 8, 0, 2);
         else if (n <= 128) QRP(2, 14, 2, 2);
@@ -691,8 +691,8 @@ static int calc_eq_g(dqmc_engine* e, int l, T* Gout, T* phase_b)
     int m = (l + 1 + (F - 1) % n_mul) % F;
     RC(mul_seq_C<T>(e, l, m, one, X, N, NN));
     // (1) column-pivoted QR of the first product (xgeqp3)
-    if (qrp_resident_fits<T>(N)) {
-        RC(qrp_resident<T>(e, N, X, NN, tau, e->jpvt, Q, NN, B));
+    if (qrp_resident_fits<T>(N)) {  // also forms det(Q) = prod det(H_j)
+        RC(qrp_resident<T>(e, N, X, NN, tau, e->jpvt, Q, NN, detq, B));
     } else {
         // streaming pivoted QR; Q from the reflectors by compact-WY GEMMs
         if (N <= 256)
@@ -705,8 +705,10 @@ static int calc_eq_g(dqmc_engine* e, int l, T* Gout, T* phase_b)
         k_extract_v<T><<<dim3(N, B), 256, 0, e->st>>>(N, X, NN, dp<T>(e->Vb), e->sVb);
         LCHK();
         RC(q_from_reflectors<T>(e, N, tau, Q, NN, B));
+        k_reflector_det<T><<<B, 64, 0, e->st>>>(N, N, X, N, NN, tau, N, detq);
+        LCHK();
     }
-    k_reflector_det<T><<<B, 64, 0, e->st>>>(N, N, X, N, NN, tau, N, detq);
+    k_eqg_diag<T><<<dim3((N + 255) / 256, B), 256, 0, e->st>>>(N, X, NN, e->dvec, N);
     LCHK();
     k_eqg_init_T<T><<<dim3(N, B), 256, 0, e->st>>>(N, X, NN, e->jpvt, N, e->dvec, N, Tm, NN);
     LCHK();
@@ -722,6 +724,8 @@ static int calc_eq_g(dqmc_engine* e, int l, T* Gout, T* phase_b)
         LCHK();
         RC(qr_blocked<T>(e, N, N, Z, N, NN, tau, N, Q, N, NN, detq, B));
         // (3d) T = diag(1/d) R * P^T T   (xtrmm L,U)
+        k_eqg_diag<T><<<dim3((N + 255) / 256, B), 256, 0, e->st>>>(N, Z, NN, e->dvec, N);
+        LCHK();
         k_eqg_post_qr<T><<<dim3(N, B), 256, 0, e->st>>>(N, Z, NN, e->perm, N, Tm, NN, e->dvec, N,
                                                        Rs, Tp, NN);
         LCHK();
@@ -729,8 +733,8 @@ static int calc_eq_g(dqmc_engine* e, int l, T* Gout, T* phase_b)
     }
     // Eq. 2.12: G = M'^-1 diag(v) Q^H with M' = diag(ds) T + diag(v) Q^H, solved
     // stably through the QR of M' (M' = Q_M R_M, G = R_M^-1 Q_M^H RHS)
-    k_eqg_final_prep<T><<<dim3(N, B), 256, 0, e->st>>>(N, Q, NN, e->dvec, N, Tm, NN, Mm, Gout, NN,
-                                                      e->vvec, N);
+    k_eqg_final_prep<T><<<dim3((N + 31) / 32, (N + 31) / 32, B), dim3(32, 8), 0, e->st>>>(
+        N, Q, NN, e->dvec, N, Tm, NN, Mm, Gout, NN, e->vvec, N);
     LCHK();
     RC(qr_blocked<T>(e, N, N, Mm, N, NN, tau, N, X, N, NN, detm, B));
     RC(gemm<T>(e, OP_C, OP_N, N, N, N, one, X, N, NN, Gout, N, NN, zero, Y, N, NN, B));
