@@ -32,11 +32,21 @@ PEAK_FP64_TFLOPS = 78.6  # MI355X dense FP64 (matrix = vector)
 GEMM_HBM_BYTES_PER_MATRIX_N256 = 1574365
 
 
-def workload(nchains, seed=7):
+# optional measurement groups of the reference (src/meas.c) on top of the
+# default sets, for profiling the contraction kernels (--all-meas).  Left
+# out: local JQ (the reference defines it for trans_sym = 0 files only) and
+# gen_suscept (N^2 bins per slice: 168 MB of accumulators per chain at
+# N = 256); the complex workload also leaves out nematic, which the
+# reference defines without Peierls phases.
+ALL_MEAS = dict(meas_bond_corr=1, meas_thermal=1, meas_2bond_corr=1, meas_energy_corr=1,
+                meas_nematic_corr=1, meas_chiral=1)
+
+
+def workload(nchains, seed=7, **extra):
     from dqmc_amd import gen
     return gen.create_batch(Nfiles=nchains, seed=seed, Nx=16, Ny=16, tp=-0.25, U=6.0, mu=-1.0,
                             dt=0.1, L=80, n_matmul=10, n_delay=32, period_eqlt=8,
-                            period_uneqlt=2, n_sweep_warm=0, n_sweep_meas=100000)
+                            period_uneqlt=2, n_sweep_warm=0, n_sweep_meas=100000, **extra)
 
 
 def flop_model(N, L, F, n_matmul, accepts_per_sweep, n_delay, q_eq, ue_fraction):
@@ -223,6 +233,8 @@ def bench_config3(args):
               n_sweep_meas=100000, meas_bond_corr=1)
     if args.all_bond_sets:  # also the thermal and two-hop (2-bond) correlators
         kw.update(meas_thermal=1, meas_2bond_corr=1)
+    if args.all_meas:
+        kw.update({k: v for k, v in ALL_MEAS.items() if k != "meas_nematic_corr"})
     eng = engine.Engine(gen.create_batch(Nfiles=1, seed=3, **kw), graphs=not args.no_graphs)
     dt1, n1 = _run_engines([eng], max(2, args.steps), 1, 1)
     eng.close()
@@ -236,7 +248,8 @@ def bench_config3(args):
                  "at t'=0 gauge-equivalent to the reference generator's alpha=1/2), complex128, "
                  "N=144 L=60 U=4 mu=-0.5 beta=6 n_matmul=10 delay=16, eq every 10 slices, "
                  "uneq + bond correlators every 2 sweeps"
-                 + (" (+ thermal and 2-bond sets)" if args.all_bond_sets else ""),
+                 + (" (+ thermal and 2-bond sets)" if args.all_bond_sets else "")
+                 + (" (+ energy and chiral groups)" if args.all_meas else ""),
                  dict({"single_chain_sweeps_per_s": round(n1 / dt1, 4),
                        "batched_over_single": round(nbs / dtb / (n1 / dt1), 3), "dtype": "c128",
                        "model_acceptance_assumed": 0.5},
@@ -389,6 +402,9 @@ def main():
                          "(for profilers: rocprofv3 --kernel-trace crashed on graph launches)")
     ap.add_argument("--all-bond-sets", action="store_true",
                     help="--config 3: also measure the thermal and two-hop bond correlators")
+    ap.add_argument("--all-meas", action="store_true",
+                    help="--config 2 / 3: the optional measurement groups on (profiling the "
+                         "contraction kernels; not the headline workload)")
     ap.add_argument("--no-tune", action="store_true",
                     help="--config 4: fixed n_matmul per bucket instead of the autotuner")
     ap.add_argument("--threads", type=int, default=8,
@@ -401,11 +417,6 @@ def main():
     import numpy as np
     from dqmc_amd import engine
 
-    if args.config != 2:
-        fn = {1: bench_config1, 3: bench_config3, 4: bench_config4, 5: bench_config5}[args.config]
-        print(json.dumps(fn(args)))
-        return
-
     if args.copy_probe > 0:
         import torch
         src = torch.ones(args.copy_probe << 17, dtype=torch.float64, device="cuda:0")
@@ -415,12 +426,17 @@ def main():
         torch.cuda.synchronize()
         del src, dst
 
+    if args.config != 2:
+        fn = {1: bench_config1, 3: bench_config3, 4: bench_config4, 5: bench_config5}[args.config]
+        print(json.dumps(fn(args)))
+        return
+
     tiles = {1: "k_gemm<double,N,N> 64x64 (4x4x4 form)", 2: "k_gemm<double,N,N> 128x64/16 (4x4x4)",
              3: "k_gemm<double,N,N> 128x64/8 (4x4x4)",
              4: "k_gemm_w 128x64/16 (16x16x4 form, fragment-order LDS)"}
 
     def timed(nch, steps, warmup, time_gemm):
-        sims = workload(nch)
+        sims = workload(nch, **(ALL_MEAS if args.all_meas else {}))
         eng = engine.Engine(sims, device=0, time_gemm=time_gemm, gemm_tile=args.gemm_tile,
                             graphs=not args.no_graphs)
         tile = eng.gemm_tile()
@@ -472,7 +488,10 @@ def main():
         "data": "synthetic (package generator, seed 7 + jump per chain)",
         "config": {"workload": "16x16 square Hubbard N=256 L=80 t'=-0.25 U=6 mu=-1 beta=8 "
                                "n_matmul=10 delay rank 32 (A in registers, B in LDS) eq every 8 slices, "
-                               "uneq every 2 sweeps",
+                               "uneq every 2 sweeps"
+                               + (" + bond, thermal, 2-bond, energy, nematic and chiral groups "
+                                  "(profiling run, not the headline workload)"
+                                  if args.all_meas else ""),
                    "chains": nch, "parallelism": "1 GPU, chains batched"},
         "graphs": not args.no_graphs,
         "eager_timed_gemms_sweeps_per_s": round(nch * args.steps / dte, 4),

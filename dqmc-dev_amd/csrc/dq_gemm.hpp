@@ -55,7 +55,7 @@ struct GemmArgs {
 
 // Tile shapes (rows x columns of C per 256-thread workgroup).  The engine's
 // autotuner times the shapes on its own product sizes and keeps the fastest.
-// GT_W: the real op-N product on the 16x16x4 form with 128-bit LDS traffic
+// GT_W: real products op(A) B on the 16x16x4 form with 128-bit LDS traffic
 // (dq_gemm16.hpp), for the shapes it takes; other products fall back to 64x64.
 enum { GT_AUTO = 0, GT_64 = 1, GT_128 = 2, GT_128x64 = 3, GT_W = 4 };
 
@@ -395,20 +395,19 @@ static hipError_t gemm_launch_tile(int opa, int opb, const GemmArgs<T>& g, int b
     return hipGetLastError();
 }
 
-// Launch with tile shape `tile` (GT_*); GT_AUTO and shapes a product is too
-// small for fall back to 64 x 64.  The 4x4x4 form is used (M4): on the
-// 256 x 256 x 256 batch-512 wraps it takes 382 us against 412 us for the
-// 16x16x4 form, whose path stays selectable through the template argument.
-// GT_128 is the 128 x 64 tile with K steps of 16 (GT_128x64 steps by 8); a
-// 128 x 128 tile needs 268 registers (one wave per SIMD) and was the slowest
-// shape measured.
+// Launch with tile shape `tile` (GT_*).  GT_W takes the real op-N products
+// whose shapes and alignments it fits (gemm_w_fits); GT_128 is the 128 x 64
+// tile on the 4x4x4 form with K steps of 16 (GT_128x64 steps by 8); GT_AUTO
+// and every product a shape does not take fall back to 64 x 64 on the 4x4x4
+// form.  (A 128 x 128 tile on the 4x4x4 form needs 268 registers, one wave
+// per SIMD, and was the slowest shape measured.)
 template <typename T>
 static hipError_t gemm_launch(int opa, int opb, const GemmArgs<T>& g, int batch, hipStream_t s,
                               int tile)
 {
     constexpr bool Z = is_cplx<T>::value;
     if constexpr (!Z) {
-        if (tile == GT_W && gemm_w_fits(opa, opb, g)) return gemm_w_launch(g, batch, s);
+        if (tile == GT_W && gemm_w_fits(opa, opb, g)) return gemm_w_launch(opa, g, batch, s);
     }
     if (tile == GT_128 && g.M > 64)  // 128 x 64 with K steps of 16 (8 complex)
         return gemm_launch_tile<T, 4, 2, Z ? 8 : 16, 1>(opa, opb, g, batch, s);

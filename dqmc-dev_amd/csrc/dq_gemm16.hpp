@@ -1,17 +1,18 @@
-// dq_gemm16.hpp -- the real N x N product on v_mfma_f64_16x16x4f64 with
-// 128-bit LDS traffic (tile GT_W of dq_gemm.hpp's launcher).
+// dq_gemm16.hpp -- real products on v_mfma_f64_16x16x4f64 with 128-bit LDS
+// traffic (tile GT_W of dq_gemm.hpp's launcher).
 //
-//   C[b] = alpha * diag(r) * A * diag(u) * B * diag(w) + beta * C,  A, B op N,
-//   M % 128 == 0, N % 64 == 0, K % 16 == 0, 16-byte aligned columns.
+//   C[b] = alpha * diag(r) * op(A) * diag(u) * B * diag(w) + beta * C,
+//   op(A) = A or A^T, B op N, M % BM == 0 (BM = 128 or 64), N % 64 == 0,
+//   K % 16 == 0, 16-byte aligned columns.
 //
-// 256 threads per 128 x 64 tile of C, four waves in a 2 x 2 arrangement, each
-// wave 64 x 32 = 4 x 2 blocks of 16 x 16.  K advances in steps of 16 through
-// two LDS buffers (the next step's global loads in flight during the current
-// step's MFMAs, one barrier per step).
+// 256 threads per BM x 64 tile of C, four waves in a 2 x 2 arrangement, each
+// wave (BM / 2) x 32 = MI x 2 blocks of 16 x 16.  K advances in steps of 16
+// through two LDS buffers (the next step's global loads in flight during the
+// current step's MFMAs, one barrier per step).
 //
 // K order inside a step: MFMA sub-step s (0..3) gives the lanes of quarter q
 // (lane / 16) the k index 4q + s, for both operands, so a lane needs
-// k = 4q .. 4q + 3 of its row of A and of its column of B.  LDS holds the
+// k = 4q .. 4q + 3 of its row of op(A) and of its column of B.  LDS holds the
 // tiles in that fragment order: per 16-row (16-column) block, two 1 KiB
 // regions, region j holding at slot l the pair k = 4(l/16) + 2j, +1 of row
 // (column) l % 16.  Every LDS access of the kernel is then one 16-byte slot
@@ -19,24 +20,26 @@
 // reads at the full 256 bytes per clock (rows 144 bytes apart, conflict-free
 // by banks, took twice as long per read on gfx950).
 //
-// Global -> LDS: B's columns are k-contiguous in memory, so a thread moves
-// k = 4q .. 4q + 3 of one column (two 16-byte loads; u is applied here) into
-// its slot of both regions.  A's columns are m-contiguous: a thread loads its
-// row's four k columns (four 8-byte loads, the 16 lanes of a quarter reading
-// 128 contiguous bytes of each) for two 16-row blocks.
+// Global -> LDS: B's columns, and op(A)'s rows for op(A) = A^T, are
+// k-contiguous in memory, so a thread moves k = 4q .. 4q + 3 of one column
+// (row) with two 16-byte loads into its slot of both regions (u is applied to
+// B here).  For op(A) = A the rows are m-contiguous: a thread loads its row's
+// four k columns (four 8-byte loads, the 16 lanes of a quarter reading 128
+// contiguous bytes of each).
 #pragma once
 #include "dq_common.hpp"
 // included by dq_gemm.hpp after GemmArgs
 
-#define GW_BM 128
 #define GW_BN 64
 #define GW_BK 16
 
-template <bool BETA>
+// MI: 16-row blocks per wave along M (BM = 32 MI); TA: op(A) = A^T
+template <int MI, bool TA, bool BETA>
 __global__ __launch_bounds__(256, 3) void k_gemm_w(GemmArgs<double> g, int tiles_m, int tiles_n,
                                                    int nwg)
 {
-    __shared__ __align__(16) double As[2][GW_BM * GW_BK];
+    constexpr int BM = 32 * MI, TB = MI / 2;  // TB: A blocks loaded per wave
+    __shared__ __align__(16) double As[2][BM * GW_BK];
     __shared__ __align__(16) double Bs[2][GW_BN * GW_BK];
 
     // Workgroup ids are dealt round-robin over the 8 XCDs; renumber so that
@@ -49,17 +52,17 @@ __global__ __launch_bounds__(256, 3) void k_gemm_w(GemmArgs<double> g, int tiles
     const int rest = lid / tiles_m;
     const int tn = rest % tiles_n;
     const int b = rest / tiles_n;
-    const int m0 = tm * GW_BM, n0 = tn * GW_BN;
+    const int m0 = tm * BM, n0 = tn * GW_BN;
     const double* A = g.A + (long)(b / g.adiv) * g.sA;
     const double* B = g.B + (long)(b / g.bdiv) * g.sB;
     double* C = g.C + (long)b * g.sC;
     const double* u = g.u ? g.u + (long)b * g.su : nullptr;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    if (g.idLc && g.idl >= g.idLc[b >> 1]) {  // identity factor (uniform)
+    if (g.idLc && g.idl >= g.idLc[b >> 1]) {  // identity factor (uniform; op N only)
         const double* X = (g.idside == 1) ? B : A;
         const int ldx = (g.idside == 1) ? g.ldb : g.lda;
-        for (int e = tid; e < GW_BM * GW_BN; e += 256) {
-            const int row = m0 + e % GW_BM, col = n0 + e / GW_BM;
+        for (int e = tid; e < BM * GW_BN; e += 256) {
+            const int row = m0 + e % BM, col = n0 + e / BM;
             double v = g.alpha * X[row + (long)ldx * col];
             if constexpr (BETA) v += g.beta * C[row + (long)g.ldc * col];
             C[row + (long)g.ldc * col] = v;
@@ -68,19 +71,27 @@ __global__ __launch_bounds__(256, 3) void k_gemm_w(GemmArgs<double> g, int tiles
     }
     const int q = lane >> 4, l16 = lane & 15;
 
-    // A loads: rows 16 (2 wave + t) + l16, k = 4q .. 4q + 3
-    const double* Ag = A + (m0 + 32 * wave + l16) + (long)g.lda * (4 * q);
+    // A loads: rows 16 (TB wave + t) + l16, k = 4q .. 4q + 3
+    const double* Ag = TA ? A + 4 * q + (long)g.lda * (m0 + 16 * TB * wave + l16)
+                          : A + (m0 + 16 * TB * wave + l16) + (long)g.lda * (4 * q);
     // B loads: column 16 wave + l16, k = 4q .. 4q + 3
     const double* Bg = B + 4 * q + (long)g.ldb * (n0 + 16 * wave + l16);
 
-    double ra[2][4];
+    double ra[TB][4];
     double2 rb[2];
     double2 ub0 = make_double2(1.0, 1.0), ub1 = ub0;
     auto gload = [&](int k0) {
 #pragma unroll
-        for (int t = 0; t < 2; t++)
+        for (int t = 0; t < TB; t++) {
+            if constexpr (TA) {
+                const double* aa = Ag + (long)g.lda * (16 * t) + k0;
+                const double2 a0 = *(const double2*)(aa), a1 = *(const double2*)(aa + 2);
+                ra[t][0] = a0.x; ra[t][1] = a0.y; ra[t][2] = a1.x; ra[t][3] = a1.y;
+            } else {
 #pragma unroll
-            for (int c = 0; c < 4; c++) ra[t][c] = Ag[16 * t + (long)g.lda * (k0 + c)];
+                for (int c = 0; c < 4; c++) ra[t][c] = Ag[16 * t + (long)g.lda * (k0 + c)];
+            }
+        }
         const double* bb = Bg + k0;
         rb[0] = *(const double2*)(bb);
         rb[1] = *(const double2*)(bb + 2);
@@ -93,34 +104,34 @@ __global__ __launch_bounds__(256, 3) void k_gemm_w(GemmArgs<double> g, int tiles
     auto slot = [&](int blk, int j) { return ((2 * blk + j) * 64 + lane) * 2; };
     auto sstore = [&](int buf) {
 #pragma unroll
-        for (int t = 0; t < 2; t++) {
-            *(double2*)(&As[buf][slot(2 * wave + t, 0)]) = make_double2(ra[t][0], ra[t][1]);
-            *(double2*)(&As[buf][slot(2 * wave + t, 1)]) = make_double2(ra[t][2], ra[t][3]);
+        for (int t = 0; t < TB; t++) {
+            *(double2*)(&As[buf][slot(TB * wave + t, 0)]) = make_double2(ra[t][0], ra[t][1]);
+            *(double2*)(&As[buf][slot(TB * wave + t, 1)]) = make_double2(ra[t][2], ra[t][3]);
         }
         *(double2*)(&Bs[buf][slot(wave, 0)]) = make_double2(rb[0].x * ub0.x, rb[0].y * ub0.y);
         *(double2*)(&Bs[buf][slot(wave, 1)]) = make_double2(rb[1].x * ub1.x, rb[1].y * ub1.y);
     };
 
     const int wr = wave >> 1, wc = wave & 1;
-    const int wm = 64 * wr, wn = 32 * wc;
-    dbl4 acc[4][2];
+    const int wm = 16 * MI * wr, wn = 32 * wc;
+    dbl4 acc[MI][2];
 #pragma unroll
-    for (int i = 0; i < 4; i++)
+    for (int i = 0; i < MI; i++)
 #pragma unroll
         for (int j = 0; j < 2; j++) acc[i][j] = (dbl4){0.0, 0.0, 0.0, 0.0};
-    double2 av[4][2], bv[2][2];
+    double2 av[MI][2], bv[2][2];
     auto reads = [&](int buf) {
 #pragma unroll
         for (int jj = 0; jj < 2; jj++) {
 #pragma unroll
-            for (int i = 0; i < 4; i++) av[i][jj] = *(const double2*)(&As[buf][slot(4 * wr + i, jj)]);
+            for (int i = 0; i < MI; i++) av[i][jj] = *(const double2*)(&As[buf][slot(MI * wr + i, jj)]);
 #pragma unroll
             for (int j = 0; j < 2; j++) bv[j][jj] = *(const double2*)(&Bs[buf][slot(2 * wc + j, jj)]);
         }
     };
     auto mf = [&](int s) {  // MFMA sub-step s: k = 4q + s
 #pragma unroll
-        for (int i = 0; i < 4; i++) {
+        for (int i = 0; i < MI; i++) {
             const double a = (s & 1) ? av[i][s >> 1].y : av[i][s >> 1].x;
 #pragma unroll
             for (int j = 0; j < 2; j++) {
@@ -152,7 +163,7 @@ __global__ __launch_bounds__(256, 3) void k_gemm_w(GemmArgs<double> g, int tiles
     const double* rr = g.r ? g.r + (long)b * g.sr : nullptr;
     const double* ww = g.w ? g.w + (long)b * g.sw : nullptr;
 #pragma unroll
-    for (int i = 0; i < 4; i++) {
+    for (int i = 0; i < MI; i++) {
         const int row = m0 + wm + 16 * i + l16;
         const double rs = rr ? rr[row] : 1.0;
 #pragma unroll
@@ -170,25 +181,37 @@ __global__ __launch_bounds__(256, 3) void k_gemm_w(GemmArgs<double> g, int tiles
     }
 }
 
-// The shapes and alignments k_gemm_w takes
+// The shapes and alignments k_gemm_w takes (op(A) = A^T also for OP_C: the
+// data are real); an identity factor needs op N
 static inline bool gemm_w_fits(int opa, int opb, const GemmArgs<double>& g)
 {
     const auto al = [](const void* p) { return ((unsigned long)p & 15) == 0; };
-    return opa == 0 && opb == 0 && g.M % GW_BM == 0 && g.N % GW_BN == 0 && g.K % GW_BK == 0 &&
-           g.M > 0 && g.N > 0 && g.K > 0 && g.lda % 2 == 0 && g.ldb % 2 == 0 && g.sA % 2 == 0 &&
-           g.sB % 2 == 0 && al(g.A) && al(g.B) && (!g.u || (al(g.u) && g.su % 2 == 0));
+    return opb == 0 && (opa == 0 || !g.idLc) && g.M % 64 == 0 && g.N % GW_BN == 0 &&
+           g.K % GW_BK == 0 && g.M > 0 && g.N > 0 && g.K > 0 && g.lda % 2 == 0 &&
+           g.ldb % 2 == 0 && g.sA % 2 == 0 && g.sB % 2 == 0 && al(g.A) && al(g.B) &&
+           (!g.u || (al(g.u) && g.su % 2 == 0));
 }
 
-static inline hipError_t gemm_w_launch(const GemmArgs<double>& g, int batch, hipStream_t s)
+// 128-row tiles where M allows, else 64
+static inline hipError_t gemm_w_launch(int opa, const GemmArgs<double>& g, int batch,
+                                       hipStream_t s)
 {
-    const int tm = g.M / GW_BM, tn = g.N / GW_BN;
+    const int bm = g.M % 128 == 0 ? 128 : 64;
+    const int tm = g.M / bm, tn = g.N / GW_BN;
     const long nwg = (long)tm * tn * batch;
     if (nwg <= 0) return hipSuccess;
     if (nwg > 0x7fffffffL) return hipErrorInvalidValue;
     const int n = (int)nwg;
-    if (g.beta_zero)
-        k_gemm_w<false><<<n, 256, 0, s>>>(g, tm, tn, n);
-    else
-        k_gemm_w<true><<<n, 256, 0, s>>>(g, tm, tn, n);
+#define GWL(MI, TA)                                                            \
+    (g.beta_zero ? k_gemm_w<MI, TA, false><<<n, 256, 0, s>>>(g, tm, tn, n) \
+                 : k_gemm_w<MI, TA, true><<<n, 256, 0, s>>>(g, tm, tn, n))
+    if (opa == 0) {
+        if (bm == 128) GWL(4, false);
+        else GWL(2, false);
+    } else {
+        if (bm == 128) GWL(4, true);
+        else GWL(2, true);
+    }
+#undef GWL
     return hipGetLastError();
 }

@@ -594,7 +594,7 @@ static int qrp_resident(dqmc_engine* e, int n, T* A, long sA, T* tau, int* jpvt,
         if (n <= 64) QRP(1, 8, 0, 4);
         else if (n <= 128) QRP(2, 16, 0, 4);
         else if (n <= 192) QRP(3, 24, 0, 4);
-        else if (n <= 256) QRP(4, 24, 8, 4);
+        else if (n <= 256) QRP(4, 24, 8, 8);
         else return set_err(DQMC_ERR_UNSUPPORTED, "resident pivoted QR of order %d", n);
     }
 #undef QRP

@@ -59,12 +59,13 @@ def _gemm(cplx, opa, opb, M, N, K, rs, tile=0, scaled=True):
 def test_gemm_kernel(cplx, tile):
     """Every tile shape, every op pair, ragged edges (sizes not multiples of
     the tile or of the K step), and the unscaled beta = 0 path.  Tile 4 (the
-    16x16x4 form) takes the real op-N products of multiples of 128 x 64 x 16
-    and hands the rest to the 64 x 64 tile."""
+    16x16x4 form) takes the real products op(A) B of multiples of 64 x 64 x 16
+    (128-row tiles where M allows) and hands the rest to the 64 x 64 tile."""
     rs = np.random.RandomState(0)
     ops = (0, 1, 2) if cplx else (0, 1)
     for (M, N, K) in [(16, 16, 16), (70, 33, 45), (256, 256, 256), (1, 130, 7), (200, 144, 77),
-                      (128, 64, 16), (384, 192, 48)]:
+                      (128, 64, 16), (384, 192, 48), (64, 192, 64), (192, 128, 256),
+                      (64, 64, 16)]:
         for a in ops:
             for b in ops:
                 assert _gemm(cplx, a, b, M, N, K, rs, tile) < 1e-14, (M, N, K, a, b)
