@@ -3,8 +3,11 @@
 // registers and LDS so that the rank can be 32 (16 complex).
 //
 // Each flush G += A B^T reads and writes both spins' G (4 N^2 values per
-// chain); with all chains flushing, that streams at HBM speed, and the number
-// of flushes falls as 1/KD.  One 256-thread workgroup per chain, both spins,
+// chain) and runs 4 N^2 KD flops on one CU's matrix cores; the number of
+// flushes falls as 1/KD.  Per chain the time splits into the flushes (about
+// 40%, bound by the CU's FP64 matrix rate and the latency of its G traffic)
+// and the serial per-acceptance chain (site scan, column and row fetch of G,
+// the gemv over the filled slots).  One 256-thread workgroup per chain, both spins,
 // thread t owns rows j = t + 256 r (r < R; R = 2 up to N = 512, rank 16):
 //   * its row of A (A[j][0..KD)) lives in registers -- the flush needs exactly
 //     that row, and a new column enters at a run-time slot by selects;
@@ -14,7 +17,8 @@
 //   * on an acceptance at site i the owner of row i publishes A[i][0..k)
 //     through LDS.
 // Lane 0 scans the precomputed site order from LDS; only an acceptance
-// touches G.  Slots past the filled ones hold zeros and are skipped.
+// touches G.  Slots past the filled ones hold zeros; the gemv walks the slots
+// in unbranched chunks up to the one holding the last filled slot.
 #pragma once
 #include "dq_common.hpp"
 #include "dq_gemm.hpp"
@@ -34,12 +38,108 @@
 // (Operand layouts as in dq_gemm.hpp.)  A VALU flush (thread j updating its
 // row) needs one LDS broadcast of B per FMA and is bound by the LDS return
 // path; this one needs one LDS read per 64 FMAs per lane.
-template <int KD, int W>
+// The chunks are software-pipelined: the next chunk's G values are requested
+// before the current chunk's MFMAs, so each wave keeps its matrix core busy
+// while the loads are in flight (one wave per SIMD: nothing else hides the
+// memory latency).  A rank-32 flush of both spins of a 256-site chain is
+// 8.4 MFLOP, about 30 us of one CU's FP64 matrix rate.
+// A chunk is in range when its 64 rows and 16 MB columns all lie inside the
+// matrix (uniform per wave); then its loads and stores are straight-line code,
+// which lets the next chunk's loads stay in flight across this chunk's MFMAs.
+template <int MB>
+__device__ inline void upd3_chunk_load(int N, int rl, int lane, int c0, bool full, const double* G,
+                                       dbl4 (&acc)[MB][4])
+{
+    if (full) {
+        const double* g = G + rl + N * (c0 + (lane >> 4));
+#pragma unroll
+        for (int mb = 0; mb < MB; mb++)
+#pragma unroll
+            for (int b = 0; b < 4; b++)
+#pragma unroll
+                for (int e = 0; e < 4; e++) acc[mb][b][e] = g[16 * b + N * (16 * mb + 4 * e)];
+        return;
+    }
+#pragma unroll
+    for (int mb = 0; mb < MB; mb++)
+#pragma unroll
+        for (int b = 0; b < 4; b++)
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                const int r = rl + 16 * b, cc = c0 + 16 * mb + 4 * e + (lane >> 4);
+                acc[mb][b][e] = (r < N && cc < N) ? G[r + N * cc] : 0.0;
+            }
+}
+
+template <int KD, int W, int MB>
+__device__ inline void upd3_chunk_mfma(int lane, int c0, const double (&af)[4][KD / 4],
+                                       const double (*B)[W], int kused, dbl4 (&acc)[MB][4])
+{
+#pragma unroll
+    for (int kb = 0; kb < KD / 4; kb++) {
+        if (4 * kb < kused) {  // uniform
+#pragma unroll
+            for (int mb = 0; mb < MB; mb++) {
+                const int cc = (c0 + 16 * mb + (lane & 15)) & (W - 1);
+                const double bf = B[4 * kb + (lane >> 4)][cc];
+#pragma unroll
+                for (int b = 0; b < 4; b++) acc[mb][b] = mfma(bf, af[b][kb], acc[mb][b]);
+            }
+        }
+    }
+}
+
+// Stores a chunk; when the chunk's columns meet the wave's rows (dg, uniform)
+// the diagonal elements (at most one per row b of a lane) go to d[] after the
+// stores, selected without branches.
+template <int MB>
+__device__ inline void upd3_chunk_store(int N, int rl, int lane, int c0, bool full, bool dg,
+                                        double* G, double* d, const dbl4 (&acc)[MB][4])
+{
+    double dv[4] = {0.0, 0.0, 0.0, 0.0};
+    bool df[4] = {false, false, false, false};
+    if (full) {
+        double* g = G + rl + N * (c0 + (lane >> 4));
+#pragma unroll
+        for (int mb = 0; mb < MB; mb++)
+#pragma unroll
+            for (int b = 0; b < 4; b++)
+#pragma unroll
+                for (int e = 0; e < 4; e++) g[16 * b + N * (16 * mb + 4 * e)] = acc[mb][b][e];
+    } else {
+#pragma unroll
+        for (int mb = 0; mb < MB; mb++)
+#pragma unroll
+            for (int b = 0; b < 4; b++)
+#pragma unroll
+                for (int e = 0; e < 4; e++) {
+                    const int r = rl + 16 * b, cc = c0 + 16 * mb + 4 * e + (lane >> 4);
+                    if (r < N && cc < N) G[r + N * cc] = acc[mb][b][e];
+                }
+    }
+    if (!dg) return;
+#pragma unroll
+    for (int mb = 0; mb < MB; mb++)
+#pragma unroll
+        for (int b = 0; b < 4; b++)
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                const int r = rl + 16 * b, cc = c0 + 16 * mb + 4 * e + (lane >> 4);
+                const bool on = (r == cc);
+                dv[b] = on ? acc[mb][b][e] : dv[b];
+                df[b] = df[b] || on;
+            }
+#pragma unroll
+    for (int b = 0; b < 4; b++)
+        if (df[b] && rl + 16 * b < N) d[rl + 16 * b] = dv[b];
+}
+
+template <int KD, int W, int MB, bool PIPE>
 __device__ inline void upd3_flush_mfma(int N, int j, int row0, double* G, const double (&A)[KD],
                                        const double (*B)[W], int kused, double* d)
 {
     constexpr int KB = KD / 4;
-    const int lane = j & 63, w = j >> 6;
+    const int lane = j & 63, w = __builtin_amdgcn_readfirstlane(j >> 6);  // wave-uniform
     double af[4][KB];
 #pragma unroll
     for (int b = 0; b < 4; b++) {
@@ -56,55 +156,127 @@ __device__ inline void upd3_flush_mfma(int N, int j, int row0, double* G, const 
         }
     }
     const int rl = row0 + 64 * w + (lane & 15);
-    constexpr int MB = 2;  // 16-column blocks per chunk
-    for (int c0 = 0; c0 < N; c0 += 16 * MB) {
-        dbl4 acc[MB][4];
-#pragma unroll
-        for (int mb = 0; mb < MB; mb++)
-#pragma unroll
-            for (int b = 0; b < 4; b++)
-#pragma unroll
-                for (int e = 0; e < 4; e++) {
-                    const int r = rl + 16 * b, cc = c0 + 16 * mb + 4 * e + (lane >> 4);
-                    acc[mb][b][e] = (r < N && cc < N) ? G[r + (long)N * cc] : 0.0;
-                }
-#pragma unroll
-        for (int kb = 0; kb < KB; kb++) {
-            if (4 * kb < kused) {  // uniform
-#pragma unroll
-                for (int mb = 0; mb < MB; mb++) {
-                    const int cc = (c0 + 16 * mb + (lane & 15)) & (W - 1);
-                    const double bf = B[4 * kb + (lane >> 4)][cc];
-#pragma unroll
-                    for (int b = 0; b < 4; b++) acc[mb][b] = mfma(bf, af[b][kb], acc[mb][b]);
-                }
-            }
+    constexpr int CW = 16 * MB;  // MB 16-column blocks per chunk
+    const bool rows_in = row0 + 64 * w + 64 <= N;  // uniform per wave
+    auto full = [&](int c0) { return rows_in && c0 + CW <= N; };
+    const int wr0 = row0 + 64 * w;  // the wave's rows [wr0, wr0 + 64)
+    auto dg = [&](int c0) { return c0 < wr0 + 64 && wr0 < c0 + CW; };
+    dbl4 x0[MB][4], x1[MB][4];
+    if constexpr (!PIPE) {  // one chunk in flight (two waves per SIMD: 256 registers)
+        for (int c0 = 0; c0 < N; c0 += CW) {
+            upd3_chunk_load<MB>(N, rl, lane, c0, full(c0), G, x0);
+            upd3_chunk_mfma<KD, W, MB>(lane, c0, af, B, kused, x0);
+            upd3_chunk_store<MB>(N, rl, lane, c0, full(c0), dg(c0), G, d, x0);
         }
-#pragma unroll
-        for (int mb = 0; mb < MB; mb++)
-#pragma unroll
-            for (int b = 0; b < 4; b++)
-#pragma unroll
-                for (int e = 0; e < 4; e++) {
-                    const int r = rl + 16 * b, cc = c0 + 16 * mb + 4 * e + (lane >> 4);
-                    if (r < N && cc < N) {
-                        G[r + (long)N * cc] = acc[mb][b][e];
-                        if (r == cc) d[r] = acc[mb][b][e];
-                    }
-                }
+        return;
+    }
+    upd3_chunk_load<MB>(N, rl, lane, 0, full(0), G, x0);
+    // two chunks per trip, ping-ponging between the register sets
+    for (int c0 = 0; c0 < N; c0 += 2 * CW) {
+        if (c0 + CW < N) upd3_chunk_load<MB>(N, rl, lane, c0 + CW, full(c0 + CW), G, x1);
+        upd3_chunk_mfma<KD, W, MB>(lane, c0, af, B, kused, x0);
+        upd3_chunk_store<MB>(N, rl, lane, c0, full(c0), dg(c0), G, d, x0);
+        if (c0 + CW >= N) break;
+        if (c0 + 2 * CW < N) upd3_chunk_load<MB>(N, rl, lane, c0 + 2 * CW, full(c0 + 2 * CW), G, x0);
+        upd3_chunk_mfma<KD, W, MB>(lane, c0 + CW, af, B, kused, x1);
+        upd3_chunk_store<MB>(N, rl, lane, c0 + CW, full(c0 + CW), dg(c0 + CW), G, d, x1);
     }
 }
 
 // The complex flush on the matrix cores: as upd3_flush_mfma with the real and
 // imaginary parts as separate real operands, four MFMAs per block,
 //   re += Br Ar - Bi Ai,   im += Br Ai + Bi Ar,
-// one 16-column block per chunk (the accumulators are twice the real ones).
+// one 16-column block per chunk (the accumulators are twice the real ones),
+// pipelined the same way.
+__device__ inline void upd3z_chunk_load(int N, int rl, int lane, int c0, bool full, const zd* G,
+                                        dbl4 (&cr)[4], dbl4 (&ci)[4])
+{
+    if (full) {
+        const zd* g = G + rl + N * (c0 + (lane >> 4));
+#pragma unroll
+        for (int b = 0; b < 4; b++)
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                const zd v = g[16 * b + N * 4 * e];
+                cr[b][e] = v.x;
+                ci[b][e] = v.y;
+            }
+        return;
+    }
+#pragma unroll
+    for (int b = 0; b < 4; b++)
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+            const int r = rl + 16 * b, cc = c0 + 4 * e + (lane >> 4);
+            const zd v = (r < N && cc < N) ? G[r + N * cc] : mkz(0.0, 0.0);
+            cr[b][e] = v.x;
+            ci[b][e] = v.y;
+        }
+}
+
+template <int KD, int W>
+__device__ inline void upd3z_chunk_mfma(int lane, int c0, const double (&ar)[4][KD / 4],
+                                        const double (&ai)[4][KD / 4], const zd (*B)[W],
+                                        int kused, dbl4 (&cr)[4], dbl4 (&ci)[4])
+{
+#pragma unroll
+    for (int kb = 0; kb < KD / 4; kb++) {
+        if (4 * kb < kused) {  // uniform
+            const zd bz = B[4 * kb + (lane >> 4)][(c0 + (lane & 15)) & (W - 1)];
+            const double br = bz.x, bi = bz.y;
+#pragma unroll
+            for (int b = 0; b < 4; b++) {
+                cr[b] = mfma(br, ar[b][kb], cr[b]);
+                cr[b] = mfma(-bi, ai[b][kb], cr[b]);
+                ci[b] = mfma(br, ai[b][kb], ci[b]);
+                ci[b] = mfma(bi, ar[b][kb], ci[b]);
+            }
+        }
+    }
+}
+
+__device__ inline void upd3z_chunk_store(int N, int rl, int lane, int c0, bool full, bool dg, zd* G,
+                                         zd* d, const dbl4 (&cr)[4], const dbl4 (&ci)[4])
+{
+    if (full) {
+        zd* g = G + rl + N * (c0 + (lane >> 4));
+#pragma unroll
+        for (int b = 0; b < 4; b++)
+#pragma unroll
+            for (int e = 0; e < 4; e++) g[16 * b + N * 4 * e] = mkz(cr[b][e], ci[b][e]);
+    } else {
+#pragma unroll
+        for (int b = 0; b < 4; b++)
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                const int r = rl + 16 * b, cc = c0 + 4 * e + (lane >> 4);
+                if (r < N && cc < N) G[r + N * cc] = mkz(cr[b][e], ci[b][e]);
+            }
+    }
+    if (!dg) return;
+    double dr[4] = {0.0, 0.0, 0.0, 0.0}, di[4] = {0.0, 0.0, 0.0, 0.0};
+    bool df[4] = {false, false, false, false};
+#pragma unroll
+    for (int b = 0; b < 4; b++)
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+            const int r = rl + 16 * b, cc = c0 + 4 * e + (lane >> 4);
+            const bool on = (r == cc);
+            dr[b] = on ? cr[b][e] : dr[b];
+            di[b] = on ? ci[b][e] : di[b];
+            df[b] = df[b] || on;
+        }
+#pragma unroll
+    for (int b = 0; b < 4; b++)
+        if (df[b] && rl + 16 * b < N) d[rl + 16 * b] = mkz(dr[b], di[b]);
+}
+
 template <int KD, int W>
 __device__ inline void upd3_flush_mfma_z(int N, int j, int row0, zd* G, const zd (&A)[KD],
                                          const zd (*B)[W], int kused, zd* d)
 {
     constexpr int KB = KD / 4;
-    const int lane = j & 63, w = j >> 6;
+    const int lane = j & 63, w = __builtin_amdgcn_readfirstlane(j >> 6);  // wave-uniform
     double ar[4][KB], ai[4][KB];
 #pragma unroll
     for (int b = 0; b < 4; b++) {
@@ -126,42 +298,21 @@ __device__ inline void upd3_flush_mfma_z(int N, int j, int row0, zd* G, const zd
         }
     }
     const int rl = row0 + 64 * w + (lane & 15);
-    for (int c0 = 0; c0 < N; c0 += 16) {
-        dbl4 cr[4], ci[4];
-#pragma unroll
-        for (int b = 0; b < 4; b++)
-#pragma unroll
-            for (int e = 0; e < 4; e++) {
-                const int r = rl + 16 * b, cc = c0 + 4 * e + (lane >> 4);
-                const zd g = (r < N && cc < N) ? G[r + (long)N * cc] : mkz(0.0, 0.0);
-                cr[b][e] = g.x;
-                ci[b][e] = g.y;
-            }
-#pragma unroll
-        for (int kb = 0; kb < KB; kb++) {
-            if (4 * kb < kused) {  // uniform
-                const zd bz = B[4 * kb + (lane >> 4)][(c0 + (lane & 15)) & (W - 1)];
-                const double br = bz.x, bi = bz.y;
-#pragma unroll
-                for (int b = 0; b < 4; b++) {
-                    cr[b] = mfma(br, ar[b][kb], cr[b]);
-                    cr[b] = mfma(-bi, ai[b][kb], cr[b]);
-                    ci[b] = mfma(br, ai[b][kb], ci[b]);
-                    ci[b] = mfma(bi, ar[b][kb], ci[b]);
-                }
-            }
-        }
-#pragma unroll
-        for (int b = 0; b < 4; b++)
-#pragma unroll
-            for (int e = 0; e < 4; e++) {
-                const int r = rl + 16 * b, cc = c0 + 4 * e + (lane >> 4);
-                if (r < N && cc < N) {
-                    const zd g = mkz(cr[b][e], ci[b][e]);
-                    G[r + (long)N * cc] = g;
-                    if (r == cc) d[r] = g;
-                }
-            }
+    constexpr int CW = 16;
+    const bool rows_in = row0 + 64 * w + 64 <= N;  // uniform per wave
+    auto full = [&](int c0) { return rows_in && c0 + CW <= N; };
+    const int wr0 = row0 + 64 * w;
+    auto dg = [&](int c0) { return c0 < wr0 + 64 && wr0 < c0 + CW; };
+    dbl4 r0[4], i0[4], r1[4], i1[4];
+    upd3z_chunk_load(N, rl, lane, 0, full(0), G, r0, i0);
+    for (int c0 = 0; c0 < N; c0 += 2 * CW) {
+        if (c0 + CW < N) upd3z_chunk_load(N, rl, lane, c0 + CW, full(c0 + CW), G, r1, i1);
+        upd3z_chunk_mfma<KD, W>(lane, c0, ar, ai, B, kused, r0, i0);
+        upd3z_chunk_store(N, rl, lane, c0, full(c0), dg(c0), G, d, r0, i0);
+        if (c0 + CW >= N) break;
+        if (c0 + 2 * CW < N) upd3z_chunk_load(N, rl, lane, c0 + 2 * CW, full(c0 + 2 * CW), G, r0, i0);
+        upd3z_chunk_mfma<KD, W>(lane, c0 + CW, ar, ai, B, kused, r1, i1);
+        upd3z_chunk_store(N, rl, lane, c0 + CW, full(c0 + CW), dg(c0 + CW), G, d, r1, i1);
     }
 }
 
@@ -170,6 +321,8 @@ template <typename T, int R, int KD, int NT = UPD3_THREADS>
 __global__ __launch_bounds__(NT) void k_update_lds(Upd2Args<T> a)
 {
     constexpr int W = NT * R;
+    constexpr int MC = KD >= 32 ? 8 : 4;  // slots per unbranched chunk
+    static_assert(KD % MC == 0 && (MC & (MC - 1)) == 0, "delay rank");
     __shared__ T du[W], dd[W];
     __shared__ int hsl[W], ordl[W];
     __shared__ double ul[W], dl0[W], dl1[W];
@@ -243,14 +396,21 @@ __global__ __launch_bounds__(NT) void k_update_lds(Upd2Args<T> a)
             __syncthreads();
             const int i = s_site;
             if (i < 0) break;
-            // the owner of row i publishes its filled slots of A
+            // the owner of row i publishes its slots of A in chunks of MC, the
+            // filled ones and the zeros after them up to the chunk's end
             if (tid == (i & (NT - 1))) {
 #pragma unroll
                 for (int r = 0; r < R; r++)
                     if (r == i / NT) {
 #pragma unroll
-                        for (int m = 0; m < KD; m++)
-                            if (m < k) { rAu[m] = Au[r][m]; rAd[m] = Ad[r][m]; }
+                        for (int mc = 0; mc < KD; mc += MC)
+                            if (mc < k) {  // uniform
+#pragma unroll
+                                for (int m = mc; m < mc + MC; m++) {
+                                    rAu[m] = Au[r][m];
+                                    rAd[m] = Ad[r][m];
+                                }
+                            }
                     }
             }
             T au[R], bu[R], ad[R], bd[R];
@@ -266,18 +426,24 @@ __global__ __launch_bounds__(NT) void k_update_lds(Upd2Args<T> a)
                 }
             }
             __syncthreads();
-            // a_k = G[:, i] + A B[i, :]^T,  b_k = G[i, :]^T + B A[i, :]^T
+            // a_k = G[:, i] + A B[i, :]^T,  b_k = G[i, :]^T + B A[i, :]^T, over the
+            // slots in chunks of MC with no branch inside a chunk, so that a
+            // chunk's LDS reads are in flight together: the slots from k to the
+            // chunk's end are zero in A, B and the published row, and add 0
 #pragma unroll
-            for (int m = 0; m < KD; m++) {
-                if (m < k) {  // uniform
-                    const T bui = Bu[m][i], bdi = Bd[m][i], rau = rAu[m], rad = rAd[m];
+            for (int mc = 0; mc < KD; mc += MC) {
+                if (mc < k) {  // uniform
 #pragma unroll
-                    for (int r = 0; r < R; r++) {
-                        const int j = tid + NT * r;
-                        au[r] += Au[r][m] * bui;
-                        bu[r] += Bu[m][j] * rau;
-                        ad[r] += Ad[r][m] * bdi;
-                        bd[r] += Bd[m][j] * rad;
+                    for (int m = mc; m < mc + MC; m++) {
+                        const T bui = Bu[m][i], bdi = Bd[m][i], rau = rAu[m], rad = rAd[m];
+#pragma unroll
+                        for (int r = 0; r < R; r++) {
+                            const int j = tid + NT * r;
+                            au[r] += Au[r][m] * bui;
+                            bu[r] += Bu[m][j] * rau;
+                            ad[r] += Ad[r][m] * bdi;
+                            bd[r] += Bd[m][j] * rad;
+                        }
                     }
                 }
             }
@@ -295,8 +461,12 @@ __global__ __launch_bounds__(NT) void k_update_lds(Upd2Args<T> a)
                     au[r] = zero; bu[r] = zero; ad[r] = zero; bd[r] = zero;
                 }
 #pragma unroll
-                for (int m = 0; m < KD; m++)
-                    if (m == k) { Au[r][m] = au[r]; Ad[r][m] = ad[r]; }
+                for (int mc = 0; mc < KD; mc += MC)
+                    if ((k & ~(MC - 1)) == mc) {  // uniform: only k's chunk selects
+#pragma unroll
+                        for (int m = mc; m < mc + MC; m++)
+                            if (m == k) { Au[r][m] = au[r]; Ad[r][m] = ad[r]; }
+                    }
                 Bu[k][j] = bu[r];
                 Bd[k][j] = bd[r];
             }
@@ -309,8 +479,10 @@ __global__ __launch_bounds__(NT) void k_update_lds(Upd2Args<T> a)
                     upd3_flush_mfma_z<KD, W>(N, tid, NT * r, Gu, Au[r], Bu, k, du);
                     upd3_flush_mfma_z<KD, W>(N, tid, NT * r, Gd, Ad[r], Bd, k, dd);
                 } else {  // every lane takes part (the A fragments are gathered)
-                    upd3_flush_mfma<KD, W>(N, tid, NT * r, Gu, Au[r], Bu, k, du);
-                    upd3_flush_mfma<KD, W>(N, tid, NT * r, Gd, Ad[r], Bd, k, dd);
+                    // pipelined 32-column chunks with one wave per SIMD; two
+                    // waves per SIMD (256 registers each) keep one chunk
+                    upd3_flush_mfma<KD, W, 2, (NT <= 256)>(N, tid, NT * r, Gu, Au[r], Bu, k, du);
+                    upd3_flush_mfma<KD, W, 2, (NT <= 256)>(N, tid, NT * r, Gd, Ad[r], Bd, k, dd);
                 }
             }
             __syncthreads();
