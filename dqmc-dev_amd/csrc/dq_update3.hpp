@@ -16,9 +16,10 @@
 //     conflicts;
 //   * on an acceptance at site i the owner of row i publishes A[i][0..k)
 //     through LDS.
-// Lane 0 scans the precomputed site order from LDS; only an acceptance
-// touches G.  Slots past the filled ones hold zeros; the gemv walks the slots
-// in unbranched chunks up to the one holding the last filled slot.
+// Wave 0 scans the precomputed site order from LDS, 64 positions at a time;
+// only an acceptance touches G.  Slots past the filled ones hold zeros; the
+// gemv walks the slots in unbranched chunks up to the one holding the last
+// filled slot, from zero while G's column and row are in flight.
 #pragma once
 #include "dq_common.hpp"
 #include "dq_gemm.hpp"
@@ -355,7 +356,7 @@ __global__ __launch_bounds__(NT) void k_update_lds(Upd2Args<T> a)
     T phase = zero;
     unsigned long long nacc = 0;
     if (tid == 0) phase = a.phase[c];
-    int ii = 0;  // lane 0's scan position
+    int ii = 0;  // wave 0's scan position
     T Au[R][KD], Ad[R][KD];
     for (;;) {
 #pragma unroll
@@ -368,30 +369,54 @@ __global__ __launch_bounds__(NT) void k_update_lds(Upd2Args<T> a)
         __syncthreads();
         int k = 0;
         for (; k < KD; k++) {
-            if (tid == 0) {
+            // the scan: wave 0 tests 64 positions of the site order at once
+            // with the current diagonal and takes the first accepting one
+            // (the draws do not depend on acceptances); its lane computes the
+            // factors, flips the field and hands the phase factor to lane 0
+            if (tid < 64) {
                 int site = -1;
-                for (; ii < N; ii++) {
-                    const int i = ordl[ii];
-                    const int sv = hsl[i];
-                    const double delu = sv ? dl1[i] : dl0[i];
-                    const double deld = sv ? dl0[i] : dl1[i];
-                    if (delu == 0.0 && deld == 0.0) continue;
-                    const T ru = one + (one - du[i]) * delu;
-                    const T rd = one + (one - dd[i]) * deld;
-                    const T prob = ru * rd;
-                    const double absprob = absv(prob);
-                    if (ul[ii] < absprob) {
-                        site = i;
+                while (ii < N) {  // ii is uniform over wave 0
+                    const int pos = ii + tid;
+                    bool acc = false;
+                    int i = 0, sv = 0;
+                    double delu = 0.0, deld = 0.0, absprob = 1.0;
+                    T ru = one, rd = one, prob = one;
+                    if (pos < N) {
+                        i = ordl[pos];
+                        sv = hsl[i];
+                        delu = sv ? dl1[i] : dl0[i];
+                        deld = sv ? dl0[i] : dl1[i];
+                        if (!(delu == 0.0 && deld == 0.0)) {
+                            ru = one + (one - du[i]) * delu;
+                            rd = one + (one - dd[i]) * deld;
+                            prob = ru * rd;
+                            absprob = absv(prob);
+                            acc = ul[pos] < absprob;
+                        }
+                    }
+                    const unsigned long long bal = __ballot(acc);
+                    if (bal == 0ull) {
+                        ii += 64;
+                        continue;
+                    }
+                    const int first = __ffsll((long long)bal) - 1;  // uniform
+                    T fac = zero;
+                    if (tid == first) {
                         s_fu = rdiv(from_re<T>(delu), ru);
                         s_fd = rdiv(from_re<T>(deld), rd);
                         hsl[i] = 1 - sv;
-                        phase = phase * (prob * (1.0 / absprob));
-                        nacc++;
-                        ii++;
-                        break;
+                        fac = prob * (1.0 / absprob);
                     }
+                    site = __builtin_amdgcn_readlane(i, first);
+                    fac = bcast(fac, first);
+                    if (tid == 0) {
+                        phase = phase * fac;
+                        nacc++;
+                    }
+                    ii += first + 1;
+                    break;
                 }
-                s_site = site;
+                if (tid == 0) s_site = site;
             }
             __syncthreads();
             const int i = s_site;
@@ -413,16 +438,18 @@ __global__ __launch_bounds__(NT) void k_update_lds(Upd2Args<T> a)
                             }
                     }
             }
-            T au[R], bu[R], ad[R], bd[R];
+            // G's column and row of site i; the gemv below runs from zero while
+            // these loads are in flight, and adds them at the end
+            T gau[R], gbu[R], gad[R], gbd[R];
 #pragma unroll
             for (int r = 0; r < R; r++) {
                 const int j = tid + NT * r;
-                au[r] = zero; bu[r] = zero; ad[r] = zero; bd[r] = zero;
+                gau[r] = zero; gbu[r] = zero; gad[r] = zero; gbd[r] = zero;
                 if (j < N) {
-                    au[r] = Gu[j + (long)N * i];
-                    bu[r] = Gu[i + (long)N * j];
-                    ad[r] = Gd[j + (long)N * i];
-                    bd[r] = Gd[i + (long)N * j];
+                    gau[r] = Gu[j + (long)N * i];
+                    gbu[r] = Gu[i + (long)N * j];
+                    gad[r] = Gd[j + (long)N * i];
+                    gbd[r] = Gd[i + (long)N * j];
                 }
             }
             __syncthreads();
@@ -430,6 +457,9 @@ __global__ __launch_bounds__(NT) void k_update_lds(Upd2Args<T> a)
             // slots in chunks of MC with no branch inside a chunk, so that a
             // chunk's LDS reads are in flight together: the slots from k to the
             // chunk's end are zero in A, B and the published row, and add 0
+            T au[R], bu[R], ad[R], bd[R];
+#pragma unroll
+            for (int r = 0; r < R; r++) { au[r] = zero; bu[r] = zero; ad[r] = zero; bd[r] = zero; }
 #pragma unroll
             for (int mc = 0; mc < KD; mc += MC) {
                 if (mc < k) {  // uniform
@@ -451,6 +481,7 @@ __global__ __launch_bounds__(NT) void k_update_lds(Upd2Args<T> a)
 #pragma unroll
             for (int r = 0; r < R; r++) {
                 const int j = tid + NT * r;
+                au[r] += gau[r]; bu[r] += gbu[r]; ad[r] += gad[r]; bd[r] += gbd[r];
                 if (j == i) { au[r] -= one; ad[r] -= one; }
                 au[r] = au[r] * fu;
                 ad[r] = ad[r] * fd;
