@@ -546,38 +546,33 @@ __global__ __launch_bounds__(256) void k_gensus_outer(MeasArgs<T> a, int ue)
     acc[a.off[ku + 3] + o] += pre * (Ad * Bu);
 }
 
-// determinant of the principal submatrix of a 4x4 matrix selected by mask
-__device__ inline double det_mask(const double (&M)[4][4], int mask)
+// All 16 principal minors of a 4x4 matrix, mn[mask] = det of the rows and
+// columns in mask (mn[0] = 1), by cofactor expansion with every index fixed at
+// compile time, so that the matrix and the minors stay in registers.
+__device__ inline void principal_minors4(const double (&M)[4][4], double (&mn)[16])
 {
-    int id[4], k = 0;
+    mn[0] = 1.0;
+#pragma unroll
+    for (int i = 0; i < 4; i++) mn[1 << i] = M[i][i];
 #pragma unroll
     for (int i = 0; i < 4; i++)
-        if (mask >> i & 1) id[k++] = i;
-    if (k == 0) return 1.0;
-    double A[4][4];
-    for (int i = 0; i < k; i++)
-        for (int j = 0; j < k; j++) A[i][j] = M[id[i]][id[j]];
-    double det = 1.0;
-    for (int q = 0; q < k; q++) {
-        int pv = q;
-        for (int i = q + 1; i < k; i++)
-            if (fabs(A[i][q]) > fabs(A[pv][q])) pv = i;
-        if (A[pv][q] == 0.0) return 0.0;
-        if (pv != q) {
-            for (int j = 0; j < k; j++) {
-                const double t = A[q][j];
-                A[q][j] = A[pv][j];
-                A[pv][j] = t;
-            }
-            det = -det;
-        }
-        det *= A[q][q];
-        for (int i = q + 1; i < k; i++) {
-            const double f = A[i][q] / A[q][q];
-            for (int j = q; j < k; j++) A[i][j] -= f * A[q][j];
-        }
-    }
-    return det;
+#pragma unroll
+        for (int j = i + 1; j < 4; j++) mn[(1 << i) | (1 << j)] = M[i][i] * M[j][j] - M[i][j] * M[j][i];
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = i + 1; j < 4; j++)
+#pragma unroll
+            for (int k = j + 1; k < 4; k++)
+                mn[(1 << i) | (1 << j) | (1 << k)] =
+                    M[i][i] * (M[j][j] * M[k][k] - M[j][k] * M[k][j]) -
+                    M[i][j] * (M[j][i] * M[k][k] - M[j][k] * M[k][i]) +
+                    M[i][k] * (M[j][i] * M[k][j] - M[j][j] * M[k][i]);
+    // Laplace expansion along rows 0 and 1
+    auto s2 = [&](int a_, int b_) { return M[0][a_] * M[1][b_] - M[0][b_] * M[1][a_]; };
+    auto c2 = [&](int a_, int b_) { return M[2][a_] * M[3][b_] - M[2][b_] * M[3][a_]; };
+    mn[15] = s2(0, 1) * c2(2, 3) - s2(0, 2) * c2(1, 3) + s2(0, 3) * c2(1, 2) +
+             s2(1, 2) * c2(0, 3) - s2(1, 3) * c2(0, 2) + s2(2, 3) * c2(0, 1);
 }
 
 // nematic (real build only, src/meas.c:518-715): over the first 2N bonds,
@@ -616,7 +611,9 @@ __global__ __launch_bounds__(256) void k_meas_ue_nematic(MeasArgs<double> a)
             const double* got = a.G0t + bo + NN * tt;
             const double* gt0 = a.Gt0 + bo + NN * tt;
             double M[4][4];
+#pragma unroll
             for (int u = 0; u < 4; u++)
+#pragma unroll
                 for (int w = 0; w < 4; w++) {
                     const long xu = site[u], xw = site[w];
                     double e;
@@ -627,9 +624,10 @@ __global__ __launch_bounds__(256) void k_meas_ue_nematic(MeasArgs<double> a)
                     else e = -gt0[xw + N * xu];
                     M[u][w] = e;
                 }
-            for (int m = 0; m < 16; m++) minor[s][m] = det_mask(M, m);
+            principal_minors4(M, minor[s]);
         }
         // bit a of mask = 1: operator a has spin up
+#pragma unroll
         for (int m = 0; m < 16; m++) {
             const double v = minor[0][m] * minor[1][15 - m];
             sn += v;
