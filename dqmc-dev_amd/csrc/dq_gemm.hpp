@@ -407,7 +407,7 @@ static hipError_t gemm_launch(int opa, int opb, const GemmArgs<T>& g, int batch,
 {
     constexpr bool Z = is_cplx<T>::value;
     if constexpr (!Z) {
-        if (tile == GT_W && gemm_w_fits(opa, opb, g)) return gemm_w_launch(opa, g, batch, s);
+        if (tile == GT_W && gemm_w_fits(opa, opb, g)) return gemm_w_launch(opa, opb, g, batch, s);
     }
     if (tile == GT_128 && g.M > 64)  // 128 x 64 with K steps of 16 (8 complex)
         return gemm_launch_tile<T, 4, 2, Z ? 8 : 16, 1>(opa, opb, g, batch, s);

@@ -1,9 +1,9 @@
 // dq_gemm16.hpp -- real products on v_mfma_f64_16x16x4f64 with 128-bit LDS
 // traffic (tile GT_W of dq_gemm.hpp's launcher).
 //
-//   C[b] = alpha * diag(r) * op(A) * diag(u) * B * diag(w) + beta * C,
-//   op(A) = A or A^T, B op N, M % BM == 0 (BM = 128 or 64), N % 64 == 0,
-//   K % 16 == 0, 16-byte aligned columns.
+//   C[b] = alpha * diag(r) * op(A) * diag(u) * op(B) * diag(w) + beta * C,
+//   op(A) = A or A^T, op(B) = B or B^T, M % BM == 0 (BM = 128 or 64),
+//   N % 64 == 0, K % 16 == 0, 16-byte aligned columns.
 //
 // 256 threads per BM x 64 tile of C, four waves in a 2 x 2 arrangement, each
 // wave (BM / 2) x 32 = MI x 2 blocks of 16 x 16.  K advances in steps of 16
@@ -25,7 +25,7 @@
 // (row) with two 16-byte loads into its slot of both regions (u is applied to
 // B here).  For op(A) = A the rows are m-contiguous: a thread loads its row's
 // four k columns (four 8-byte loads, the 16 lanes of a quarter reading 128
-// contiguous bytes of each).
+// contiguous bytes of each); op(B) = B^T is loaded the same way.
 #pragma once
 #include "dq_common.hpp"
 // included by dq_gemm.hpp after GemmArgs
@@ -33,8 +33,9 @@
 #define GW_BN 64
 #define GW_BK 16
 
-// MI: 16-row blocks per wave along M (BM = 32 MI); TA: op(A) = A^T
-template <int MI, bool TA, bool BETA>
+// MI: 16-row blocks per wave along M (BM = 32 MI); TA: op(A) = A^T;
+// TBT: op(B) = B^T
+template <int MI, bool TA, bool BETA, bool TBT = false>
 __global__ __launch_bounds__(256, 3) void k_gemm_w(GemmArgs<double> g, int tiles_m, int tiles_n,
                                                    int nwg)
 {
@@ -74,8 +75,9 @@ __global__ __launch_bounds__(256, 3) void k_gemm_w(GemmArgs<double> g, int tiles
     // A loads: rows 16 (TB wave + t) + l16, k = 4q .. 4q + 3
     const double* Ag = TA ? A + 4 * q + (long)g.lda * (m0 + 16 * TB * wave + l16)
                           : A + (m0 + 16 * TB * wave + l16) + (long)g.lda * (4 * q);
-    // B loads: column 16 wave + l16, k = 4q .. 4q + 3
-    const double* Bg = B + 4 * q + (long)g.ldb * (n0 + 16 * wave + l16);
+    // B loads: column 16 wave + l16 of op(B), k = 4q .. 4q + 3
+    const double* Bg = TBT ? B + (n0 + 16 * wave + l16) + (long)g.ldb * (4 * q)
+                           : B + 4 * q + (long)g.ldb * (n0 + 16 * wave + l16);
 
     double ra[TB][4];
     double2 rb[2];
@@ -92,9 +94,17 @@ __global__ __launch_bounds__(256, 3) void k_gemm_w(GemmArgs<double> g, int tiles
                 for (int c = 0; c < 4; c++) ra[t][c] = Ag[16 * t + (long)g.lda * (k0 + c)];
             }
         }
-        const double* bb = Bg + k0;
-        rb[0] = *(const double2*)(bb);
-        rb[1] = *(const double2*)(bb + 2);
+        if constexpr (TBT) {
+            double x[4];
+#pragma unroll
+            for (int c = 0; c < 4; c++) x[c] = Bg[(long)g.ldb * (k0 + c)];
+            rb[0] = make_double2(x[0], x[1]);
+            rb[1] = make_double2(x[2], x[3]);
+        } else {
+            const double* bb = Bg + k0;
+            rb[0] = *(const double2*)(bb);
+            rb[1] = *(const double2*)(bb + 2);
+        }
         if (u) {
             ub0 = *(const double2*)(u + k0 + 4 * q);
             ub1 = *(const double2*)(u + k0 + 4 * q + 2);
@@ -186,14 +196,15 @@ __global__ __launch_bounds__(256, 3) void k_gemm_w(GemmArgs<double> g, int tiles
 static inline bool gemm_w_fits(int opa, int opb, const GemmArgs<double>& g)
 {
     const auto al = [](const void* p) { return ((unsigned long)p & 15) == 0; };
-    return opb == 0 && (opa == 0 || !g.idLc) && g.M % 64 == 0 && g.N % GW_BN == 0 &&
+    return (opa == 0 || opb == 0) && ((opa == 0 && opb == 0) || !g.idLc) && g.M % 64 == 0 &&
+           g.N % GW_BN == 0 &&
            g.K % GW_BK == 0 && g.M > 0 && g.N > 0 && g.K > 0 && g.lda % 2 == 0 &&
            g.ldb % 2 == 0 && g.sA % 2 == 0 && g.sB % 2 == 0 && al(g.A) && al(g.B) &&
            (!g.u || (al(g.u) && g.su % 2 == 0));
 }
 
 // 128-row tiles where M allows, else 64
-static inline hipError_t gemm_w_launch(int opa, const GemmArgs<double>& g, int batch,
+static inline hipError_t gemm_w_launch(int opa, int opb, const GemmArgs<double>& g, int batch,
                                        hipStream_t s)
 {
     const int bm = g.M % 128 == 0 ? 128 : 64;
@@ -202,15 +213,18 @@ static inline hipError_t gemm_w_launch(int opa, const GemmArgs<double>& g, int b
     if (nwg <= 0) return hipSuccess;
     if (nwg > 0x7fffffffL) return hipErrorInvalidValue;
     const int n = (int)nwg;
-#define GWL(MI, TA)                                                            \
-    (g.beta_zero ? k_gemm_w<MI, TA, false><<<n, 256, 0, s>>>(g, tm, tn, n) \
-                 : k_gemm_w<MI, TA, true><<<n, 256, 0, s>>>(g, tm, tn, n))
-    if (opa == 0) {
-        if (bm == 128) GWL(4, false);
-        else GWL(2, false);
+#define GWL(MI, TA, TB)                                                            \
+    (g.beta_zero ? k_gemm_w<MI, TA, false, TB><<<n, 256, 0, s>>>(g, tm, tn, n) \
+                 : k_gemm_w<MI, TA, true, TB><<<n, 256, 0, s>>>(g, tm, tn, n))
+    if (opb != 0) {  // op(A) = A only
+        if (bm == 128) GWL(4, false, true);
+        else GWL(2, false, true);
+    } else if (opa == 0) {
+        if (bm == 128) GWL(4, false, false);
+        else GWL(2, false, false);
     } else {
-        if (bm == 128) GWL(4, true);
-        else GWL(2, true);
+        if (bm == 128) GWL(4, true, false);
+        else GWL(2, true, false);
     }
 #undef GWL
     return hipGetLastError();

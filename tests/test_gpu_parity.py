@@ -59,8 +59,9 @@ def _gemm(cplx, opa, opb, M, N, K, rs, tile=0, scaled=True):
 def test_gemm_kernel(cplx, tile):
     """Every tile shape, every op pair, ragged edges (sizes not multiples of
     the tile or of the K step), and the unscaled beta = 0 path.  Tile 4 (the
-    16x16x4 form) takes the real products op(A) B of multiples of 64 x 64 x 16
-    (128-row tiles where M allows) and hands the rest to the 64 x 64 tile."""
+    16x16x4 form) takes the real products op(A) op(B), at most one of them
+    transposed, of multiples of 64 x 64 x 16 (128-row tiles where M allows) and
+    hands the rest to the 64 x 64 tile."""
     rs = np.random.RandomState(0)
     ops = (0, 1, 2) if cplx else (0, 1)
     for (M, N, K) in [(16, 16, 16), (70, 33, 45), (256, 256, 256), (1, 130, 7), (200, 144, 77),
