@@ -409,6 +409,10 @@ def main():
                     help="--config 4: fixed n_matmul per bucket instead of the autotuner")
     ap.add_argument("--threads", type=int, default=8,
                     help="host threads driving the per-shape engines of --config 4")
+    ap.add_argument("--engines", type=int, default=2,
+                    help="--config 2: split the chains over this many engines (own HIP streams, "
+                         "driven from one host thread each): while one engine's Metropolis "
+                         "kernel holds one CU per chain, the other's products use the rest")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (chains shard trivially across GPUs, "
@@ -452,9 +456,48 @@ def main():
         eng.close()
         return dt, acc, gstats, tile
 
+    def timed_split(nch, ne, steps, warmup):
+        """The chains split over ne engines run concurrently, one host thread
+        each.  Every engine replays the graphs of both sweep patterns (with and
+        without the unequal-time measurement), captured before the threads
+        start: stream capture fails while another thread makes HIP calls."""
+        import threading
+        per = [nch // ne + (1 if i < nch % ne else 0) for i in range(ne)]
+        sims = workload(nch, **(ALL_MEAS if args.all_meas else {}))
+        engs, k = [], 0
+        for n in per:
+            eng = engine.Engine(sims[k:k + n], device=0, gemm_tile=args.gemm_tile,
+                                graphs=not args.no_graphs)
+            k += n
+            eng.run(0)  # initial G outside the timed region
+            eng.run(warmup)
+            engs.append(eng)
+        tile = engs[0].gemm_tile()
+        acc0 = sum(e.accepts() for e in engs)
+        ths = [threading.Thread(target=e.run, args=(steps,)) for e in engs]
+        t0 = time.perf_counter()
+        for t in ths:
+            t.start()
+        for t in ths:
+            t.join()
+        dt = time.perf_counter() - t0
+        acc = sum(e.accepts() for e in engs) - acc0
+        for e in engs:
+            e.close()
+        return dt, acc, tile
+
     nch = args.chains
-    # the production path: every sweep one HIP-graph launch (eager with --no-graphs)
-    dt, acc, _, tile = timed(nch, args.steps, args.warmup, False)
+    # the production path: every sweep one HIP-graph launch per engine (eager
+    # with --no-graphs); the chains in args.engines engines run concurrently
+    ne = max(1, min(args.engines, nch))
+    if ne > 1:
+        # two warmup sweeps at least, so that both sweep patterns are captured
+        # before the engines run concurrently
+        warm = max(args.warmup, 2)
+        dt, acc, tile = timed_split(nch, ne, args.steps, warm)
+    else:
+        warm = args.warmup
+        dt, acc, _, tile = timed(nch, args.steps, args.warmup, False)
     value = nch * args.steps / dt
     # the same workload launched kernel by kernel with a HIP event pair around
     # every N x N x N GEMM on the engine stream: the dominant kernel's average
@@ -479,7 +522,7 @@ def main():
         "unit": "sweeps/s",
         "n_gpus": 1,
         "steps": args.steps,
-        "warmup": args.warmup,
+        "warmup": warm,
         "ms_per_step": round(dt / args.steps * 1e3, 3),
         "higher_is_better": True,
         "scaling": "weak",
@@ -492,7 +535,8 @@ def main():
                                + (" + bond, thermal, 2-bond, energy, nematic and chiral groups "
                                   "(profiling run, not the headline workload)"
                                   if args.all_meas else ""),
-                   "chains": nch, "parallelism": "1 GPU, chains batched"},
+                   "chains": nch, "parallelism": "1 GPU, chains batched",
+                   "engines": ne},
         "graphs": not args.no_graphs,
         "eager_timed_gemms_sweeps_per_s": round(nch * args.steps / dte, 4),
         "single_chain_sweeps_per_s": None if single is None else round(single, 4),
