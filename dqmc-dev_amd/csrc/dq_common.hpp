@@ -87,6 +87,27 @@ __device__ inline double readlane_d(double v, int l)
     return __longlong_as_double((long long)(unsigned)lo | ((long long)hi << 32));
 }
 
+// Elementwise kernels over a batch of m x n column-major blocks run on the grid
+// (row blocks, columns, batch entries), so that a thread's row i, column j and
+// entry b come from its indices with no integer division (64-bit divisions of
+// a flat index made these kernels instruction-bound).
+// Each thread takes row i of EW_COLS consecutive columns j (EW_FOR_J).
+#define EW_COLS 4
+#define EW_IB(m_)                                        \
+    const int i = blockIdx.x * blockDim.x + threadIdx.x; \
+    const int b = blockIdx.z;                            \
+    if (i >= (m_)) return
+#define EW_FOR_J(n_)                                                                  \
+    for (int j = blockIdx.y * EW_COLS, j_end_ = (j + EW_COLS < (n_) ? j + EW_COLS : (n_)); \
+         j < j_end_; j++)
+static inline int ew_threads(int m) { return m > 128 ? 256 : (m > 64 ? 128 : 64); }
+static inline dim3 ew_grid(int m, int n, int batch)
+{
+    const int t = ew_threads(m);
+    return dim3((unsigned)((m + t - 1) / t), (unsigned)((n + EW_COLS - 1) / EW_COLS),
+                (unsigned)batch);
+}
+
 // --- wave64 reductions in registers (no LDS round trips) -------------------
 // gfx950 lane exchanges:  permlane32_swap(x, y) -> ([x lanes 0-31, y lanes
 // 0-31], [x lanes 32-63, y lanes 32-63]);  permlane16_swap(x, y) ->

@@ -10,14 +10,8 @@ template <typename T>
 __global__ void k_copy_mn(int m, int n, const T* src, long ssrc, int lds, T alpha, T* dst,
                           long sdst, int ldd, int batch)
 {
-    const long mn = (long)m * n;
-    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < mn * batch;
-         e += (long)gridDim.x * blockDim.x) {
-        const int b = e / mn;
-        const long r = e % mn;
-        const int i = r % m, j = r / m;
-        dst[b * sdst + i + (long)ldd * j] = alpha * src[b * ssrc + i + (long)lds * j];
-    }
+    EW_IB(m);
+    EW_FOR_J(n) dst[b * sdst + i + (long)ldd * j] = alpha * src[b * ssrc + i + (long)lds * j];
 }
 
 // dst[b] (n x n, ldd) = upper triangle of src[b] (lds), zeros below
@@ -25,15 +19,9 @@ template <typename T>
 __global__ void k_copy_upper(int n, const T* src, long ssrc, int lds, T* dst, long sdst, int ldd,
                              int batch)
 {
-    const long nn = (long)n * n;
-    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < nn * batch;
-         e += (long)gridDim.x * blockDim.x) {
-        const int b = e / nn;
-        const long r = e % nn;
-        const int i = r % n, j = r / n;
-        dst[b * sdst + i + (long)ldd * j] =
-            (i <= j) ? src[b * ssrc + i + (long)lds * j] : from_re<T>(0.0);
-    }
+    EW_IB(n);
+    EW_FOR_J(n)
+        dst[b * sdst + i + (long)ldd * j] = (i <= j) ? src[b * ssrc + i + (long)lds * j] : from_re<T>(0.0);
 }
 
 // upper triangle (incl. diagonal) of src written into dst; dst's strictly
@@ -42,40 +30,24 @@ template <typename T>
 __global__ void k_store_upper(int n, const T* src, long ssrc, int lds, T* dst, long sdst, int ldd,
                               int batch)
 {
-    const long nn = (long)n * n;
-    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < nn * batch;
-         e += (long)gridDim.x * blockDim.x) {
-        const int b = e / nn;
-        const long r = e % nn;
-        const int i = r % n, j = r / n;
-        if (i <= j) dst[b * sdst + i + (long)ldd * j] = src[b * ssrc + i + (long)lds * j];
-    }
+    EW_IB(n);
+    EW_FOR_J(n) if (i <= j) dst[b * sdst + i + (long)ldd * j] = src[b * ssrc + i + (long)lds * j];
 }
 
 // zero the strictly lower part of an m x n panel (ld)
 template <typename T>
 __global__ void k_zero_lower(int m, int n, T* A, long sA, int lda, int batch)
 {
-    const long mn = (long)m * n;
-    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < mn * batch;
-         e += (long)gridDim.x * blockDim.x) {
-        const int b = e / mn;
-        const long r = e % mn;
-        const int i = r % m, j = r / m;
-        if (i > j) A[b * sA + i + (long)lda * j] = from_re<T>(0.0);
-    }
+    EW_IB(m);
+    EW_FOR_J(n) if (i > j) A[b * sA + i + (long)lda * j] = from_re<T>(0.0);
 }
 
 // A[b] (n x n, lda) = alpha * I + beta_keep * A  (beta_keep 0 or 1)
 template <typename T>
 __global__ void k_set_identity(int n, T* A, long sA, int lda, int keep, int batch)
 {
-    const long nn = (long)n * n;
-    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < nn * batch;
-         e += (long)gridDim.x * blockDim.x) {
-        const int b = e / nn;
-        const long r = e % nn;
-        const int i = r % n, j = r / n;
+    EW_IB(n);
+    EW_FOR_J(n) {
         T* p = A + b * sA + i + (long)lda * j;
         const T d = from_re<T>(i == j ? 1.0 : 0.0);
         *p = keep ? (*p + d) : d;
@@ -87,12 +59,9 @@ template <typename T>
 __global__ void k_rowscale(int N, const T* A, long sA, int adiv, const double* s, long ss, T* X,
                            long sX, int batch)
 {
-    const long NN = (long)N * N;
-    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < NN * batch;
-         e += (long)gridDim.x * blockDim.x) {
-        const int b = e / NN;
-        const long r = e % NN;
-        const int i = r % N;
+    EW_IB(N);
+    EW_FOR_J(N) {
+        const long r = i + (long)N * j;
         X[b * sX + r] = s[b * ss + i] * A[(b / adiv) * sA + r];
     }
 }

@@ -58,10 +58,10 @@ static int launch_trsm_upper(dqmc_engine* e, int n, int ncols, const T* U, long 
 template <typename T>
 static int trtri_upper(dqmc_engine* e, int n, T* X, long sX, int ldx, T* W, long sW, int B)
 {
-    k_set_identity<T><<<nblk((long)n * n * B), 256, 0, e->st>>>(n, W, sW, n, 0, B);
+    k_set_identity<T><<<ew_grid(n, n, B), ew_threads(n), 0, e->st>>>(n, W, sW, n, 0, B);
     LCHK();
     RC(trsm_upper_blocked<T>(e, n, n, X, ldx, sX, W, n, sW, B));
-    k_store_upper<T><<<nblk((long)n * n * B), 256, 0, e->st>>>(n, W, sW, n, X, sX, ldx, B);
+    k_store_upper<T><<<ew_grid(n, n, B), ew_threads(n), 0, e->st>>>(n, W, sW, n, X, sX, ldx, B);
     LCHK();
     return 0;
 }
@@ -115,10 +115,10 @@ struct UeSteps {
     int inverse(T* C, int l) const  // C (stride LNN) = B_l^-1
     {
         if (half)
-            k_copy_mn<T><<<nblk(NN * B), 256, 0, e->st>>>(N, N, dp<T>(e->Bthi) + l * NN, LNN, N,
+            k_copy_mn<T><<<ew_grid(N, N, B), ew_threads(N), 0, e->st>>>(N, N, dp<T>(e->Bthi) + l * NN, LNN, N,
                                                            one, C, LNN, N, B);
         else
-            k_rowscale<T><<<nblk(NN * B), 256, 0, e->st>>>(N, dp<T>(e->iexpK), NN, 1,
+            k_rowscale<T><<<ew_grid(N, N, B), ew_threads(N), 0, e->st>>>(N, dp<T>(e->iexpK), NN, 1,
                                                           e->iscale + (long)N * l, LN, C, LNN, B);
         LCHK();
         return 0;
@@ -202,7 +202,7 @@ static int ue_expand(dqmc_engine* e, T* Gred, long sU2, bool half)
     const int lstop_first = (rstop_last + 1) % L;
     const int dstop_last = rstop_last, ustop_first = lstop_first;
     auto cpy = [&](const T* src, T* dst) -> int {
-        k_copy_mn<T><<<nblk(NN * B), 256, 0, e->st>>>(N, N, src, sU2, NE, one, dst, LNN, N, B);
+        k_copy_mn<T><<<ew_grid(N, N, B), ew_threads(N), 0, e->st>>>(N, N, src, sU2, NE, one, dst, LNN, N, B);
         LCHK();
         if (half) RC(half_wrap_slice<T>(e, dst));
         return 0;
@@ -263,7 +263,7 @@ static int ue_expand(dqmc_engine* e, T* Gred, long sU2, bool half)
             const T alpha = (next == 0) ? mone : one;
             RC(st.left(Gt0 + next * NN, LNN, Gt0 + m * NN, LNN, m, false, alpha, zero));
             if (next == 0) {
-                k_set_identity<T><<<nblk(NN * B), 256, 0, e->st>>>(N, Gt0 + next * NN, LNN, N, 1, B);
+                k_set_identity<T><<<ew_grid(N, N, B), ew_threads(N), 0, e->st>>>(N, Gt0 + next * NN, LNN, N, 1, B);
                 LCHK();
             }
             m = next;
@@ -299,7 +299,7 @@ static int ue_calc(dqmc_engine* e, bool half)
     for (int q = 0; q < E - 1; q++)
         RC(mul_seq_C<T>(e, q * 2, (q + 1) * 2, mone, BLK(q + 1, q), NE, sU2));
     RC(mul_seq_C<T>(e, (E - 1) * 2, 0, one, BLK(0, E - 1), NE, sU2));
-    k_set_identity<T><<<nblk((long)NE * NE * B), 256, 0, e->st>>>(NE, Gred, sU2, NE, 1, B);
+    k_set_identity<T><<<ew_grid(NE, NE, B), ew_threads(NE), 0, e->st>>>(NE, Gred, sU2, NE, 1, B);
     LCHK();
     ps_o.stop();
 
@@ -307,7 +307,7 @@ static int ue_calc(dqmc_engine* e, bool half)
     {
     ProfSec ps_b(e, PS_bsofi);
     if (E == 1) {
-        k_set_identity<T><<<nblk(NN * B), 256, 0, e->st>>>(N, Wa, sU2, N, 0, B);
+        k_set_identity<T><<<ew_grid(N, N, B), ew_threads(N), 0, e->st>>>(N, Wa, sU2, N, 0, B);
         LCHK();
         k_getrf<T><<<B, 1024, 0, e->st>>>(N, Gred, Wa, sU2, e->ipiv, N);
         LCHK();
@@ -323,7 +323,7 @@ static int ue_calc(dqmc_engine* e, bool half)
         else GRS(16);
 #undef GRS
         LCHK();
-        k_copy_mn<T><<<nblk(NN * B), 256, 0, e->st>>>(N, N, Wa, sU2, N, one, Gred, sU2, NE, B);
+        k_copy_mn<T><<<ew_grid(N, N, B), ew_threads(N), 0, e->st>>>(N, N, Wa, sU2, N, one, Gred, sU2, NE, B);
         LCHK();
     } else {
         for (int l = 0; l < E - 2; l++) {
@@ -335,7 +335,7 @@ static int ue_calc(dqmc_engine* e, bool half)
                 T* P = BLK(l, cols[t]);
                 RC(gemm<T>(e, OP_C, OP_N, N2, N, N2, one, Ql, N2, sQb, P, NL, sU2, zero, Wa, N2,
                            sU2, B));
-                k_copy_mn<T><<<nblk((long)N2 * N * B), 256, 0, e->st>>>(N2, N, Wa, sU2, N2, one, P,
+                k_copy_mn<T><<<ew_grid(N2, N, B), ew_threads(N2), 0, e->st>>>(N2, N, Wa, sU2, N2, one, P,
                                                                        sU2, NL, B);
                 LCHK();
             }
@@ -350,55 +350,55 @@ static int ue_calc(dqmc_engine* e, bool half)
             RC(trtri_upper<T>(e, 3 * N, BLK(E - 3, E - 3), sU2, NL, Wa, sU2, B));
             if (E > 3) {
                 const int m3 = N * (E - 3);
-                k_copy_upper<T><<<nblk(NN * B), 256, 0, e->st>>>(N, BLK(E - 1, E - 1), sU2, NL, Wb,
+                k_copy_upper<T><<<ew_grid(N, N, B), ew_threads(N), 0, e->st>>>(N, BLK(E - 1, E - 1), sU2, NL, Wb,
                                                                 sU2, N, B);
                 LCHK();
                 RC(gemm<T>(e, OP_N, OP_N, m3, N, N, one, BLK(0, E - 1), NL, sU2, Wb, N, sU2, zero,
                            Wa, m3, sU2, B));
-                k_copy_mn<T><<<nblk((long)m3 * N * B), 256, 0, e->st>>>(m3, N, Wa, sU2, m3, one,
+                k_copy_mn<T><<<ew_grid(m3, N, B), ew_threads(m3), 0, e->st>>>(m3, N, Wa, sU2, m3, one,
                                                                        BLK(0, E - 1), sU2, NL, B);
                 LCHK();
                 for (int l = E - 4; l >= 0; l--) {
                     RC(trtri_upper<T>(e, N, BLK(l, l), sU2, NL, Wa, sU2, B));
-                    k_copy_upper<T><<<nblk(NN * B), 256, 0, e->st>>>(N, BLK(l, l), sU2, NL, Wb,
+                    k_copy_upper<T><<<ew_grid(N, N, B), ew_threads(N), 0, e->st>>>(N, BLK(l, l), sU2, NL, Wb,
                                                                     sU2, N, B);
                     LCHK();
                     const int cols[2] = {E - 1, l + 1};
                     for (int t = 0; t < 2; t++) {
                         RC(gemm<T>(e, OP_N, OP_N, N, N, N, mone, Wb, N, sU2, BLK(l, cols[t]), NL,
                                    sU2, zero, Wa, N, sU2, B));
-                        k_copy_mn<T><<<nblk(NN * B), 256, 0, e->st>>>(N, N, Wa, sU2, N, one,
+                        k_copy_mn<T><<<ew_grid(N, N, B), ew_threads(N), 0, e->st>>>(N, N, Wa, sU2, N, one,
                                                                      BLK(l, cols[t]), sU2, NL, B);
                         LCHK();
                     }
                     RC(gemm<T>(e, OP_N, OP_N, N, N * (E - l - 2), N, one, BLK(l, l + 1), NL, sU2,
                                BLK(l + 1, l + 2), NL, sU2, one, BLK(l, l + 2), NL, sU2, B));
-                    k_copy_upper<T><<<nblk(NN * B), 256, 0, e->st>>>(N, BLK(l + 1, l + 1), sU2, NL,
+                    k_copy_upper<T><<<ew_grid(N, N, B), ew_threads(N), 0, e->st>>>(N, BLK(l + 1, l + 1), sU2, NL,
                                                                     Wb, sU2, N, B);
                     LCHK();
                     RC(gemm<T>(e, OP_N, OP_N, N, N, N, one, BLK(l, l + 1), NL, sU2, Wb, N, sU2,
                                zero, Wa, N, sU2, B));
-                    k_copy_mn<T><<<nblk(NN * B), 256, 0, e->st>>>(N, N, Wa, sU2, N, one,
+                    k_copy_mn<T><<<ew_grid(N, N, B), ew_threads(N), 0, e->st>>>(N, N, Wa, sU2, N, one,
                                                                  BLK(l, l + 1), sU2, NL, B);
                     LCHK();
                 }
             }
         }
         // multiply by Q^H from the right
-        k_zero_lower<T><<<nblk(4 * NN * B), 256, 0, e->st>>>(N2, N2, BLK(E - 2, E - 2), sU2, NL, B);
+        k_zero_lower<T><<<ew_grid(N2, N2, B), ew_threads(N2), 0, e->st>>>(N2, N2, BLK(E - 2, E - 2), sU2, NL, B);
         LCHK();
         RC(gemm<T>(e, OP_N, OP_C, NL, N2, N2, one, BLK(0, E - 2), NL, sU2, Qlast, N2, sQb, zero,
                    Wa, NL, sU2, B));
-        k_copy_mn<T><<<nblk((long)NL * N2 * B), 256, 0, e->st>>>(NL, N2, Wa, sU2, NL, one,
+        k_copy_mn<T><<<ew_grid(NL, N2, B), ew_threads(NL), 0, e->st>>>(NL, N2, Wa, sU2, NL, one,
                                                                 BLK(0, E - 2), sU2, NL, B);
         LCHK();
         for (int l = E - 3; l >= 0; l--) {
             T* Ql = Qb + (long)l * 4 * NN;
-            k_zero_lower<T><<<nblk(2 * NN * B), 256, 0, e->st>>>(N2, N, BLK(l, l), sU2, NL, B);
+            k_zero_lower<T><<<ew_grid(N2, N, B), ew_threads(N2), 0, e->st>>>(N2, N, BLK(l, l), sU2, NL, B);
             LCHK();
             RC(gemm<T>(e, OP_N, OP_C, NL, N2, N2, one, BLK(0, l), NL, sU2, Ql, N2, sQb, zero, Wa,
                        NL, sU2, B));
-            k_copy_mn<T><<<nblk((long)NL * N2 * B), 256, 0, e->st>>>(NL, N2, Wa, sU2, NL, one,
+            k_copy_mn<T><<<ew_grid(NL, N2, B), ew_threads(NL), 0, e->st>>>(NL, N2, Wa, sU2, NL, one,
                                                                     BLK(0, l), sU2, NL, B);
             LCHK();
         }

@@ -207,15 +207,11 @@ template <typename T>
 __global__ void k_colscale(int N, const T* A, long sA, int adiv, const double* s, long ss, T* X,
                            long sX, int batch, const int* Lc = nullptr, int l = 0)
 {
-    const long NN = (long)N * N;
-    const long total = NN * batch;
-    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total;
-         e += (long)gridDim.x * blockDim.x) {
-        const int b = e / NN;
-        const long r = e % NN;
-        const int j = r / N;
+    EW_IB(N);
+    EW_FOR_J(N) {
+        const long r = i + (long)N * j;
         if (Lc && l >= Lc[b >> 1])
-            X[b * sX + r] = from_re<T>((r % N) == j ? 1.0 : 0.0);
+            X[b * sX + r] = from_re<T>(i == j ? 1.0 : 0.0);
         else
             X[b * sX + r] = A[(b / adiv) * sA + r] * s[b * ss + j];
     }

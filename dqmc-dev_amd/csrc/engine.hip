@@ -385,14 +385,8 @@ template <typename T>
 __global__ void k_copy_block(int N, const T* src, long ssrc, int lds, T alpha, T* dst, long sdst,
                              int ldd, int batch)
 {
-    const long NN = (long)N * N;
-    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < NN * batch;
-         e += (long)gridDim.x * blockDim.x) {
-        const int b = e / NN;
-        const long r = e % NN;
-        const int i = r % N, j = r / N;
-        dst[b * sdst + i + (long)ldd * j] = alpha * src[b * ssrc + i + (long)lds * j];
-    }
+    EW_IB(N);
+    EW_FOR_J(N) dst[b * sdst + i + (long)ldd * j] = alpha * src[b * ssrc + i + (long)lds * j];
 }
 
 // --------------------------------------------------------------------------
@@ -414,7 +408,7 @@ static int build_C(dqmc_engine* e, int f0, int nf)
         int cur = ((n - 1) % 2 == 0) ? 0 : 1;  // the last product lands in C_f
         {
             ProfSec ps(e, PS_calcb);
-            k_colscale<T><<<nblk(NN * B), 256, 0, e->st>>>(N, dp<T>(e->expK), NN, 1,
+            k_colscale<T><<<ew_grid(N, N, B), ew_threads(N), 0, e->st>>>(N, dp<T>(e->expK), NN, 1,
                                                           e->escale + (long)N * l0, LN, bufs[cur],
                                                           (long)F * NN, B,
                                                           e->ragged ? e->d_Lc : nullptr, l0);
@@ -441,7 +435,7 @@ static int mul_seq_C(dqmc_engine* e, int min, int maxp1, T alpha, T* out, int ld
     const T* C = dp<T>(e->C);
     const int n_mul = (min == maxp1) ? F : (F + maxp1 - min) % F;
     if (n_mul == 1) {
-        k_copy_block<T><<<nblk(NN * B), 256, 0, e->st>>>(N, C + min * NN, sC, N, alpha, out, so,
+        k_copy_block<T><<<ew_grid(N, N, B), ew_threads(N), 0, e->st>>>(N, C + min * NN, sC, N, alpha, out, so,
                                                         ldo, B);
         LCHK();
         return 0;
@@ -492,7 +486,7 @@ static int accumulate_q(dqmc_engine* e, int m, int kmax, int NB, T* Q, int ldq, 
     const long sW = e->sWq;
     const T one = from_re<T>(1.0), mone = from_re<T>(-1.0), zero = from_re<T>(0.0);
     const int np = (kmax + NB - 1) / NB;
-    k_set_identity<T><<<nblk((long)m * m * batch), 256, 0, e->st>>>(m, Q, sQ, ldq, 0, batch);
+    k_set_identity<T><<<ew_grid(m, m, batch), ew_threads(m), 0, e->st>>>(m, Q, sQ, ldq, 0, batch);
     LCHK();
     for (int p = np - 1; p >= 0; p--) {
         const int k0 = p * NB;
@@ -739,7 +733,7 @@ static int calc_eq_g(dqmc_engine* e, int l, T* Gout, T* phase_b)
     RC(qr_blocked<T>(e, N, N, Mm, N, NN, tau, N, X, N, NN, detm, B));
     RC(gemm<T>(e, OP_C, OP_N, N, N, N, one, X, N, NN, Gout, N, NN, zero, Y, N, NN, B));
     RC(trsm_upper_blocked<T>(e, N, N, Mm, N, NN, Y, N, NN, B));
-    k_copy_block<T><<<nblk(NN * B), 256, 0, e->st>>>(N, Y, NN, N, one, Gout, NN, N, B);
+    k_copy_block<T><<<ew_grid(N, N, B), ew_threads(N), 0, e->st>>>(N, Y, NN, N, one, Gout, NN, N, B);
     LCHK();
     k_eqg_phase<T><<<B, 64, 0, e->st>>>(N, detq, detm, e->vvec, N, Mm, NN, phase_b);
     LCHK();
