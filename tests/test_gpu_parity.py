@@ -363,6 +363,45 @@ def test_graph_replay_matches_eager(nflux):
         ee.close()
 
 
+def test_engines_replaying_graphs_concurrently_match_one_engine():
+    """The chains split over two engines whose sweeps run at the same time from
+    two host threads (each engine on its own stream, its graphs captured
+    before the threads start) give the chains of one engine of all of them,
+    bit for bit: the way bench.py runs the headline batch."""
+    import threading
+    sims = gen.create_batch(Nfiles=4, seed=17, Nx=6, Ny=6, U=4.0, mu=-0.4, L=20, n_matmul=5,
+                            period_eqlt=5, period_uneqlt=2, n_sweep_warm=2, n_sweep_meas=20)
+    # one GEMM tile for all three (the timed tile choice could differ by batch)
+    one = engine.Engine(sims, gemm_tile=1)
+    parts = [engine.Engine(sims[:2], gemm_tile=1), engine.Engine(sims[2:], gemm_tile=1)]
+    try:
+        one.run(8)
+        for e in parts:
+            e.run(2)  # both sweep patterns captured, one engine at a time
+        ths = [threading.Thread(target=e.run, args=(6,)) for e in parts]
+        for t in ths:
+            t.start()
+        for t in ths:
+            t.join()
+        for c in range(len(sims)):
+            e, cc = parts[c // 2], c % 2
+            so, sp = one.get_state(c), e.get_state(cc)
+            assert np.array_equal(so["hs"], sp["hs"]) and np.array_equal(so["rng"], sp["rng"]), c
+            go, _ = one.green(c, 0)
+            gp, _ = e.green(cc, 0)
+            assert np.array_equal(go, gp), c
+            qo, uo = one.get_meas(c)
+            qp, up = e.get_meas(cc)
+            for k in ("density", "nn", "zz"):
+                assert np.array_equal(qo[k], qp[k]), (c, k)
+            for k in ("gt0", "nn"):
+                assert np.array_equal(uo[k], up[k]), (c, k)
+    finally:
+        one.close()
+        for e in parts:
+            e.close()
+
+
 def test_set_params_per_chain_and_every_chain_match_fresh_engines():
     """dqmc_engine_set_params on one chain, and on every chain at once (chain
     -1: one upload copied on the device), gives the chains of an engine built
