@@ -51,6 +51,10 @@ struct GemmArgs {
     // other operand, scales ignored (an identity slice of a padded chain);
     // idLc null: none.  Square products with op N only.
     const int* idLc; int idl; int idside;
+    // 1: op(A) = A is upper triangular (zeros below the diagonal), so that a
+    // tile of rows m0.. needs only k >= m0 (the 16x16x4 kernel skips the rest;
+    // the other kernels compute the zeros)
+    int a_upper;
 };
 
 // Tile shapes (rows x columns of C per 256-thread workgroup).  The engine's

@@ -152,10 +152,12 @@ __global__ __launch_bounds__(256, 3) void k_gemm_w(GemmArgs<double> g, int tiles
     };
 
     const int nk = g.K / GW_BK;
-    gload(0);
-    sstore(0);
+    // an upper-triangular A contributes nothing below row m0's diagonal
+    const int kt0 = (!TA && g.a_upper) ? (m0 < g.K ? m0 : g.K) / GW_BK : 0;
+    gload(kt0 * GW_BK);
+    sstore(kt0 & 1);
     __syncthreads();
-    for (int kt = 0; kt < nk; kt++) {
+    for (int kt = kt0; kt < nk; kt++) {
         const int cur = kt & 1;
         const bool more = kt + 1 < nk;
         if (more) gload((kt + 1) * GW_BK);

@@ -346,9 +346,11 @@ template <typename T>
 static int gemm(dqmc_engine* e, int opa, int opb, int M, int N, int K, T alpha, const T* A, int lda,
                 long sA, const T* B, int ldb, long sB, T beta, T* C, int ldc, long sC, int batch,
                 const double* r = nullptr, long sr = 0, const double* u = nullptr, long su = 0,
-                const double* w = nullptr, long sw = 0, int id_l = -1, int id_side = 0)
+                const double* w = nullptr, long sw = 0, int id_l = -1, int id_side = 0,
+                int a_upper = 0)
 {
     GemmArgs<T> g;
+    g.a_upper = a_upper;
     // B_l of an identity slice (l >= the chain's own L): the product by it is
     // a copy of the other operand (batch entry b belongs to chain b / 2)
     g.idLc = (e->ragged && id_l >= 0) ? e->d_Lc : nullptr;
@@ -723,7 +725,9 @@ static int calc_eq_g(dqmc_engine* e, int l, T* Gout, T* phase_b)
         k_eqg_post_qr<T><<<dim3(N, B), 256, 0, e->st>>>(N, Z, NN, e->perm, N, Tm, NN, e->dvec, N,
                                                        Rs, Tp, NN);
         LCHK();
-        RC(gemm<T>(e, OP_N, OP_N, N, N, N, one, Rs, N, NN, Tp, N, NN, zero, Tm, N, NN, B));
+        // Rs is upper triangular: its product skips the zero blocks
+        RC(gemm<T>(e, OP_N, OP_N, N, N, N, one, Rs, N, NN, Tp, N, NN, zero, Tm, N, NN, B, nullptr,
+                   0, nullptr, 0, nullptr, 0, -1, 0, 1));
     }
     // Eq. 2.12: G = M'^-1 diag(v) Q^H with M' = diag(ds) T + diag(v) Q^H, solved
     // stably through the QR of M' (M' = Q_M R_M, G = R_M^-1 Q_M^H RHS)
