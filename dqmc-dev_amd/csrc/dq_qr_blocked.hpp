@@ -244,6 +244,45 @@ __global__ __launch_bounds__(256) void k_trsm_diag64(int nk, int ncols, const T*
         if (lane < nk && c0 + q < ncols) X[lane + (long)ldx * (c0 + q)] = x[q];
 }
 
+// The same solve for a full diagonal block of order 64 of a real U, one thread
+// per right-hand side: the thread's column of X_k in registers, U_kk staged
+// once per block in LDS with coalesced loads and read back with uniform
+// addresses (broadcasts), the reciprocals of its diagonal computed once, back
+// substitution as straight-line FMAs.  The lane-per-row form above spends a
+// broadcast, a select and a masked update per column and step.
+constexpr int TRSM_COLS = 256;  // right-hand sides per block
+__global__ __launch_bounds__(TRSM_COLS) void k_trsm_diag64_cols(int ncols, const double* U_,
+                                                                long sU, int ldu, double* X_,
+                                                                long sX, int ldx)
+{
+    __shared__ double Us[64 * 64];  // column-major, leading dimension 64
+    __shared__ double dinv[64];
+    const int b = blockIdx.y;
+    const double* U = U_ + b * sU;
+    double* X = X_ + b * sX;
+    const int t = threadIdx.x;
+    const int c = blockIdx.x * TRSM_COLS + t;
+    const long cl = c < ncols ? c : ncols - 1;  // in-bounds column for the load
+    double x[64];
+#pragma unroll
+    for (int i = 0; i < 64; i++) x[i] = X[i + ldx * cl];
+    for (int q = t; q < 64 * 64; q += TRSM_COLS) Us[q] = U[(q & 63) + (long)ldu * (q >> 6)];
+    __syncthreads();
+    if (t < 64) dinv[t] = 1.0 / Us[t + 64 * t];
+    __syncthreads();
+#pragma unroll
+    for (int k = 63; k >= 0; k--) {
+        const double xk = x[k] * dinv[k];
+        x[k] = xk;
+#pragma unroll
+        for (int i = 0; i < k; i++) x[i] -= Us[i + 64 * k] * xk;
+    }
+    if (c < ncols) {
+#pragma unroll
+        for (int i = 0; i < 64; i++) X[i + ldx * (long)c] = x[i];
+    }
+}
+
 // phase of the determinant of a product of Householder reflectors stored
 // LAPACK-style (V below the diagonal of A, tau): prod_i (1 - tau_i v_i^H v_i)
 template <typename T>

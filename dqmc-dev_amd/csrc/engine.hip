@@ -630,6 +630,7 @@ template <typename T>
 static int trsm_upper_blocked(dqmc_engine* e, int n, int ncols, const T* U, int ldu, long sU, T* X,
                               int ldx, long sX, int batch)
 {
+    if (n <= 0 || ncols <= 0) return 0;
     const int BS = 64;
     const T one = from_re<T>(1.0), mone = from_re<T>(-1.0);
     const int nblocks = (n + BS - 1) / BS;
@@ -639,6 +640,15 @@ static int trsm_upper_blocked(dqmc_engine* e, int n, int ncols, const T* U, int 
         if (k0 + nk < n)
             RC(gemm<T>(e, OP_N, OP_N, nk, ncols, n - k0 - nk, mone, U + k0 + (long)ldu * (k0 + nk),
                        ldu, sU, X + k0 + nk, ldx, sX, one, X + k0, ldx, sX, batch));
+        if constexpr (!is_cplx<T>::value) {
+            if (nk == 64) {  // one thread per right-hand side
+                k_trsm_diag64_cols<<<dim3((ncols + TRSM_COLS - 1) / TRSM_COLS, batch), TRSM_COLS,
+                                     0, e->st>>>(
+                    ncols, U + k0 + (long)ldu * k0, sU, ldu, X + k0, sX, ldx);
+                LCHK();
+                continue;
+            }
+        }
         dim3 grid((ncols + 4 * LU_NC - 1) / (4 * LU_NC), batch);
         k_trsm_diag64<T><<<grid, 256, 0, e->st>>>(nk, ncols, U + k0 + (long)ldu * k0, sU, ldu,
                                                   X + k0, sX, ldx);
