@@ -23,9 +23,13 @@
 // Global -> LDS: B's columns, and op(A)'s rows for op(A) = A^T, are
 // k-contiguous in memory, so a thread moves k = 4q .. 4q + 3 of one column
 // (row) with two 16-byte loads into its slot of both regions (u is applied to
-// B here).  For op(A) = A the rows are m-contiguous: a thread loads its row's
-// four k columns (four 8-byte loads, the 16 lanes of a quarter reading 128
-// contiguous bytes of each); op(B) = B^T is loaded the same way.
+// B here).  For op(A) = A the rows are m-contiguous: a thread loads a 2 x 2
+// block (rows 2p, 2p + 1 of columns k, k + 1: two 16-byte loads, the 8 lanes
+// of a column pair reading 128 contiguous bytes), which is exactly the
+// (k, k + 1) pair of two row slots; the lanes whose k pair is odd write their
+// odd row first, so each of the two LDS writes covers all 16 bank groups.
+// op(B) = B^T is loaded one element per k (four 8-byte loads, the 16 lanes of
+// a quarter reading 128 contiguous bytes of each).
 #pragma once
 #include "dq_common.hpp"
 // included by dq_gemm.hpp after GemmArgs
@@ -72,14 +76,16 @@ __global__ __launch_bounds__(256, 3) void k_gemm_w(GemmArgs<double> g, int tiles
     }
     const int q = lane >> 4, l16 = lane & 15;
 
-    // A loads: rows 16 (TB wave + t) + l16, k = 4q .. 4q + 3
+    // A loads: op(A) = A^T rows 16 (TB wave + t) + l16, k = 4q .. 4q + 3;
+    // op(A) = A rows 16 (TB wave + t) + 2p, +1 of columns k = 2kp, +1
+    const int p2 = lane & 7, kp = lane >> 3;
     const double* Ag = TA ? A + 4 * q + (long)g.lda * (m0 + 16 * TB * wave + l16)
-                          : A + (m0 + 16 * TB * wave + l16) + (long)g.lda * (4 * q);
+                          : A + (m0 + 16 * TB * wave + 2 * p2) + (long)g.lda * (2 * kp);
     // B loads: column 16 wave + l16 of op(B), k = 4q .. 4q + 3
     const double* Bg = TBT ? B + (n0 + 16 * wave + l16) + (long)g.ldb * (4 * q)
                            : B + 4 * q + (long)g.ldb * (n0 + 16 * wave + l16);
 
-    double ra[TB][4];
+    double ra[TB][4];  // op(A) = A^T: k = 4q .. 4q + 3; op(A) = A: [col][row] of the 2 x 2 block
     double2 rb[2];
     double2 ub0 = make_double2(1.0, 1.0), ub1 = ub0;
     auto gload = [&](int k0) {
@@ -90,8 +96,9 @@ __global__ __launch_bounds__(256, 3) void k_gemm_w(GemmArgs<double> g, int tiles
                 const double2 a0 = *(const double2*)(aa), a1 = *(const double2*)(aa + 2);
                 ra[t][0] = a0.x; ra[t][1] = a0.y; ra[t][2] = a1.x; ra[t][3] = a1.y;
             } else {
-#pragma unroll
-                for (int c = 0; c < 4; c++) ra[t][c] = Ag[16 * t + (long)g.lda * (k0 + c)];
+                const double* aa = Ag + 16 * t + (long)g.lda * k0;
+                const double2 a0 = *(const double2*)(aa), a1 = *(const double2*)(aa + g.lda);
+                ra[t][0] = a0.x; ra[t][1] = a0.y; ra[t][2] = a1.x; ra[t][3] = a1.y;
             }
         }
         if constexpr (TBT) {
@@ -112,11 +119,23 @@ __global__ __launch_bounds__(256, 3) void k_gemm_w(GemmArgs<double> g, int tiles
     };
     // slot `lane` of region j of block blk: double offset ((2 blk + j) 64 + lane) 2
     auto slot = [&](int blk, int j) { return ((2 * blk + j) * 64 + lane) * 2; };
+    const bool odd = kp & 1;
     auto sstore = [&](int buf) {
 #pragma unroll
         for (int t = 0; t < TB; t++) {
-            *(double2*)(&As[buf][slot(TB * wave + t, 0)]) = make_double2(ra[t][0], ra[t][1]);
-            *(double2*)(&As[buf][slot(TB * wave + t, 1)]) = make_double2(ra[t][2], ra[t][3]);
+            if constexpr (TA) {
+                *(double2*)(&As[buf][slot(TB * wave + t, 0)]) = make_double2(ra[t][0], ra[t][1]);
+                *(double2*)(&As[buf][slot(TB * wave + t, 1)]) = make_double2(ra[t][2], ra[t][3]);
+            } else {
+                // row 2 p2 + e, k = 2 kp + c: region kp & 1, slot 2 p2 + e + 16 (kp >> 1)
+                double* base = &As[buf][(((2 * (TB * wave + t) + (kp & 1)) * 64) + 16 * (kp >> 1) +
+                                         2 * p2) * 2];
+                // ra[t][2 c + e]: column k + c, row 2 p2 + e; first row 2 p2 + odd
+                const double f0 = odd ? ra[t][1] : ra[t][0], f1 = odd ? ra[t][3] : ra[t][2];
+                const double s0 = odd ? ra[t][0] : ra[t][1], s1 = odd ? ra[t][2] : ra[t][3];
+                *(double2*)(base + 2 * odd) = make_double2(f0, f1);
+                *(double2*)(base + 2 - 2 * odd) = make_double2(s0, s1);
+            }
         }
         *(double2*)(&Bs[buf][slot(wave, 0)]) = make_double2(rb[0].x * ub0.x, rb[0].y * ub0.y);
         *(double2*)(&Bs[buf][slot(wave, 1)]) = make_double2(rb[1].x * ub1.x, rb[1].y * ub1.y);
