@@ -28,8 +28,7 @@
 // of a column pair reading 128 contiguous bytes), which is exactly the
 // (k, k + 1) pair of two row slots; the lanes whose k pair is odd write their
 // odd row first, so each of the two LDS writes covers all 16 bank groups.
-// op(B) = B^T is loaded one element per k (four 8-byte loads, the 16 lanes of
-// a quarter reading 128 contiguous bytes of each).
+// op(B) = B^T (n-contiguous) moves the same way.
 #pragma once
 #include "dq_common.hpp"
 // included by dq_gemm.hpp after GemmArgs
@@ -81,8 +80,9 @@ __global__ __launch_bounds__(256, 3) void k_gemm_w(GemmArgs<double> g, int tiles
     const int p2 = lane & 7, kp = lane >> 3;
     const double* Ag = TA ? A + 4 * q + (long)g.lda * (m0 + 16 * TB * wave + l16)
                           : A + (m0 + 16 * TB * wave + 2 * p2) + (long)g.lda * (2 * kp);
-    // B loads: column 16 wave + l16 of op(B), k = 4q .. 4q + 3
-    const double* Bg = TBT ? B + (n0 + 16 * wave + l16) + (long)g.ldb * (4 * q)
+    // B loads: op(B) = B column 16 wave + l16, k = 4q .. 4q + 3; op(B) = B^T
+    // columns 16 wave + 2p, +1 of k = 2kp, +1
+    const double* Bg = TBT ? B + (n0 + 16 * wave + 2 * p2) + (long)g.ldb * (2 * kp)
                            : B + 4 * q + (long)g.ldb * (n0 + 16 * wave + l16);
 
     double ra[TB][4];  // op(A) = A^T: k = 4q .. 4q + 3; op(A) = A: [col][row] of the 2 x 2 block
@@ -101,20 +101,19 @@ __global__ __launch_bounds__(256, 3) void k_gemm_w(GemmArgs<double> g, int tiles
                 ra[t][0] = a0.x; ra[t][1] = a0.y; ra[t][2] = a1.x; ra[t][3] = a1.y;
             }
         }
-        if constexpr (TBT) {
-            double x[4];
-#pragma unroll
-            for (int c = 0; c < 4; c++) x[c] = Bg[(long)g.ldb * (k0 + c)];
-            rb[0] = make_double2(x[0], x[1]);
-            rb[1] = make_double2(x[2], x[3]);
+        if constexpr (TBT) {  // rb[c]: k = 2kp + c of columns 2p, 2p + 1
+            const double* bb = Bg + (long)g.ldb * k0;
+            rb[0] = *(const double2*)(bb);
+            rb[1] = *(const double2*)(bb + g.ldb);
+            if (u) ub0 = *(const double2*)(u + k0 + 2 * kp);
         } else {
             const double* bb = Bg + k0;
             rb[0] = *(const double2*)(bb);
             rb[1] = *(const double2*)(bb + 2);
-        }
-        if (u) {
-            ub0 = *(const double2*)(u + k0 + 4 * q);
-            ub1 = *(const double2*)(u + k0 + 4 * q + 2);
+            if (u) {
+                ub0 = *(const double2*)(u + k0 + 4 * q);
+                ub1 = *(const double2*)(u + k0 + 4 * q + 2);
+            }
         }
     };
     // slot `lane` of region j of block blk: double offset ((2 blk + j) 64 + lane) 2
@@ -137,8 +136,16 @@ __global__ __launch_bounds__(256, 3) void k_gemm_w(GemmArgs<double> g, int tiles
                 *(double2*)(base + 2 - 2 * odd) = make_double2(s0, s1);
             }
         }
-        *(double2*)(&Bs[buf][slot(wave, 0)]) = make_double2(rb[0].x * ub0.x, rb[0].y * ub0.y);
-        *(double2*)(&Bs[buf][slot(wave, 1)]) = make_double2(rb[1].x * ub1.x, rb[1].y * ub1.y);
+        if constexpr (TBT) {  // as op(A) = A, the pair k = 2kp, +1 scaled by u
+            double* base = &Bs[buf][(((2 * wave + (kp & 1)) * 64) + 16 * (kp >> 1) + 2 * p2) * 2];
+            const double f0 = odd ? rb[0].y : rb[0].x, f1 = odd ? rb[1].y : rb[1].x;
+            const double s0 = odd ? rb[0].x : rb[0].y, s1 = odd ? rb[1].x : rb[1].y;
+            *(double2*)(base + 2 * odd) = make_double2(f0 * ub0.x, f1 * ub0.y);
+            *(double2*)(base + 2 - 2 * odd) = make_double2(s0 * ub0.x, s1 * ub0.y);
+        } else {
+            *(double2*)(&Bs[buf][slot(wave, 0)]) = make_double2(rb[0].x * ub0.x, rb[0].y * ub0.y);
+            *(double2*)(&Bs[buf][slot(wave, 1)]) = make_double2(rb[1].x * ub1.x, rb[1].y * ub1.y);
+        }
     };
 
     const int wr = wave >> 1, wc = wave & 1;
