@@ -26,8 +26,9 @@
 // B here).  For op(A) = A the rows are m-contiguous: a thread loads a 2 x 2
 // block (rows 2p, 2p + 1 of columns k, k + 1: two 16-byte loads, the 8 lanes
 // of a column pair reading 128 contiguous bytes), which is exactly the
-// (k, k + 1) pair of two row slots; the lanes whose k pair is odd write their
-// odd row first, so each of the two LDS writes covers all 16 bank groups.
+// (k, k + 1) pair of two row slots; lanes are numbered so that the 16-byte
+// slots of every 8 consecutive lanes of each of the two LDS writes fall in 8
+// consecutive bank groups (no bank conflicts by the counters).
 // op(B) = B^T (n-contiguous) moves the same way.
 #pragma once
 #include "dq_common.hpp"
@@ -77,7 +78,7 @@ __global__ __launch_bounds__(256, 3) void k_gemm_w(GemmArgs<double> g, int tiles
 
     // A loads: op(A) = A^T rows 16 (TB wave + t) + l16, k = 4q .. 4q + 3;
     // op(A) = A rows 16 (TB wave + t) + 2p, +1 of columns k = 2kp, +1
-    const int p2 = lane & 7, kp = lane >> 3;
+    const int p2 = (lane >> 1) & 7, kp = (lane & 1) | ((lane >> 4) << 1);
     const double* Ag = TA ? A + 4 * q + (long)g.lda * (m0 + 16 * TB * wave + l16)
                           : A + (m0 + 16 * TB * wave + 2 * p2) + (long)g.lda * (2 * kp);
     // B loads: op(B) = B column 16 wave + l16, k = 4q .. 4q + 3; op(B) = B^T
@@ -156,14 +157,11 @@ __global__ __launch_bounds__(256, 3) void k_gemm_w(GemmArgs<double> g, int tiles
 #pragma unroll
         for (int j = 0; j < 2; j++) acc[i][j] = (dbl4){0.0, 0.0, 0.0, 0.0};
     double2 av[MI][2], bv[2][2];
-    auto reads = [&](int buf) {
+    auto reads = [&](int buf, int jj) {  // region jj: sub-steps 2 jj, 2 jj + 1
 #pragma unroll
-        for (int jj = 0; jj < 2; jj++) {
+        for (int i = 0; i < MI; i++) av[i][jj] = *(const double2*)(&As[buf][slot(MI * wr + i, jj)]);
 #pragma unroll
-            for (int i = 0; i < MI; i++) av[i][jj] = *(const double2*)(&As[buf][slot(MI * wr + i, jj)]);
-#pragma unroll
-            for (int j = 0; j < 2; j++) bv[j][jj] = *(const double2*)(&Bs[buf][slot(2 * wc + j, jj)]);
-        }
+        for (int j = 0; j < 2; j++) bv[j][jj] = *(const double2*)(&Bs[buf][slot(2 * wc + j, jj)]);
     };
     auto mf = [&](int s) {  // MFMA sub-step s: k = 4q + s
 #pragma unroll
@@ -187,8 +185,14 @@ __global__ __launch_bounds__(256, 3) void k_gemm_w(GemmArgs<double> g, int tiles
         const int cur = kt & 1;
         const bool more = kt + 1 < nk;
         if (more) gload((kt + 1) * GW_BK);
-        reads(cur);
+        // the second region's reads go out after the first sub-step's MFMAs
+        // (twelve reads at once stall the wave's issue) and land during the
+        // second sub-step's
+        reads(cur, 0);
         mf(0);
+        __builtin_amdgcn_sched_barrier(0);
+        reads(cur, 1);
+        __builtin_amdgcn_sched_barrier(0);
         mf(1);
         // the next step's LDS writes between the MFMA halves (2% fewer cycles
         // than after them: the loads have landed and the LDS is idle)
