@@ -39,9 +39,12 @@
 
 // MI: 16-row blocks per wave along M (BM = 32 MI); TA: op(A) = A^T;
 // TBT: op(B) = B^T
+// (two workgroups per CU for the 128-row beta products: their C tile
+// prefetch needs 240 registers)
 template <int MI, bool TA, bool BETA, bool TBT = false>
-__global__ __launch_bounds__(256, 3) void k_gemm_w(GemmArgs<double> g, int tiles_m, int tiles_n,
-                                                   int nwg)
+__global__ __launch_bounds__(256, (BETA && MI == 4) ? 2 : 3) void k_gemm_w(GemmArgs<double> g,
+                                                                          int tiles_m, int tiles_n,
+                                                                          int nwg)
 {
     constexpr int BM = 32 * MI, TB = MI / 2;  // TB: A blocks loaded per wave
     __shared__ __align__(16) double As[2][BM * GW_BK];
@@ -156,6 +159,20 @@ __global__ __launch_bounds__(256, 3) void k_gemm_w(GemmArgs<double> g, int tiles
     for (int i = 0; i < MI; i++)
 #pragma unroll
         for (int j = 0; j < 2; j++) acc[i][j] = (dbl4){0.0, 0.0, 0.0, 0.0};
+    // beta != 0: the C tile is requested before the K loop, so that its
+    // latency hides under the MFMAs (the beta products here are the QR
+    // chain's short-K updates, whose epilogue read was exposed)
+    dbl4 cpre[BETA ? MI : 1][2];
+    if constexpr (BETA) {
+#pragma unroll
+        for (int i = 0; i < MI; i++)
+#pragma unroll
+            for (int j = 0; j < 2; j++)
+#pragma unroll
+                for (int e = 0; e < 4; e++)
+                    cpre[i][j][e] = C[(m0 + wm + 16 * i + l16) +
+                                      (long)g.ldc * (n0 + wn + 16 * j + 4 * e + q)];
+    }
     double2 av[MI][2], bv[2][2];
     auto reads = [&](int buf, int jj) {  // region jj: sub-steps 2 jj, 2 jj + 1
 #pragma unroll
@@ -217,7 +234,7 @@ __global__ __launch_bounds__(256, 3) void k_gemm_w(GemmArgs<double> g, int tiles
                 if (rr) v = v * rs;
                 if (ww) v = v * ww[col];
                 const long o = row + (long)g.ldc * col;
-                if constexpr (BETA) v += g.beta * C[o];
+                if constexpr (BETA) v += g.beta * cpre[i][j][e];
                 C[o] = v;
             }
     }

@@ -421,7 +421,7 @@ static int ue_measure(dqmc_engine* e)
     const int L = e->L, nc = e->nc;
     if (e->ij_percol) {
         const int thr = (e->num_ij + 63) / 64 * 64;
-        k_meas_ue_strip<T, is_cplx<T>::value ? 8 : 16><<<dim3(L, nc), thr, 0, e->st>>>(ma);
+        k_meas_ue_strip<T, 8><<<dim3(L, nc), thr, 0, e->st>>>(ma);
     } else {
         k_meas_ue<T><<<dim3((e->num_ij + 255) / 256, L, nc), 256, 0, e->st>>>(ma);
     }
