@@ -57,10 +57,14 @@ __device__ inline void larfg_sel(zd alpha, double xnorm2, zd& beta, zd& tau, zd&
 // squared norms over rows > j exactly; the dots v^H x and the norms each take
 // one batched register reduction.  The best live column (largest norm, first
 // on ties) is copied to the candidate buffer cw.
+// Every row slot takes part: v is zero above row j, so those rows add exact
+// zeros to the dots and stay unchanged by the update, and the row mask mk
+// (1 below row j, else 0) keeps them out of the norms.  (Skipping the slots
+// above j by a test per slot was compiled to selects on every value.)
 template <typename T, int RPL, int CB>
-__device__ __forceinline__ void qrp_apply_batch(T (&xb)[CB][RPL], unsigned lv, int q0, int j,
-                                                int lane, const T (&v)[RPL], T ct, int t0,
-                                                double rm0, double& bestv, int& bestq, T* cw)
+__device__ __forceinline__ void qrp_apply_batch(T (&xb)[CB][RPL], unsigned lv, int q0, int lane,
+                                                const T (&v)[RPL], T ct, const double (&mk)[RPL],
+                                                double& bestv, int& bestq, T* cw)
 {
     constexpr int NP = val_parts<T>::n;
     const T zero = from_re<T>(0.0);
@@ -69,14 +73,12 @@ __device__ __forceinline__ void qrp_apply_batch(T (&xb)[CB][RPL], unsigned lv, i
     for (int u = 0; u < CB * NP; u++) wr[u] = 0.0;
 #pragma unroll
     for (int t = 0; t < RPL; t++) {
-        if (64 * t + 63 >= j) {
 #pragma unroll
-            for (int u = 0; u < CB; u++) {
-                T d;
-                get_part(&wr[u * NP], d);
-                d += cj(v[t]) * xb[u][t];
-                put_part(&wr[u * NP], d);
-            }
+        for (int u = 0; u < CB; u++) {
+            T d;
+            get_part(&wr[u * NP], d);
+            d += cj(v[t]) * xb[u][t];
+            put_part(&wr[u * NP], d);
         }
     }
     wave_sum_many<CB * NP>(wr, lane);
@@ -93,13 +95,10 @@ __device__ __forceinline__ void qrp_apply_batch(T (&xb)[CB][RPL], unsigned lv, i
     for (int u = 0; u < CB; u++) sn[u] = 0.0;
 #pragma unroll
     for (int t = 0; t < RPL; t++) {
-        if (64 * t + 63 >= j) {
-            const double mk = (t == t0) ? rm0 : 1.0;
 #pragma unroll
-            for (int u = 0; u < CB; u++) {
-                xb[u][t] -= v[t] * f[u];
-                sn[u] += mk * abs2(xb[u][t]);
-            }
+        for (int u = 0; u < CB; u++) {
+            xb[u][t] -= v[t] * f[u];
+            sn[u] += mk[t] * abs2(xb[u][t]);
         }
     }
     wave_sum_many<CB>(sn, lane);
@@ -257,8 +256,9 @@ __global__ __launch_bounds__(QRP_THREADS) void k_qrp_res(int m, int n, T* A_, in
         // (3) apply H^H to the live columns, recompute their norms, and keep
         //     the best one as this wave's next candidate
         const T ct = cj(tj);
-        const int t0 = j >> 6;
-        const double rm0 = (lane > (j & 63)) ? 1.0 : 0.0;   // rows > j in row slot t0
+        double mk[RPL];  // rows > j
+#pragma unroll
+        for (int t = 0; t < RPL; t++) mk[t] = (lane + 64 * t > j) ? 1.0 : 0.0;
         double bestv = -1.0;
         int bestq = -1;
         for (int s0 = 0; s0 < SL; s0 += 2) {
@@ -270,8 +270,8 @@ __global__ __launch_bounds__(QRP_THREADS) void k_qrp_res(int m, int n, T* A_, in
 #pragma unroll
                 for (int t = 0; t < RPL; t++)
                     xb[u][t] = (s0 + u < SL) ? Ls[wave][s0 + u][lane + 64 * t] : zero;
-            qrp_apply_batch<T, RPL, 2>(xb, lv, wave + QRP_WAVES * s0, j, lane, v, ct, t0, rm0,
-                                       bestv, bestq, Cw[mybuf]);
+            qrp_apply_batch<T, RPL, 2>(xb, lv, wave + QRP_WAVES * s0, lane, v, ct, mk, bestv,
+                                       bestq, Cw[mybuf]);
 #pragma unroll
             for (int u = 0; u < 2; u++)
                 if (s0 + u < SL) {
@@ -284,8 +284,8 @@ __global__ __launch_bounds__(QRP_THREADS) void k_qrp_res(int m, int n, T* A_, in
             const unsigned lv = (unsigned)(live >> (SL + c0)) & ((1u << CB) - 1);
             if (lv == 0) continue;  // whole batch retired
             qrp_apply_batch<T, RPL, CB>(*reinterpret_cast<T(*)[CB][RPL]>(&x[c0][0]), lv,
-                                        wave + QRP_WAVES * (SL + c0), j, lane, v, ct, t0, rm0,
-                                        bestv, bestq, Cw[mybuf]);
+                                        wave + QRP_WAVES * (SL + c0), lane, v, ct, mk, bestv,
+                                        bestq, Cw[mybuf]);
         }
         if (lane == 0) {
             candv[cb ^ 1][wave] = bestv;
