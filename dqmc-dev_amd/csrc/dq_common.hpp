@@ -131,12 +131,15 @@ __device__ inline void swap16_d(double& x, double& y)
     x = __longlong_as_double(((long long)hi[0] << 32) | lo[0]);
     y = __longlong_as_double(((long long)hi[1] << 32) | lo[1]);
 }
+// The controls used here (row rotations and mirrors, quad permutations) read a
+// valid lane for every lane, so the old-value operand is never taken: the
+// plain DPP move needs no v_mov of a fill value before it.
 template <int CTRL>
 __device__ inline double dpp_d(double v)
 {
     const long long u = __double_as_longlong(v);
-    const int lo = __builtin_amdgcn_update_dpp(0, (int)u, CTRL, 0xf, 0xf, false);
-    const int hi = __builtin_amdgcn_update_dpp(0, (int)(u >> 32), CTRL, 0xf, 0xf, false);
+    const int lo = __builtin_amdgcn_mov_dpp((int)u, CTRL, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(u >> 32), CTRL, 0xf, 0xf, false);
     return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
 }
 #define DPP_ROR(n) (0x120 + (n))
