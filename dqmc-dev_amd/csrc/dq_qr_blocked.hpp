@@ -29,8 +29,14 @@
 //       v_j is zero outside rows j .. mp-1, so the row loops run unmasked;
 //   (a) the owner of column j + 1, right after its own update of that column,
 //       builds reflector j + 1 into the other v buffer;
-//   (c) the owner of column j - 1 finishes T(0:j-1, j-1) from the dots of
-//       step j - 1 (double-buffered), off the critical path.
+//   the dots V(:,jc)^H v_j (jc < j) go into the strict upper triangle of the
+//   T buffer, tau_j onto its diagonal.
+// After the last step T comes from these in blocks of 8 columns: the diagonal
+// blocks by the recurrence (one wave per block, in parallel), then block
+// column B from the finished T(0:8B, 0:8B) as T12 = -T1 (V1^H V2) T2 (the
+// xlarft merge).  Building T(0:j-1, j-1) inside the steps put one wave's
+// serial sum of up to 64 LDS reads on every step's critical path (176 against
+// 124 us per panel without it).
 // f(integral_constant<int, i>) for i = 0 .. K-1, unrolled at compile time
 template <int K, int I = 0, typename F>
 __device__ __forceinline__ void static_for(F&& f)
@@ -51,7 +57,6 @@ __global__ __launch_bounds__(64 * NW, 4) void k_qr_panel(int m, int k0, int nbp,
     constexpr int NP = val_parts<T>::n;
     __shared__ T v[2][64 * QM];
     __shared__ T Ts[NB * NB];
-    __shared__ T z[2][NB];
     __shared__ T s_det;
     const int b = blockIdx.x;
     T* A = A_ + b * sA + k0 + (long)lda * k0;  // panel origin
@@ -105,22 +110,6 @@ __global__ __launch_bounds__(64 * NW, 4) void k_qr_panel(int m, int k0, int nbp,
             s_det = s_det * (one - t * vv);
         }
     };
-    // (c) T(0:jp, jp) = -tau_jp T(0:jp, 0:jp) z(0:jp), z = V(:,0:jp)^H v_jp.
-    // T is upper triangular (zeros below the diagonal), so the sum may start
-    // at 0; groups of 8 independent LDS reads are issued together.
-    auto t_column = [&](int jp) __attribute__((always_inline)) {
-        if (wave == jp % NW && lane < jp) {
-            const T* zz = z[jp & 1];
-            T s = zero;
-            for (int a0 = 0; a0 < jp; a0 += 8) {
-#pragma unroll
-                for (int u = 0; u < 8; u++)
-                    if (a0 + u < jp) s += Ts[lane + NB * (a0 + u)] * zz[a0 + u];
-            }
-            Ts[lane + NB * jp] = -(Ts[jp + NB * jp] * s);
-        }
-    };
-
     if (nbp > 0 && wave == 0) reflector(std::integral_constant<int, 0>(), 0);
     __syncthreads();
     // the register slot of the step's columns is a compile-time constant
@@ -135,7 +124,6 @@ __global__ __launch_bounds__(64 * NW, 4) void k_qr_panel(int m, int k0, int nbp,
             const int nxt = (j + 1) % NW;
             const bool crit = j + 1 < nbp && wave == nxt;
             if (crit) __builtin_amdgcn_s_setprio(2);
-            if (j > 0) t_column(j - 1);
             // (b) d_jc = v_j^H x_jc for every column of the wave, summed over
             //     the wave in one register reduction
             const T* vb = v[j & 1];
@@ -159,7 +147,7 @@ __global__ __launch_bounds__(64 * NW, 4) void k_qr_panel(int m, int k0, int nbp,
 #pragma unroll
                     for (int q = 0; q < QM; q++) x[q][c2] -= vb[lane + 64 * q] * f;
                 } else if (jc < j && lane == 0) {
-                    z[j & 1][jc] = cj(d);
+                    Ts[jc + NB * j] = cj(d);  // V(:,jc)^H v_j
                 }
             }
             // (a) of step j + 1 by the owner of column j + 1
@@ -174,8 +162,43 @@ __global__ __launch_bounds__(64 * NW, 4) void k_qr_panel(int m, int k0, int nbp,
             __syncthreads();
         }
     });
-    if (nbp > 0) t_column(nbp - 1);
     __syncthreads();
+    // T from S = strict upper V^H V (in Ts) and tau (its diagonal).  Columns
+    // past nbp have tau = 0 and S = 0, so their T columns come out zero.
+    constexpr int NBK = NB / 8;
+    static_assert(NB % 8 == 0 && NBK <= NW, "one wave per diagonal block");
+    // diagonal blocks: T(i, jj) = -tau_jj sum_{a = i}^{jj-1} T(i, a) S(a, jj)
+    if (wave < NBK && lane < 8) {
+        const int o = 8 * wave, i = o + lane;
+        for (int jj = 1; jj < 8; jj++) {
+            const int jc = o + jj;
+            T sum = zero;
+            for (int a = i; a < jc; a++) sum += Ts[i + NB * a] * Ts[a + NB * jc];
+            if (i < jc) Ts[i + NB * jc] = -(Ts[jc + NB * jc] * sum);
+        }
+    }
+    __syncthreads();
+    // block column B: W = T(0:8B, 0:8B) S(0:8B, cols), then T(0:8B, cols) =
+    // -W T_BB, W passing through the (now free) v buffer
+    T* Wb = &v[0][0];
+    for (int B = 1; B < NBK; B++) {
+        const int rows = 8 * B, c0 = 8 * B;
+        const int i = tid % rows, cc = tid / rows;
+        const bool act = tid < 8 * rows;
+        if (act) {
+            T w = zero;
+            for (int a = i; a < rows; a++) w += Ts[i + NB * a] * Ts[a + NB * (c0 + cc)];
+            Wb[i + rows * cc] = w;
+        }
+        __syncthreads();
+        if (act) {
+            T t = zero;
+            for (int c2 = 0; c2 <= cc; c2++)
+                t += Wb[i + rows * c2] * Ts[(c0 + c2) + NB * (c0 + cc)];
+            Ts[i + NB * (c0 + cc)] = -t;
+        }
+        __syncthreads();
+    }
     // write back: R (upper) and the reflectors (below) into A, explicit V, T, tau
     T* V = V_ + b * sV + k0 + (long)ldv * k0;
     // the row index passes through an empty asm statement, so the write
