@@ -619,7 +619,7 @@ static int q_from_reflectors(dqmc_engine* e, int n, const T* tau, T* Q, long sQ,
         RC(gemm<T>(e, OP_C, OP_N, nbp, nbp, n - k0, one, Vk, n, sV, Vk, n, sV, zero,
                    Tb + (long)p * NB * NB, NB, sT, batch));
     }
-    k_larft_gram<T><<<dim3(np, batch), 64, 0, e->st>>>(NB, n, tau, n, Tb, sT);
+    k_larft_gram<T><<<dim3(np, batch), 512, 0, e->st>>>(NB, n, tau, n, Tb, sT);
     LCHK();
     return accumulate_q<T>(e, n, n, NB, Q, n, sQ, batch);
 }
