@@ -276,21 +276,24 @@ __global__ __launch_bounds__(256) void k_eqg_diag(int n, const T* R_, long sR, d
 
 // After the pivoted QR of the first product (src/greens.c:91-101), with d
 // from k_eqg_diag: T[:, jpvt_j - 1] = diag(1/d) R[:, j] (upper part).
+// Grid (row blocks, column groups of EQG_COLS, batch).
+#define EQG_COLS 16
 template <typename T>
 __global__ void k_eqg_init_T(int n, const T* R_, long sR, const int* jpvt_, long spiv,
                              const double* d_, long sd, T* Tm_, long sT)
 {
-    const int b = blockIdx.y;
+    const int b = blockIdx.z;
     const T* R = R_ + b * sR;
     const int* jp = jpvt_ + b * spiv;
     const double* d = d_ + b * sd;
     T* Tm = Tm_ + b * sT;
-    const int j = blockIdx.x;  // one block per column
-    const int c = jp[j] - 1;
-    for (int i = threadIdx.x; i < n; i += blockDim.x) {
-        const double vi = 1.0 / d[i];
-        Tm[i + (long)n * c] = (i <= j) ? vi * R[i + (long)n * j] : from_re<T>(0.0);
-    }
+    // thread: row i of EQG_COLS columns j (1 / d_i formed once per row)
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double vi = 1.0 / d[i];
+    const int j0 = blockIdx.y * EQG_COLS, j1 = min(j0 + EQG_COLS, n);
+    for (int j = j0; j < j1; j++)
+        Tm[i + (long)n * (jp[j] - 1)] = (i <= j) ? vi * R[i + (long)n * j] : from_re<T>(0.0);
 }
 
 // Pre-pivoting (src/greens.c:116-132): squared column norms of Y, stable
@@ -326,9 +329,10 @@ __global__ __launch_bounds__(1024) void k_prepivot(int n, const T* Y_, long sY, 
     __syncthreads();
     int* pg = perm_ + b * sp;
     for (int j = tid; j < n; j += blockDim.x) pg[j] = perm[j];
-    for (long e = tid; e < (long)n * n; e += blockDim.x) {
-        const int i = e % n, c = e / n;
-        Z[e] = Y[i + (long)n * perm[c]];
+    // column by column: no 64-bit division per element
+    for (int c = wave; c < n; c += nw) {
+        const T* y = Y + (long)n * perm[c];
+        for (int i = lane; i < n; i += 64) Z[i + (long)n * c] = y[i];
     }
 }
 
@@ -338,18 +342,22 @@ template <typename T>
 __global__ void k_eqg_post_qr(int n, const T* R_, long sR, const int* perm_, long sp, const T* Tm_,
                               long sT, const double* d_, long sd, T* Rs_, T* Tp_, long sW)
 {
-    const int b = blockIdx.y;
+    const int b = blockIdx.z;
     const T* R = R_ + b * sR;
     const int* perm = perm_ + b * sp;
     const T* Tm = Tm_ + b * sT;
     const double* d = d_ + b * sd;
     T* Rs = Rs_ + b * sW;
     T* Tp = Tp_ + b * sW;
-    const int c = blockIdx.x;
-    for (int i = threadIdx.x; i < n; i += blockDim.x) {
-        const double vi = 1.0 / d[i];
+    // thread: row i of EQG_COLS columns c (1 / d_i and perm_i once per row)
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double vi = 1.0 / d[i];
+    const int pi = perm[i];
+    const int c0 = blockIdx.y * EQG_COLS, c1 = min(c0 + EQG_COLS, n);
+    for (int c = c0; c < c1; c++) {
         Rs[i + (long)n * c] = (i <= c) ? vi * R[i + (long)n * c] : from_re<T>(0.0);
-        Tp[i + (long)n * c] = Tm[perm[i] + (long)n * c];
+        Tp[i + (long)n * c] = Tm[pi + (long)n * c];
     }
 }
 

@@ -716,7 +716,8 @@ static int calc_eq_g(dqmc_engine* e, int l, T* Gout, T* phase_b)
     }
     k_eqg_diag<T><<<dim3((N + 255) / 256, B), 256, 0, e->st>>>(N, X, NN, e->dvec, N);
     LCHK();
-    k_eqg_init_T<T><<<dim3(N, B), 256, 0, e->st>>>(N, X, NN, e->jpvt, N, e->dvec, N, Tm, NN);
+    k_eqg_init_T<T><<<dim3((N + 255) / 256, (N + EQG_COLS - 1) / EQG_COLS, B), 256, 0, e->st>>>(
+        N, X, NN, e->jpvt, N, e->dvec, N, Tm, NN);
     LCHK();
     while (m != l) {
         const int next = (m + n_mul) % F;
@@ -732,8 +733,8 @@ static int calc_eq_g(dqmc_engine* e, int l, T* Gout, T* phase_b)
         // (3d) T = diag(1/d) R * P^T T   (xtrmm L,U)
         k_eqg_diag<T><<<dim3((N + 255) / 256, B), 256, 0, e->st>>>(N, Z, NN, e->dvec, N);
         LCHK();
-        k_eqg_post_qr<T><<<dim3(N, B), 256, 0, e->st>>>(N, Z, NN, e->perm, N, Tm, NN, e->dvec, N,
-                                                       Rs, Tp, NN);
+        k_eqg_post_qr<T><<<dim3((N + 255) / 256, (N + EQG_COLS - 1) / EQG_COLS, B), 256, 0,
+                           e->st>>>(N, Z, NN, e->perm, N, Tm, NN, e->dvec, N, Rs, Tp, NN);
         LCHK();
         // Rs is upper triangular: its product skips the zero blocks
         RC(gemm<T>(e, OP_N, OP_N, N, N, N, one, Rs, N, NN, Tp, N, NN, zero, Tm, N, NN, B, nullptr,
