@@ -167,8 +167,9 @@ __global__ __launch_bounds__(64 * NW, 4) void k_qr_panel(int m, int k0, int nbp,
     // past nbp have tau = 0 and S = 0, so their T columns come out zero.
     constexpr int NBK = NB / 8;
     static_assert(NB % 8 == 0 && NBK <= NW, "one wave per diagonal block");
+    const int nbk = (nbp + 7) / 8;  // blocks past the panel's columns stay zero
     // diagonal blocks: T(i, jj) = -tau_jj sum_{a = i}^{jj-1} T(i, a) S(a, jj)
-    if (wave < NBK && lane < 8) {
+    if (wave < nbk && lane < 8) {
         const int o = 8 * wave, i = o + lane;
         for (int jj = 1; jj < 8; jj++) {
             const int jc = o + jj;
@@ -181,7 +182,7 @@ __global__ __launch_bounds__(64 * NW, 4) void k_qr_panel(int m, int k0, int nbp,
     // block column B: W = T(0:8B, 0:8B) S(0:8B, cols), then T(0:8B, cols) =
     // -W T_BB, W passing through the (now free) v buffer
     T* Wb = &v[0][0];
-    for (int B = 1; B < NBK; B++) {
+    for (int B = 1; B < nbk; B++) {
         const int rows = 8 * B, c0 = 8 * B;
         const int i = tid % rows, cc = tid / rows;
         const bool act = tid < 8 * rows;

@@ -327,7 +327,8 @@ __global__ __launch_bounds__(512) void k_larft_gram(int NB, int kmax, const T* t
         S[e] = v;
     }
     __syncthreads();
-    if (lane < 8) {  // diagonal block `wave`
+    const int nbk = (nbp + 7) / 8;  // blocks past the panel's columns stay zero
+    if (wave < nbk && lane < 8) {  // diagonal block `wave`
         const int o = 8 * wave, i = o + lane;
         for (int jj = 1; jj < 8; jj++) {
             const int jc = o + jj;
@@ -337,7 +338,7 @@ __global__ __launch_bounds__(512) void k_larft_gram(int NB, int kmax, const T* t
         }
     }
     __syncthreads();
-    for (int B = 1; B < 8; B++) {
+    for (int B = 1; B < nbk; B++) {
         const int rows = 8 * B, c0 = 8 * B;
         const int i = tid % rows, cc = tid / rows;
         const bool act = tid < 8 * rows;
